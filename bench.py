@@ -1,0 +1,179 @@
+"""Benchmark of the D3Q19 lattice-Boltzmann step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Default workload: lid-driven cavity, D3Q19 BGK, dense 256^3 grid (254^3 fluid
+cells), FP32 storage and FP32 collision, Re = 1000 (nu = U N / Re, U = 0.05),
+W = 100 warm-up steps then K = 500 timed steps.  A step is one launch of the
+fused pull-stream + bounce-back + BGK kernel over the whole grid; inputs are
+resident in HBM when the timed region starts.
+
+Prints ONE JSON line: MLUPS over fluid cells (value), the roofline of the step
+kernel (algorithmic bytes 2*19*sizeof(storage) per fluid cell / average launch
+time from HIP events on the launch stream, vs 8 TB/s), the velocity error
+against a binary64 GPU run of the same case, and the CPU baseline (the CPU
+oracle, all host threads, on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "D3Q19 MLUPS & %HBM roofline; time-to-steady warm-start vs plain LBM"
+HBM_PEAK_GBS = 8000.0
+STORAGE_BYTES = {"f64": 8, "f32": 4, "f32dev": 4, "f16dev": 2}
+
+WORKLOADS = {
+    # name: (N, storage, compute, Re, default steps, default warmup)
+    "cavity256_f32": (256, "f32", "f32", 1000.0, 500, 100),
+    "cavity256_f16dev": (256, "f16dev", "f32", 1000.0, 500, 100),
+    "cavity256_f32dev": (256, "f32dev", "f32", 1000.0, 500, 100),
+    "cavity256_f64": (256, "f64", "f64", 1000.0, 200, 20),
+    "cavity64_f64": (64, "f64", "f64", 100.0, 1000, 0),
+}
+
+
+def traffic_from_profiles(kernel_tag: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by a copy kernel of known size), if committed."""
+    p = ROOT / "profiles" / f"traffic_{kernel_tag}.json"
+    if not p.exists():
+        return None
+    try:
+        return float(json.loads(p.read_text())["bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(N, storage, compute, omega, budget_s=12.0):
+    """Oracle (the C restatement of the reference step) on the host, all
+    threads, a few steps of the same grid."""
+    from oracle import oracle as orc
+    st = {"f64": orc.F64, "f32": orc.F32, "f32dev": orc.F32DEV, "f16dev": orc.F16DEV}[storage]
+    cm = {"f64": orc.COMPUTE_F64, "f32": orc.COMPUTE_F32, "ref": orc.COMPUTE_REF}[compute]
+    o = orc.DenseOracle((N, N, N), st, cm)
+    o.setup_cavity(0.05)
+    o.run(1, omega)                       # first touch
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        o.run(1, omega)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or steps >= 50:
+            break
+    fluid = (N - 2) ** 3
+    return {"value": fluid * steps / el / 1e6, "unit": "MLUPS", "cores": int(orc.lib.or_get_threads()),
+            "kind": "port",
+            "sample": f"{steps} steps of the {N}^3 cavity ({storage} storage, {compute} compute), "
+                      f"{el:.1f} s, CPU oracle (C restatement of LbmTools.h, OpenMP over z)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="cavity256_f32", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-error", action="store_true", help="skip the FP64 comparison run")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the multi-GPU z-slab decomposition is a design, not part of this build)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import neon_amd as nx
+
+    N, storage, compute, re, dsteps, dwarm = WORKLOADS[a.workload]
+    steps = dsteps if a.steps is None else a.steps
+    warmup = dwarm if a.warmup is None else a.warmup
+    ulb = 0.05
+    nu = ulb * N / re
+    omega = 1.0 / (3.0 * nu + 0.5)
+
+    nx.init(0)
+    stream = nx.Stream()
+    lat = nx.DenseLattice((N, N, N), storage, compute)
+    lat.setup_cavity(ulb, stream=stream.handle)
+    lat.run(warmup, omega, stream=stream.handle)
+    stream.sync()
+
+    e0, e1 = nx.Event(), nx.Event()
+    nx.sync()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    lat.run(steps, omega, stream=stream.handle)
+    e1.record(stream)
+    e1.sync()
+    nx.sync()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_ms(e1)
+    ms_step = ms / steps
+
+    fluid = (N - 2) ** 3
+    mlups = fluid * steps / (ms * 1e-3) / 1e6
+    mlups_ref = N ** 3 * steps / (ms * 1e-3) / 1e6
+    b_alg = 2 * 19 * STORAGE_BYTES[storage]
+    achieved = fluid * b_alg / (ms_step * 1e-3) / 1e9
+    tag = f"d3q19_dense_{storage}_{N}"
+    traffic = traffic_from_profiles(tag)
+
+    err = None
+    if not a.no_error and storage != "f64":
+        _, u = lat.moments(stream=stream.handle)
+        ref = nx.DenseLattice((N, N, N), "f64", "f64")
+        ref.setup_cavity(ulb)
+        ref.run(warmup + steps, omega)
+        _, u64 = ref.moments()
+        d = u.astype(np.float64) - u64
+        err = {"max_rel": float(np.abs(d).max() / np.abs(u64).max()),
+               "l2_rel": float(np.sqrt((d * d).sum() / (u64 * u64).sum())),
+               "reference": "binary64 GPU run of the same case (bit-exact with the CPU oracle)"}
+        del ref
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(N, storage, compute, omega)
+
+    out = {
+        "metric": METRIC,
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": storage,
+        "data": "synthetic (lid-driven cavity initialised at rest, f = w_q)",
+        "config": {"workload": f"lid-driven cavity D3Q19 BGK, dense {N}^3 ({N-2}^3 fluid), "
+                               f"{storage} storage + {compute} collision, Re={re:g}, "
+                               f"lid u=({ulb},0,0), {steps} timed steps after {warmup} warm-up",
+                   "grid": "dGrid", "N": N, "storage": storage, "compute": compute,
+                   "omega": omega, "device": nx.device_name()},
+        "mlups_ref_convention": round(mlups_ref, 1),
+        "wall_s": round(wall, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic,
+                     "kernel": f"k_step<{storage},{compute}> ({b_alg} B/fluid cell)"},
+        "velocity_error_vs_f64": err,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,199 @@
+/*
+ * neon_amd.h -- C ABI of the MI355X-native D3Q19 lattice-Boltzmann step.
+ *
+ * Every entry point returns NX_OK (0) or an NX_ERR_* code; the text of the
+ * last error of the calling thread is returned by nx_last_error().  No C++
+ * exception crosses this boundary.  Device pointers are plain pointers
+ * obtained from nx_alloc (or any HIP allocation on the current device);
+ * the library never frees memory it did not allocate.  Launches are
+ * asynchronous on the given stream (NULL = the default stream).
+ *
+ * Cell flag word (one uint32 per cell, the counterpart of the reference's
+ * CellType {wallNghBitflag, classification}, CellType.h:3-35):
+ *   bits  0..18  wall-neighbour mask: bit q set <=> the upstream cell x-c_q of
+ *                population q is not a bulk cell (LbmTools.h:327-377)
+ *   bits 24..25  classification (NX_CELL_*)
+ *   bit  28      NX_FLAG_OPEN: at least one masked link points outside the
+ *                grid or at an inactive voxel (its bounce-back term is 0)
+ *
+ * Population layouts:
+ *   dense (dGrid SoA, dField_imp.h:67-87):  f[q][z][y][x], index
+ *          x + X*y + X*Y*z + X*Y*Z*q
+ *   block (bGrid AoSoA, bPartition_imp.h:95-107): index
+ *          (b*19 + q)*512 + x + 8*y + 64*z   (8^3 blocks)
+ * Storage types: NX_F64, NX_F32 hold f; NX_F32DEV / NX_F16DEV hold f - w_q
+ * for bulk cells and f itself for non-bulk (wall) cells.
+ */
+#ifndef NEON_AMD_H
+#define NEON_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------- */
+#define NX_OK 0
+#define NX_ERR_INVALID 1     /* bad argument (shape, dtype, null pointer) */
+#define NX_ERR_HIP 2         /* HIP runtime error                         */
+#define NX_ERR_OOM 3         /* device allocation failed                  */
+#define NX_ERR_UNSUPPORTED 4 /* combination not implemented               */
+#define NX_ERR_NOT_INIT 5    /* nx_init was not called                    */
+
+/* ---- storage / compute types ----------------------------------------- */
+#define NX_F64 0
+#define NX_F32 1
+#define NX_F32DEV 2 /* f - w_q in binary32 */
+#define NX_F16DEV 3 /* f - w_q in binary16, round to nearest even */
+
+#define NX_COMPUTE_F64 0 /* every operation in binary64                 */
+#define NX_COMPUTE_F32 1 /* every operation in binary32                 */
+#define NX_COMPUTE_REF 2 /* float storage with the reference's literal  */
+                         /* promotions (LbmTools.h:216-279)             */
+
+/* ---- cell classification (CellType.h:5-11) --------------------------- */
+#define NX_CELL_BOUNCEBACK 0
+#define NX_CELL_MOVINGWALL 1
+#define NX_CELL_BULK 2
+#define NX_CELL_UNDEFINED 3
+#define NX_CLASS_SHIFT 24
+#define NX_MASK_BITS 0x7FDFFu
+#define NX_FLAG_OPEN (1u << 28)
+
+#define NX_PERIODIC_X 1
+#define NX_PERIODIC_Y 2
+#define NX_PERIODIC_Z 4
+
+typedef struct nx_stream_s* nx_stream;
+typedef struct nx_event_s* nx_event;
+typedef struct nx_bgrid_s* nx_bgrid;
+
+/* Dense grid: one partition of a dGrid (dGrid_imp.h:7-185). */
+typedef struct {
+    int32_t dim[3];   /* X, Y, Z cells, wall layer included          */
+    int32_t storage;  /* NX_F64 / NX_F32 / NX_F32DEV / NX_F16DEV     */
+    int32_t compute;  /* NX_COMPUTE_*                                 */
+    int32_t periodic; /* NX_PERIODIC_* bits                           */
+} nx_dense_desc;
+
+/* Block-sparse grid: one partition of a bGrid<8,8,8> (bGrid_imp.h:19-246). */
+typedef struct {
+    int32_t dim[3];
+    int32_t storage;
+    int32_t compute;
+    int32_t periodic;
+    uint32_t n_blocks;
+    uint32_t reserved;
+    const int32_t* origins; /* device, 3 x n_blocks: block origin (x,y,z)   */
+    const uint32_t* masks;  /* device, 16 x n_blocks: active bits, x+8y+64z */
+    const uint32_t* conn;   /* device, 27 x n_blocks: neighbour block or    */
+                            /* 0xFFFFFFFF, direction i+3j+9k               */
+} nx_block_desc;
+
+typedef struct {
+    double omega;    /* BGK relaxation frequency 1/tau                  */
+    double force[3]; /* body acceleration g (Guo forcing); 0 = none     */
+} nx_lbm_params;
+
+/* ---- runtime (replaces libNeonSys GpuDevice / Backend, GpuDevice.h:151-190,
+ *      src/set/Backend.cpp:46-55,404-419) -------------------------------- */
+int nx_init(int device);
+const char* nx_last_error(void);
+int nx_device_count(int* count);
+int nx_device_name(char* buf, int len);
+int nx_sync(void); /* Backend::syncAll */
+int nx_stream_create(nx_stream* s);
+int nx_stream_sync(nx_stream s);
+int nx_stream_destroy(nx_stream s);
+int nx_event_create(nx_event* e);
+int nx_event_record(nx_event e, nx_stream s);
+int nx_event_sync(nx_event e);
+int nx_event_elapsed_ms(nx_event start, nx_event stop, float* ms);
+int nx_event_destroy(nx_event e);
+
+/* ---- memory (replaces MemDevice / MemMirror updateDeviceData / updateHostData) */
+int nx_alloc(size_t bytes, void** dev);
+int nx_free(void* dev);
+int nx_memset(void* dev, int value, size_t bytes, nx_stream s);
+int nx_h2d(void* dst, const void* src, size_t bytes, nx_stream s);
+int nx_d2h(void* dst, const void* src, size_t bytes, nx_stream s);
+int nx_d2d(void* dst, const void* src, size_t bytes, nx_stream s);
+int nx_host_alloc(size_t bytes, void** host); /* pinned */
+int nx_host_free(void* host);
+
+/* Bytes of one population field (19 cardinalities) for a storage type. */
+size_t nx_pop_bytes_dense(const nx_dense_desc* d);
+size_t nx_pop_bytes_block(const nx_block_desc* d);
+
+/* ---- dense D3Q19 (dGrid) -----------------------------------------------
+ * nx_cavity_flags_dense: lid-driven cavity classification of
+ *   RunCavityTwoPop.cu:208-224 (outer layer bounceBack, y = Y-1 movingWall,
+ *   mask bits cleared).
+ * nx_d3q19_fill_dense: populations of RunCavityTwoPop.cu:168-206 by class:
+ *   bulk f=w_q, bounceBack f=0, movingWall f=-6 w_q ulb (c_q . ulid).
+ * nx_d3q19_wall_mask_dense: LbmContainers::computeWallNghMask
+ *   (LbmTools.h:327-377); may run in place (flags_in == flags_out).
+ * nx_d3q19_step_dense: LbmContainers::iteration (LbmTools.h:284-325):
+ *   pull stream + bounce-back + BGK for every bulk cell; fin != fout.
+ * nx_d3q19_moments_dense: LbmContainers::computeRhoAndU
+ *   (LbmTools.h:379-437); rho [n], u [3][n]; double for NX_F64 storage,
+ *   float otherwise.
+ */
+int nx_cavity_flags_dense(const nx_dense_desc* d, uint32_t* flags, nx_stream s);
+int nx_d3q19_fill_dense(const nx_dense_desc* d, void* pop, const uint32_t* flags,
+                        double ulb, const double ulid[3], nx_stream s);
+int nx_d3q19_wall_mask_dense(const nx_dense_desc* d, const uint32_t* flags_in,
+                             uint32_t* flags_out, nx_stream s);
+int nx_d3q19_step_dense(const nx_dense_desc* d, const void* fin, void* fout,
+                        const uint32_t* flags, const nx_lbm_params* p, nx_stream s);
+int nx_d3q19_moments_dense(const nx_dense_desc* d, const void* fin,
+                           const uint32_t* flags, void* rho, void* u, nx_stream s);
+
+/* FP64 reductions on the GPU (wave64 DPP + two-level tree; deterministic).
+ * mass: out[0] = sum over bulk cells of sum_q f_q  (device double*).
+ * residual: u from the post-stream moments of fin; out[0] = sum |u-u_prev|^2,
+ *   out[1] = sum |u|^2 over bulk cells, then u_prev <- u
+ *   (u_prev: float [3][n], caller-owned). */
+int nx_d3q19_mass_dense(const nx_dense_desc* d, const void* fin,
+                        const uint32_t* flags, double* out, nx_stream s);
+int nx_d3q19_residual_dense(const nx_dense_desc* d, const void* fin,
+                            const uint32_t* flags, float* u_prev, double* out,
+                            nx_stream s);
+
+/* ---- block-sparse D3Q19 (bGrid, 8^3 blocks) -----------------------------
+ * Host builder restating bGrid's constructor for one partition
+ * (bGrid_imp.h:19-246, Partitioner1D.h:106-255, SpanClassifier.h:170-236):
+ * blocks holding >= 1 active voxel, ordered bz, by, bx; bitmask bit
+ * x+8y+64z; connectivity direction i+3j+9k for offset (i-1,j-1,k-1).
+ * periodic != 0 wraps the connectivity (no counterpart in the reference;
+ * periodic == 0 reproduces it).  `active` is X*Y*Z bytes, x fastest. */
+int nx_bgrid_create(const int32_t dim[3], const uint8_t* active, int32_t periodic,
+                    nx_bgrid* g);
+int nx_bgrid_create_cb(const int32_t dim[3],
+                       int (*active)(int32_t x, int32_t y, int32_t z, void* user),
+                       void* user, int32_t periodic, nx_bgrid* g);
+int nx_bgrid_info(nx_bgrid g, uint32_t* n_blocks, uint64_t* n_active);
+int nx_bgrid_tables(nx_bgrid g, int32_t* origins, uint32_t* masks, uint32_t* conn);
+int nx_bgrid_destroy(nx_bgrid g);
+
+int nx_cavity_flags_block(const nx_block_desc* d, uint32_t* flags, nx_stream s);
+int nx_d3q19_fill_block(const nx_block_desc* d, void* pop, const uint32_t* flags,
+                        double ulb, const double ulid[3], nx_stream s);
+int nx_d3q19_wall_mask_block(const nx_block_desc* d, const uint32_t* flags_in,
+                             uint32_t* flags_out, nx_stream s);
+int nx_d3q19_step_block(const nx_block_desc* d, const void* fin, void* fout,
+                        const uint32_t* flags, const nx_lbm_params* p, nx_stream s);
+int nx_d3q19_moments_block(const nx_block_desc* d, const void* fin,
+                           const uint32_t* flags, void* rho, void* u, nx_stream s);
+int nx_d3q19_mass_block(const nx_block_desc* d, const void* fin,
+                        const uint32_t* flags, double* out, nx_stream s);
+int nx_d3q19_residual_block(const nx_block_desc* d, const void* fin,
+                            const uint32_t* flags, float* u_prev, double* out,
+                            nx_stream s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEON_AMD_H */

@@ -1,0 +1,8 @@
+"""neon_amd -- MI355X-native D3Q19 lattice-Boltzmann step (dGrid / bGrid path).
+
+The compute lives in libneon_amd.so (hand-written gfx950 HIP kernels behind
+the C ABI of include/neon_amd.h).  This package only binds it.
+"""
+from . import abi  # noqa: F401  (raises ImportError when the library is not built)
+from .lattice import (BlockGrid, BlockLattice, DenseLattice, DeviceBuffer, Event,  # noqa: F401
+                      Stream, device_name, init, params, sync)

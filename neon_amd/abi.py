@@ -1,0 +1,127 @@
+"""ctypes binding of the C ABI declared in include/neon_amd.h.
+
+The shared library neon_amd/lib/libneon_amd.so must have been built
+(``python -m neon_amd.build``); there is no fallback: importing this module
+without it raises ImportError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import pathlib
+
+LIB_PATH = pathlib.Path(__file__).resolve().parent / "lib" / "libneon_amd.so"
+
+NX_OK, NX_ERR_INVALID, NX_ERR_HIP, NX_ERR_OOM, NX_ERR_UNSUPPORTED, NX_ERR_NOT_INIT = range(6)
+NX_F64, NX_F32, NX_F32DEV, NX_F16DEV = range(4)
+NX_COMPUTE_F64, NX_COMPUTE_F32, NX_COMPUTE_REF = range(3)
+NX_CELL_BOUNCEBACK, NX_CELL_MOVINGWALL, NX_CELL_BULK, NX_CELL_UNDEFINED = range(4)
+NX_CLASS_SHIFT = 24
+NX_MASK_BITS = 0x7FDFF
+NX_FLAG_OPEN = 1 << 28
+NX_PERIODIC_X, NX_PERIODIC_Y, NX_PERIODIC_Z = 1, 2, 4
+
+STORAGE_BYTES = {NX_F64: 8, NX_F32: 4, NX_F32DEV: 4, NX_F16DEV: 2}
+STORAGE_NAMES = {"f64": NX_F64, "f32": NX_F32, "f32dev": NX_F32DEV, "f16dev": NX_F16DEV}
+COMPUTE_NAMES = {"f64": NX_COMPUTE_F64, "f32": NX_COMPUTE_F32, "ref": NX_COMPUTE_REF}
+
+
+class NeonError(RuntimeError):
+    """Error returned by the C ABI (the counterpart of Neon::NeonException)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[nx error {code}] {msg}")
+        self.code = code
+
+
+class DenseDesc(C.Structure):
+    _fields_ = [("dim", C.c_int32 * 3), ("storage", C.c_int32), ("compute", C.c_int32),
+                ("periodic", C.c_int32)]
+
+
+class BlockDesc(C.Structure):
+    _fields_ = [("dim", C.c_int32 * 3), ("storage", C.c_int32), ("compute", C.c_int32),
+                ("periodic", C.c_int32), ("n_blocks", C.c_uint32), ("reserved", C.c_uint32),
+                ("origins", C.c_void_p), ("masks", C.c_void_p), ("conn", C.c_void_p)]
+
+
+class LbmParams(C.Structure):
+    _fields_ = [("omega", C.c_double), ("force", C.c_double * 3)]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_SZ = C.c_size_t
+_D = C.c_double
+_DP = C.POINTER(DenseDesc)
+_BP = C.POINTER(BlockDesc)
+_PP = C.POINTER(LbmParams)
+_U32 = C.c_uint32
+_I32 = C.c_int32
+
+# name -> (restype, argtypes); every entry point of include/neon_amd.h
+SIGNATURES = {
+    "nx_init": (_I, [_I]),
+    "nx_last_error": (C.c_char_p, []),
+    "nx_device_count": (_I, [C.POINTER(_I)]),
+    "nx_device_name": (_I, [C.c_char_p, _I]),
+    "nx_sync": (_I, []),
+    "nx_stream_create": (_I, [C.POINTER(_P)]),
+    "nx_stream_sync": (_I, [_P]),
+    "nx_stream_destroy": (_I, [_P]),
+    "nx_event_create": (_I, [C.POINTER(_P)]),
+    "nx_event_record": (_I, [_P, _P]),
+    "nx_event_sync": (_I, [_P]),
+    "nx_event_elapsed_ms": (_I, [_P, _P, C.POINTER(C.c_float)]),
+    "nx_event_destroy": (_I, [_P]),
+    "nx_alloc": (_I, [_SZ, C.POINTER(_P)]),
+    "nx_free": (_I, [_P]),
+    "nx_memset": (_I, [_P, _I, _SZ, _P]),
+    "nx_h2d": (_I, [_P, _P, _SZ, _P]),
+    "nx_d2h": (_I, [_P, _P, _SZ, _P]),
+    "nx_d2d": (_I, [_P, _P, _SZ, _P]),
+    "nx_host_alloc": (_I, [_SZ, C.POINTER(_P)]),
+    "nx_host_free": (_I, [_P]),
+    "nx_pop_bytes_dense": (_SZ, [_DP]),
+    "nx_pop_bytes_block": (_SZ, [_BP]),
+    "nx_cavity_flags_dense": (_I, [_DP, _P, _P]),
+    "nx_d3q19_fill_dense": (_I, [_DP, _P, _P, _D, C.POINTER(_D), _P]),
+    "nx_d3q19_wall_mask_dense": (_I, [_DP, _P, _P, _P]),
+    "nx_d3q19_step_dense": (_I, [_DP, _P, _P, _P, _PP, _P]),
+    "nx_d3q19_moments_dense": (_I, [_DP, _P, _P, _P, _P, _P]),
+    "nx_d3q19_mass_dense": (_I, [_DP, _P, _P, _P, _P]),
+    "nx_d3q19_residual_dense": (_I, [_DP, _P, _P, _P, _P, _P]),
+    "nx_bgrid_create": (_I, [C.POINTER(_I32), _P, _I32, C.POINTER(_P)]),
+    "nx_bgrid_create_cb": (_I, [C.POINTER(_I32), _P, _P, _I32, C.POINTER(_P)]),
+    "nx_bgrid_info": (_I, [_P, C.POINTER(_U32), C.POINTER(C.c_uint64)]),
+    "nx_bgrid_tables": (_I, [_P, _P, _P, _P]),
+    "nx_bgrid_destroy": (_I, [_P]),
+    "nx_cavity_flags_block": (_I, [_BP, _P, _P]),
+    "nx_d3q19_fill_block": (_I, [_BP, _P, _P, _D, C.POINTER(_D), _P]),
+    "nx_d3q19_wall_mask_block": (_I, [_BP, _P, _P, _P]),
+    "nx_d3q19_step_block": (_I, [_BP, _P, _P, _P, _PP, _P]),
+    "nx_d3q19_moments_block": (_I, [_BP, _P, _P, _P, _P, _P]),
+    "nx_d3q19_mass_block": (_I, [_BP, _P, _P, _P, _P]),
+    "nx_d3q19_residual_block": (_I, [_BP, _P, _P, _P, _P, _P]),
+}
+
+
+def _load() -> C.CDLL:
+    if not LIB_PATH.exists():
+        raise ImportError(
+            f"{LIB_PATH} is not built; run `python -m neon_amd.build` (HIP library for gfx950)")
+    lib = C.CDLL(str(LIB_PATH))
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(code: int) -> None:
+    """Raise NeonError for a non-zero status of the C ABI."""
+    if code != NX_OK:
+        msg = lib.nx_last_error()
+        raise NeonError(code, msg.decode() if msg else "")

@@ -1,0 +1,517 @@
+// d3q19_block.hip -- D3Q19 kernels on the block-sparse grid (the bGrid path,
+// 8^3 blocks).
+//
+// Layout: AoSoA populations, index (b*19 + q)*512 + x + 8y + 64z
+// (bPartition_imp.h:95-107); flags (b*512 + v); active bitmask 16 words per
+// block; 27-entry block connectivity (bGrid_imp.h:139-172).
+//
+// One workgroup of 512 threads per block, thread v = x + 8y + 64z, so each
+// wavefront owns one z-plane of the block and every population store is 64
+// contiguous values.  The block's neighbourhood is staged once per
+// workgroup: the own bitmask in LDS, and the connectivity row read with
+// scalar loads (uniform per workgroup).  Per population the neighbour block
+// of a lane is chosen with compile-time candidate directions -- a D3Q19
+// velocity has at most two non-zero components, so at most three neighbour
+// blocks (x-face, y/z-face, edge) can be involved -- and selected with
+// v_cndmask, no per-lane table lookup.  This restates the neighbour rule of
+// bPartition::helpGetNghIdx / helpNghPitch (bPartition_imp.h:109-124,
+// 218-309).
+#include <algorithm>
+#include <type_traits>
+
+#include "nx_common.hpp"
+
+namespace nx {
+namespace {
+
+constexpr int kBV = 512;  // voxels per block
+constexpr uint32_t kBStride = Q * kBV;
+
+struct BGeom {
+    int32_t X, Y, Z;
+    int32_t periodic;
+    uint32_t n_blocks;
+    const int32_t* origins;
+    const uint32_t* masks;
+    const uint32_t* conn;
+};
+
+BGeom make_bgeom(const nx_block_desc* d)
+{
+    return BGeom{d->dim[0],   d->dim[1], d->dim[2], d->periodic,
+                 d->n_blocks, d->origins, d->masks, d->conn};
+}
+
+struct Par {
+    double omega;
+    double g[3];
+};
+
+// Neighbour voxel (block t, in-block index nv) of voxel (x,y,z) of block b at
+// offset -c_q (the upstream voxel of population q).
+template <int q>
+__device__ __forceinline__ void upstream(uint32_t b, const uint32_t* __restrict__ cb, int x, int y,
+                                         int z, uint32_t& t, uint32_t& nv)
+{
+    constexpr int s0 = -CX[q], s1 = -CY[q], s2 = -CZ[q];
+    const bool c0 = s0 != 0 && x == (s0 < 0 ? 0 : 7);
+    const bool c1 = s1 != 0 && y == (s1 < 0 ? 0 : 7);
+    const bool c2 = s2 != 0 && z == (s2 < 0 ? 0 : 7);
+    t = b;
+    if constexpr (s0 != 0) t = c0 ? cb[13 + s0] : t;
+    if constexpr (s1 != 0) t = c1 ? cb[13 + 3 * s1] : t;
+    if constexpr (s2 != 0) t = c2 ? cb[13 + 9 * s2] : t;
+    if constexpr (s0 != 0 && s1 != 0) t = (c0 && c1) ? cb[13 + s0 + 3 * s1] : t;
+    if constexpr (s0 != 0 && s2 != 0) t = (c0 && c2) ? cb[13 + s0 + 9 * s2] : t;
+    if constexpr (s1 != 0 && s2 != 0) t = (c1 && c2) ? cb[13 + 3 * s1 + 9 * s2] : t;
+    nv = (uint32_t)(((x + s0) & 7) + 8 * ((y + s1) & 7) + 64 * ((z + s2) & 7));
+}
+
+__device__ __forceinline__ bool bit_of(const uint32_t* w, uint32_t v)
+{
+    return (w[v >> 5] >> (v & 31u)) & 1u;
+}
+
+// validity of (t, nv) as seen from block b (own mask in LDS)
+__device__ __forceinline__ bool valid_voxel(uint32_t b, uint32_t t, uint32_t nv,
+                                            const uint32_t* smask, const uint32_t* __restrict__ masks)
+{
+    if (t == kInvalidBlock) return false;
+    if (t == b) return bit_of(smask, nv);
+    return (masks[(uint64_t)t * 16 + (nv >> 5)] >> (nv & 31u)) & 1u;
+}
+
+template <int q>
+struct QI {
+    static constexpr int value = q;
+};
+
+template <int... qs, typename F>
+__device__ __forceinline__ void for_q(std::integer_sequence<int, qs...>, F&& f)
+{
+    (f(QI<qs>{}), ...);
+}
+using AllQ = std::make_integer_sequence<int, Q>;
+
+// Pull stream of one bulk voxel (LbmTools.h:78-168); masked links whose
+// upstream voxel is absent or inactive contribute 0 (NX_FLAG_OPEN).
+template <int S>
+__device__ __forceinline__ void pull(const typename Stor<S>::T* __restrict__ fin, const BGeom& g,
+                                     uint32_t b, const uint32_t* __restrict__ cb,
+                                     const uint32_t* smask, uint32_t v, uint32_t fl,
+                                     typename Stor<S>::P f[Q])
+{
+    using T = typename Stor<S>::T;
+    using P = typename Stor<S>::P;
+    const int x = (int)(v & 7u), y = (int)((v >> 3) & 7u), z = (int)(v >> 6);
+    const uint32_t m = fl & NX_MASK_BITS;
+    const bool open = (fl & NX_FLAG_OPEN) != 0;
+    const uint64_t own = (uint64_t)b * kBStride + v;
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (q == 9) {
+            f[9] = Stor<S>::bulk(fin[own + 9 * kBV], 9);
+        } else {
+            uint32_t t, nv;
+            upstream<q>(b, cb, x, y, z, t, nv);
+            const bool w = (m >> q) & 1u;
+            // an absent block can only be reached through inconsistent flags;
+            // treat it as a missing link rather than addressing through it
+            bool ok = t != kInvalidBlock;
+            if (w && open) ok = valid_voxel(b, t, nv, smask, g.masks);
+            const uint32_t src = w ? (uint32_t)OPP[q] : (uint32_t)q;
+            const uint64_t a = ok ? (uint64_t)t * kBStride + src * kBV + nv : own;
+            T val = fin[a];
+            if (!ok) val = T(0);
+            f[q] = w ? Stor<S>::raw(val) : Stor<S>::bulk(val, q);
+        }
+    });
+    if (m) {
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (q != 9) {
+                if ((m >> q) & 1u) {
+                    const P o = Stor<S>::bulk(fin[own + (uint64_t)OPP[q] * kBV], OPP[q]);
+                    f[q] = o + f[q];
+                }
+            }
+        });
+    }
+}
+
+// Stage the block's bitmask in LDS; returns whether voxel v is active.
+__device__ __forceinline__ bool stage(const BGeom& g, uint32_t b, uint32_t* smask)
+{
+    if (threadIdx.x < 16) smask[threadIdx.x] = g.masks[(uint64_t)b * 16 + threadIdx.x];
+    __syncthreads();
+    return bit_of(smask, threadIdx.x);
+}
+
+template <int S, int CM, bool FORCE>
+__global__ void __launch_bounds__(kBV)
+    k_step(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
+           const uint32_t* __restrict__ flags, BGeom g, Par p)
+{
+    using R = typename Real<S, CM>::R;
+    __shared__ uint32_t smask[16];
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t v = threadIdx.x;
+    if (!stage(g, b, smask)) return;
+    const uint32_t fl = __builtin_nontemporal_load(&flags[(uint64_t)b * kBV + v]);
+    if (cls_of(fl) != NX_CELL_BULK) return;
+    const uint32_t* cb = g.conn + (uint64_t)b * 27;
+    typename Stor<S>::P f[Q];
+    pull<S>(fin, g, b, cb, smask, v, fl, f);
+    R out[Q];
+    collide<S, CM, FORCE>(f, p.omega, p.g, out);
+    const uint64_t own = (uint64_t)b * kBStride + v;
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        __builtin_nontemporal_store(Stor<S>::template enc<R>(out[q], q), &fout[own + q * kBV]);
+}
+
+__global__ void __launch_bounds__(kBV) k_cavity_flags(uint32_t* flags, BGeom g)
+{
+    __shared__ uint32_t smask[16];
+    const uint32_t b = blockIdx.x, v = threadIdx.x;
+    const bool act = stage(g, b, smask);
+    int cls = NX_CELL_UNDEFINED;
+    if (act) {
+        const int x = g.origins[3 * (uint64_t)b] + (int)(v & 7u);
+        const int y = g.origins[3 * (uint64_t)b + 1] + (int)((v >> 3) & 7u);
+        const int z = g.origins[3 * (uint64_t)b + 2] + (int)(v >> 6);
+        cls = NX_CELL_BULK;
+        if (x == 0 || x == g.X - 1 || y == 0 || y == g.Y - 1 || z == 0 || z == g.Z - 1) {
+            cls = NX_CELL_BOUNCEBACK;
+            if (y == g.Y - 1) cls = NX_CELL_MOVINGWALL;
+        }
+    }
+    flags[(uint64_t)b * kBV + v] = (uint32_t)cls << NX_CLASS_SHIFT;
+}
+
+template <int S>
+__global__ void __launch_bounds__(kBV)
+    k_fill(typename Stor<S>::T* pop, const uint32_t* flags, BGeom g, double ulb, double l0,
+           double l1, double l2)
+{
+    __shared__ uint32_t smask[16];
+    const uint32_t b = blockIdx.x, v = threadIdx.x;
+    const bool act = stage(g, b, smask);
+    const int cls = act ? cls_of(flags[(uint64_t)b * kBV + v]) : NX_CELL_UNDEFINED;
+    const uint64_t own = (uint64_t)b * kBStride + v;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        typename Stor<S>::T val;
+        if (cls == NX_CELL_BULK) {
+            val = Stor<S>::template enc<double>(wq(q), q);
+        } else if (cls == NX_CELL_MOVINGWALL) {
+            const double w = -6. * wq(q) * ulb * (CX[q] * l0 + CY[q] * l1 + CZ[q] * l2);
+            val = (typename Stor<S>::T)(typename Stor<S>::P)w;
+        } else {
+            val = (typename Stor<S>::T)0.;
+        }
+        pop[own + q * kBV] = val;
+    }
+}
+
+// computeWallNghMask on the block grid; absent or inactive upstream voxels
+// count as non-bulk (their NghData default is CellType::undefined).
+__global__ void __launch_bounds__(kBV)
+    k_wall_mask(const uint32_t* fin, uint32_t* fout, BGeom g)
+{
+    __shared__ uint32_t smask[16];
+    const uint32_t b = blockIdx.x, v = threadIdx.x;
+    const bool act = stage(g, b, smask);
+    const uint64_t i = (uint64_t)b * kBV + v;
+    const uint32_t fl = fin[i];
+    if (!act || cls_of(fl) != NX_CELL_BULK) {
+        fout[i] = fl;
+        return;
+    }
+    const uint32_t* cb = g.conn + (uint64_t)b * 27;
+    const int x = (int)(v & 7u), y = (int)((v >> 3) & 7u), z = (int)(v >> 6);
+    uint32_t m = 0, open = 0;
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (q != 9) {
+            uint32_t t, nv;
+            upstream<q>(b, cb, x, y, z, t, nv);
+            if (!valid_voxel(b, t, nv, smask, g.masks)) {
+                m |= 1u << q;
+                open = NX_FLAG_OPEN;
+            } else if (cls_of(fin[(uint64_t)t * kBV + nv]) != NX_CELL_BULK) {
+                m |= 1u << q;
+            }
+        }
+    });
+    fout[i] = ((uint32_t)NX_CELL_BULK << NX_CLASS_SHIFT) | m | open;
+}
+
+template <int S, int CM>
+__global__ void __launch_bounds__(kBV)
+    k_moments(const typename Stor<S>::T* __restrict__ fin, const uint32_t* __restrict__ flags,
+              void* rho_, void* u_, BGeom g)
+{
+    using R = typename Real<S, CM>::R;
+    using M = typename std::conditional<S == NX_F64, double, float>::type;
+    __shared__ uint32_t smask[16];
+    const uint32_t b = blockIdx.x, v = threadIdx.x;
+    const bool act = stage(g, b, smask);
+    const uint32_t fl = act ? flags[(uint64_t)b * kBV + v] : (uint32_t)NX_CELL_UNDEFINED << 24;
+    const int cls = cls_of(fl);
+    typename Stor<S>::P f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = 0;
+    const uint64_t own = (uint64_t)b * kBStride + v;
+    if (cls == NX_CELL_BULK) pull<S>(fin, g, b, g.conn + (uint64_t)b * 27, smask, v, fl, f);
+    if (cls == NX_CELL_MOVINGWALL)
+        for (int a = 0; a < 3; a++) f[a] = Stor<S>::raw(fin[own + a * kBV]);
+    R r, uu[3];
+    cell_moments<S, CM>(cls, f, r, uu);
+    M* rho = (M*)rho_;
+    M* u = (M*)u_;
+    rho[(uint64_t)b * kBV + v] = (M)r;
+    for (int a = 0; a < 3; a++) u[((uint64_t)b * 3 + a) * kBV + v] = (M)uu[a];
+}
+
+constexpr int kRedGrid = 2048;
+
+template <int S>
+__global__ void __launch_bounds__(kBV)
+    k_mass(const typename Stor<S>::T* __restrict__ fin, const uint32_t* __restrict__ flags,
+           BGeom g, double* partials)
+{
+    __shared__ double lds[16];
+    double acc[1] = {0.0};
+    const uint32_t v = threadIdx.x;
+    for (uint32_t b = blockIdx.x; b < g.n_blocks; b += gridDim.x) {
+        const bool act = (g.masks[(uint64_t)b * 16 + (v >> 5)] >> (v & 31u)) & 1u;
+        if (!act || cls_of(flags[(uint64_t)b * kBV + v]) != NX_CELL_BULK) continue;
+        const uint64_t own = (uint64_t)b * kBStride + v;
+        double r = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) r += (double)Stor<S>::bulk(fin[own + q * kBV], q);
+        acc[0] += r;
+    }
+    block_sum_f64<1>(acc, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
+}
+
+template <int S, int CM>
+__global__ void __launch_bounds__(kBV)
+    k_residual(const typename Stor<S>::T* __restrict__ fin, const uint32_t* __restrict__ flags,
+               float* __restrict__ u_prev, BGeom g, double* partials)
+{
+    using R = typename Real<S, CM>::R;
+    __shared__ double lds[32];
+    __shared__ uint32_t smask[16];
+    double acc[2] = {0.0, 0.0};
+    const uint32_t v = threadIdx.x;
+    for (uint32_t b = blockIdx.x; b < g.n_blocks; b += gridDim.x) {
+        __syncthreads();  // smask reuse across iterations
+        const bool act = stage(g, b, smask);
+        if (!act) continue;
+        const uint32_t fl = flags[(uint64_t)b * kBV + v];
+        if (cls_of(fl) != NX_CELL_BULK) continue;
+        typename Stor<S>::P f[Q];
+        pull<S>(fin, g, b, g.conn + (uint64_t)b * 27, smask, v, fl, f);
+        R r, uu[3];
+        cell_moments<S, CM>(NX_CELL_BULK, f, r, uu);
+        for (int a = 0; a < 3; a++) {
+            const uint64_t k = ((uint64_t)b * 3 + a) * kBV + v;
+            const float un = (float)uu[a];
+            const double du = (double)un - (double)u_prev[k];
+            acc[0] += du * du;
+            acc[1] += (double)un * (double)un;
+            u_prev[k] = un;
+        }
+    }
+    block_sum_f64<2>(acc, lds);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = acc[0];
+        partials[2 * blockIdx.x + 1] = acc[1];
+    }
+}
+
+int check(const nx_block_desc* d)
+{
+    if (!d) return set_error(NX_ERR_INVALID, "null descriptor");
+    if (!d->masks || !d->conn || !d->origins)
+        return set_error(NX_ERR_INVALID, "block descriptor without device tables");
+    if (d->n_blocks == 0) return set_error(NX_ERR_INVALID, "empty block grid");
+    if (d->n_blocks > 0x7FFFFFFFu) return set_error(NX_ERR_INVALID, "too many blocks");
+    if (d->storage < NX_F64 || d->storage > NX_F16DEV)
+        return set_error(NX_ERR_INVALID, "unknown storage type");
+    if (d->compute < NX_COMPUTE_F64 || d->compute > NX_COMPUTE_REF)
+        return set_error(NX_ERR_INVALID, "unknown compute mode");
+    if ((d->storage == NX_F32DEV || d->storage == NX_F16DEV) && d->compute != NX_COMPUTE_F32)
+        return set_error(NX_ERR_UNSUPPORTED, "deviation storage requires NX_COMPUTE_F32");
+    if (d->storage == NX_F64 && d->compute == NX_COMPUTE_F32)
+        return set_error(NX_ERR_UNSUPPORTED, "binary64 storage requires binary64 compute");
+    return NX_OK;
+}
+
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+template <typename F>
+int by_type(const nx_block_desc* d, F&& f)
+{
+    switch (d->storage) {
+        case NX_F64: f(IC<NX_F64>{}, IC<NX_COMPUTE_F64>{}); return NX_OK;
+        case NX_F32:
+            if (d->compute == NX_COMPUTE_F64) f(IC<NX_F32>{}, IC<NX_COMPUTE_F64>{});
+            else if (d->compute == NX_COMPUTE_REF) f(IC<NX_F32>{}, IC<NX_COMPUTE_REF>{});
+            else f(IC<NX_F32>{}, IC<NX_COMPUTE_F32>{});
+            return NX_OK;
+        case NX_F32DEV: f(IC<NX_F32DEV>{}, IC<NX_COMPUTE_F32>{}); return NX_OK;
+        case NX_F16DEV: f(IC<NX_F16DEV>{}, IC<NX_COMPUTE_F32>{}); return NX_OK;
+    }
+    return set_error(NX_ERR_INVALID, "unknown storage type");
+}
+
+}  // namespace
+}  // namespace nx
+
+using namespace nx;
+
+extern "C" size_t nx_pop_bytes_block(const nx_block_desc* d)
+{
+    if (!d) return 0;
+    const size_t n = (size_t)d->n_blocks * kBStride;
+    switch (d->storage) {
+        case NX_F64: return n * 8;
+        case NX_F32:
+        case NX_F32DEV: return n * 4;
+        case NX_F16DEV: return n * 2;
+    }
+    return 0;
+}
+
+extern "C" int nx_cavity_flags_block(const nx_block_desc* d, uint32_t* flags, nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!flags) return set_error(NX_ERR_INVALID, "null flags");
+    hipLaunchKernelGGL(k_cavity_flags, dim3(d->n_blocks), dim3(kBV), 0, (hipStream_t)s, flags,
+                       make_bgeom(d));
+    NX_CHECK_LAUNCH("k_cavity_flags(block)");
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_fill_block(const nx_block_desc* d, void* pop, const uint32_t* flags,
+                                   double ulb, const double ulid[3], nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!pop || !flags || !ulid) return set_error(NX_ERR_INVALID, "null pointer");
+    const BGeom g = make_bgeom(d);
+    const hipStream_t st = (hipStream_t)s;
+    if (int e = by_type(d, [&](auto sc, auto) {
+            constexpr int S = decltype(sc)::value;
+            hipLaunchKernelGGL((k_fill<S>), dim3(g.n_blocks), dim3(kBV), 0, st,
+                               (typename Stor<S>::T*)pop, flags, g, ulb, ulid[0], ulid[1], ulid[2]);
+        }))
+        return e;
+    NX_CHECK_LAUNCH("k_fill(block)");
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_wall_mask_block(const nx_block_desc* d, const uint32_t* flags_in,
+                                        uint32_t* flags_out, nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!flags_in || !flags_out) return set_error(NX_ERR_INVALID, "null flags");
+    hipLaunchKernelGGL(k_wall_mask, dim3(d->n_blocks), dim3(kBV), 0, (hipStream_t)s, flags_in,
+                       flags_out, make_bgeom(d));
+    NX_CHECK_LAUNCH("k_wall_mask(block)");
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_step_block(const nx_block_desc* d, const void* fin, void* fout,
+                                   const uint32_t* flags, const nx_lbm_params* p, nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!fin || !fout || !flags || !p) return set_error(NX_ERR_INVALID, "null pointer");
+    if (fin == fout)
+        return set_error(NX_ERR_INVALID, "fin and fout must differ (two-population scheme)");
+    const BGeom g = make_bgeom(d);
+    const Par par{p->omega, {p->force[0], p->force[1], p->force[2]}};
+    const bool force = p->force[0] != 0. || p->force[1] != 0. || p->force[2] != 0.;
+    if (force && d->compute == NX_COMPUTE_REF)
+        return set_error(NX_ERR_UNSUPPORTED,
+                         "body force requires NX_COMPUTE_F32 or NX_COMPUTE_F64");
+    const hipStream_t st = (hipStream_t)s;
+    if (int e = by_type(d, [&](auto sc, auto cc) {
+            constexpr int S = decltype(sc)::value;
+            constexpr int CM = decltype(cc)::value;
+            using T = typename Stor<S>::T;
+            if (force)
+                hipLaunchKernelGGL((k_step<S, CM, true>), dim3(g.n_blocks), dim3(kBV), 0, st,
+                                   (const T*)fin, (T*)fout, flags, g, par);
+            else
+                hipLaunchKernelGGL((k_step<S, CM, false>), dim3(g.n_blocks), dim3(kBV), 0, st,
+                                   (const T*)fin, (T*)fout, flags, g, par);
+        }))
+        return e;
+    NX_CHECK_LAUNCH("k_step(block)");
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_moments_block(const nx_block_desc* d, const void* fin,
+                                      const uint32_t* flags, void* rho, void* u, nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!fin || !flags || !rho || !u) return set_error(NX_ERR_INVALID, "null pointer");
+    const BGeom g = make_bgeom(d);
+    const hipStream_t st = (hipStream_t)s;
+    if (int e = by_type(d, [&](auto sc, auto cc) {
+            constexpr int S = decltype(sc)::value;
+            constexpr int CM = decltype(cc)::value;
+            hipLaunchKernelGGL((k_moments<S, CM>), dim3(g.n_blocks), dim3(kBV), 0, st,
+                               (const typename Stor<S>::T*)fin, flags, rho, u, g);
+        }))
+        return e;
+    NX_CHECK_LAUNCH("k_moments(block)");
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_mass_block(const nx_block_desc* d, const void* fin, const uint32_t* flags,
+                                   double* out, nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!fin || !flags || !out) return set_error(NX_ERR_INVALID, "null pointer");
+    const BGeom g = make_bgeom(d);
+    const int nb = (int)std::min<uint32_t>(g.n_blocks, kRedGrid);
+    void* part = nullptr;
+    if (int e = scratch(sizeof(double) * nb, &part)) return e;
+    const hipStream_t st = (hipStream_t)s;
+    if (int e = by_type(d, [&](auto sc, auto) {
+            constexpr int S = decltype(sc)::value;
+            hipLaunchKernelGGL((k_mass<S>), dim3(nb), dim3(kBV), 0, st,
+                               (const typename Stor<S>::T*)fin, flags, g, (double*)part);
+        }))
+        return e;
+    NX_CHECK_LAUNCH("k_mass(block)");
+    return finish_reduce((const double*)part, nb, 1, out, st);
+}
+
+extern "C" int nx_d3q19_residual_block(const nx_block_desc* d, const void* fin,
+                                       const uint32_t* flags, float* u_prev, double* out,
+                                       nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!fin || !flags || !u_prev || !out) return set_error(NX_ERR_INVALID, "null pointer");
+    const BGeom g = make_bgeom(d);
+    const int nb = (int)std::min<uint32_t>(g.n_blocks, kRedGrid);
+    void* part = nullptr;
+    if (int e = scratch(2 * sizeof(double) * nb, &part)) return e;
+    const hipStream_t st = (hipStream_t)s;
+    if (int e = by_type(d, [&](auto sc, auto cc) {
+            constexpr int S = decltype(sc)::value;
+            constexpr int CM = decltype(cc)::value;
+            hipLaunchKernelGGL((k_residual<S, CM>), dim3(nb), dim3(kBV), 0, st,
+                               (const typename Stor<S>::T*)fin, flags, u_prev, g, (double*)part);
+        }))
+        return e;
+    NX_CHECK_LAUNCH("k_residual(block)");
+    return finish_reduce((const double*)part, nb, 2, out, st);
+}

@@ -1,0 +1,517 @@
+// d3q19_dense.hip -- D3Q19 kernels on a dense grid (the dGrid path).
+//
+// Layout: SoA populations f[q][z][y][x] (dField_imp.h:67-87,
+// dPartition.h:90-99), one uint32 flag word per cell.  One thread per cell,
+// 256 cells (4 wavefronts) per workgroup over the linear cell index, so every
+// population load and store of a wavefront is one contiguous 64-cell run;
+// the +-1 x shifts of the pull read a neighbouring cache line that the
+// adjacent wavefront of the same workgroup fetches anyway.  Workgroups are
+// remapped so each XCD walks a contiguous slab (L2 keeps the row-boundary
+// lines).  Each population is read once and written once per step:
+// 2*19*sizeof(storage) bytes per cell update.
+#include <algorithm>
+#include <type_traits>
+
+#include "nx_common.hpp"
+
+namespace nx {
+namespace {
+
+constexpr int kWG = 256;
+
+struct DGeom {
+    int32_t X, Y, Z;
+    uint32_t XY;
+    uint32_t n;       // cells
+    uint64_t plane;   // cells per population plane
+    int32_t periodic;
+};
+
+DGeom make_geom(const nx_dense_desc* d)
+{
+    DGeom g;
+    g.X = d->dim[0];
+    g.Y = d->dim[1];
+    g.Z = d->dim[2];
+    g.XY = (uint32_t)g.X * (uint32_t)g.Y;
+    g.n = g.XY * (uint32_t)g.Z;
+    g.plane = g.n;
+    g.periodic = d->periodic;
+    return g;
+}
+
+struct Par {
+    double omega;
+    double g[3];
+};
+
+// Upstream index of population q of cell i (x - c_q), with periodic wrap.
+// Returns false when the upstream cell lies outside the grid.
+__device__ __forceinline__ bool upstream(const DGeom& g, int x, int y, int z, int q, uint32_t& j)
+{
+    int a = x - CX[q], b = y - CY[q], c = z - CZ[q];
+    if (g.periodic & NX_PERIODIC_X) a = a < 0 ? a + g.X : (a >= g.X ? a - g.X : a);
+    if (g.periodic & NX_PERIODIC_Y) b = b < 0 ? b + g.Y : (b >= g.Y ? b - g.Y : b);
+    if (g.periodic & NX_PERIODIC_Z) c = c < 0 ? c + g.Z : (c >= g.Z ? c - g.Z : c);
+    j = (uint32_t)a + (uint32_t)g.X * (uint32_t)b + g.XY * (uint32_t)c;
+    return a >= 0 && b >= 0 && c >= 0 && a < g.X && b < g.Y && c < g.Z;
+}
+
+__device__ __forceinline__ void coords(const DGeom& g, uint32_t i, int& x, int& y, int& z)
+{
+    z = (int)(i / g.XY);
+    const uint32_t r = i - (uint32_t)z * g.XY;
+    y = (int)(r / (uint32_t)g.X);
+    x = (int)(r - (uint32_t)y * (uint32_t)g.X);
+}
+
+// Pull stream of one bulk cell (LbmTools.h:78-168): for q != 9,
+//   wall bit q set:  f_q = fin(x, opp q) + fin(x - c_q, opp q)
+//   otherwise:       f_q = fin(x - c_q, q)
+// A masked link whose upstream cell does not exist contributes 0
+// (NX_FLAG_OPEN; the reference leaves that value undefined, NghData.h:13-16).
+template <int S, bool PERIODIC>
+__device__ __forceinline__ void pull(const typename Stor<S>::T* __restrict__ fin, const DGeom& g,
+                                     uint32_t i, uint32_t fl, typename Stor<S>::P f[Q])
+{
+    using T = typename Stor<S>::T;
+    using P = typename Stor<S>::P;
+    const uint32_t m = fl & NX_MASK_BITS;
+    uint32_t jj[Q];
+    uint32_t missing = 0;
+    if (PERIODIC || (fl & NX_FLAG_OPEN)) {
+        int x, y, z;
+        coords(g, i, x, y, z);
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            if (q == 9) continue;
+            uint32_t j;
+            const bool ok = upstream(g, x, y, z, q, j);
+            jj[q] = ok ? j : i;
+            missing |= ok ? 0u : (1u << q);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            if (q == 9) continue;
+            const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
+            // flags from the wall-mask kernel keep this in range; the clamp
+            // keeps any caller-supplied flag word from addressing outside
+            jj[q] = min(i - (uint32_t)off, g.n - 1u);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        if (q == 9) {
+            f[9] = Stor<S>::bulk(fin[9 * g.plane + i], 9);
+            continue;
+        }
+        const bool w = (m >> q) & 1u;
+        const uint64_t src = w ? (uint64_t)OPP[q] : (uint64_t)q;
+        T v = fin[src * g.plane + jj[q]];
+        if ((missing >> q) & 1u) v = T(0);
+        f[q] = w ? Stor<S>::raw(v) : Stor<S>::bulk(v, q);
+    }
+    if (m) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            if (q == 9) continue;
+            if ((m >> q) & 1u) {
+                const P own = Stor<S>::bulk(fin[(uint64_t)OPP[q] * g.plane + i], OPP[q]);
+                f[q] = own + f[q];
+            }
+        }
+    }
+}
+
+template <int S, int CM, bool FORCE, bool PERIODIC>
+__global__ void __launch_bounds__(kWG)
+    k_step(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
+           const uint32_t* __restrict__ flags, DGeom g, Par p)
+{
+    using R = typename Real<S, CM>::R;
+    const uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * kWG + threadIdx.x;
+    if (i >= g.n) return;
+    const uint32_t fl = __builtin_nontemporal_load(&flags[i]);
+    if (cls_of(fl) != NX_CELL_BULK) return;
+    typename Stor<S>::P f[Q];
+    pull<S, PERIODIC>(fin, g, i, fl, f);
+    R out[Q];
+    collide<S, CM, FORCE>(f, p.omega, p.g, out);
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        __builtin_nontemporal_store(Stor<S>::template enc<R>(out[q], q), &fout[q * g.plane + i]);
+}
+
+// Cavity classification, RunCavityTwoPop.cu:208-224.
+__global__ void __launch_bounds__(kWG) k_cavity_flags(uint32_t* flags, DGeom g)
+{
+    const uint32_t i = blockIdx.x * kWG + threadIdx.x;
+    if (i >= g.n) return;
+    int x, y, z;
+    coords(g, i, x, y, z);
+    int cls = NX_CELL_BULK;
+    if (x == 0 || x == g.X - 1 || y == 0 || y == g.Y - 1 || z == 0 || z == g.Z - 1) {
+        cls = NX_CELL_BOUNCEBACK;
+        if (y == g.Y - 1) cls = NX_CELL_MOVINGWALL;
+    }
+    flags[i] = (uint32_t)cls << NX_CLASS_SHIFT;
+}
+
+// Initial populations by class, RunCavityTwoPop.cu:168-206.
+template <int S>
+__global__ void __launch_bounds__(kWG)
+    k_fill(typename Stor<S>::T* pop, const uint32_t* flags, DGeom g, double ulb, double l0,
+           double l1, double l2)
+{
+    const uint32_t i = blockIdx.x * kWG + threadIdx.x;
+    if (i >= g.n) return;
+    const int cls = cls_of(flags[i]);
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        typename Stor<S>::T v;
+        if (cls == NX_CELL_BULK) {
+            v = Stor<S>::template enc<double>(wq(q), q);
+        } else if (cls == NX_CELL_MOVINGWALL) {
+            const double val = -6. * wq(q) * ulb * (CX[q] * l0 + CY[q] * l1 + CZ[q] * l2);
+            // binary16 goes through binary32 (the oracle's rounding path)
+            v = (typename Stor<S>::T)(typename Stor<S>::P)val;
+        } else {
+            v = (typename Stor<S>::T)0.;
+        }
+        pop[q * g.plane + i] = v;
+    }
+}
+
+// computeWallNghMask, LbmTools.h:327-377: bit q <=> upstream cell of q is not
+// bulk.  In place is allowed: only the mask bits of bulk cells change and the
+// kernel reads classification bits only.
+__global__ void __launch_bounds__(kWG)
+    k_wall_mask(const uint32_t* fin, uint32_t* fout, DGeom g)
+{
+    const uint32_t i = blockIdx.x * kWG + threadIdx.x;
+    if (i >= g.n) return;
+    const uint32_t fl = fin[i];
+    if (cls_of(fl) != NX_CELL_BULK) {
+        fout[i] = fl;
+        return;
+    }
+    int x, y, z;
+    coords(g, i, x, y, z);
+    uint32_t m = 0, open = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        if (q == 9) continue;
+        uint32_t j;
+        if (!upstream(g, x, y, z, q, j)) {
+            m |= 1u << q;
+            open = NX_FLAG_OPEN;
+        } else if (cls_of(fin[j]) != NX_CELL_BULK) {
+            m |= 1u << q;
+        }
+    }
+    fout[i] = ((uint32_t)NX_CELL_BULK << NX_CLASS_SHIFT) | m | open;
+}
+
+template <typename M>
+__device__ __forceinline__ void put_moments(M* rho, M* u, const DGeom& g, uint32_t i, double r,
+                                            const double uu[3])
+{
+    rho[i] = (M)r;
+    u[i] = (M)uu[0];
+    u[g.plane + i] = (M)uu[1];
+    u[2 * g.plane + i] = (M)uu[2];
+}
+
+// computeRhoAndU, LbmTools.h:379-437
+template <int S, int CM, bool PERIODIC>
+__global__ void __launch_bounds__(kWG)
+    k_moments(const typename Stor<S>::T* __restrict__ fin, const uint32_t* __restrict__ flags,
+              void* rho_, void* u_, DGeom g)
+{
+    using R = typename Real<S, CM>::R;
+    using M = typename std::conditional<S == NX_F64, double, float>::type;
+    const uint32_t i = blockIdx.x * kWG + threadIdx.x;
+    if (i >= g.n) return;
+    const uint32_t fl = flags[i];
+    const int cls = cls_of(fl);
+    typename Stor<S>::P f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = 0;
+    if (cls == NX_CELL_BULK) pull<S, PERIODIC>(fin, g, i, fl, f);
+    if (cls == NX_CELL_MOVINGWALL)
+        for (int a = 0; a < 3; a++) f[a] = Stor<S>::raw(fin[a * g.plane + i]);
+    R r, uu[3];
+    cell_moments<S, CM>(cls, f, r, uu);
+    const double ud[3] = {(double)uu[0], (double)uu[1], (double)uu[2]};
+    put_moments<M>((M*)rho_, (M*)u_, g, i, (double)r, ud);
+}
+
+constexpr int kRedGrid = 2048;
+
+// FP64 mass: sum over bulk cells of sum_q f_q (f decoded, summed in q order)
+template <int S>
+__global__ void __launch_bounds__(kWG)
+    k_mass(const typename Stor<S>::T* __restrict__ fin, const uint32_t* __restrict__ flags,
+           DGeom g, double* partials)
+{
+    __shared__ double lds[16];
+    double acc[1] = {0.0};
+    for (uint32_t i = blockIdx.x * kWG + threadIdx.x; i < g.n; i += gridDim.x * kWG) {
+        if (cls_of(flags[i]) != NX_CELL_BULK) continue;
+        double r = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) r += (double)Stor<S>::bulk(fin[q * g.plane + i], q);
+        acc[0] += r;
+    }
+    block_sum_f64<1>(acc, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = acc[0];
+}
+
+// residual: sum |u - u_prev|^2 and sum |u|^2 over bulk cells; u_prev <- u
+template <int S, int CM, bool PERIODIC>
+__global__ void __launch_bounds__(kWG)
+    k_residual(const typename Stor<S>::T* __restrict__ fin, const uint32_t* __restrict__ flags,
+               float* __restrict__ u_prev, DGeom g, double* partials)
+{
+    using R = typename Real<S, CM>::R;
+    __shared__ double lds[32];
+    double acc[2] = {0.0, 0.0};
+    for (uint32_t i = blockIdx.x * kWG + threadIdx.x; i < g.n; i += gridDim.x * kWG) {
+        const uint32_t fl = flags[i];
+        if (cls_of(fl) != NX_CELL_BULK) continue;
+        typename Stor<S>::P f[Q];
+        pull<S, PERIODIC>(fin, g, i, fl, f);
+        R r, uu[3];
+        cell_moments<S, CM>(NX_CELL_BULK, f, r, uu);
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            const float un = (float)uu[a];
+            const double du = (double)un - (double)u_prev[a * g.plane + i];
+            acc[0] += du * du;
+            acc[1] += (double)un * (double)un;
+            u_prev[a * g.plane + i] = un;
+        }
+    }
+    block_sum_f64<2>(acc, lds);
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = acc[0];
+        partials[2 * blockIdx.x + 1] = acc[1];
+    }
+}
+
+int check(const nx_dense_desc* d)
+{
+    if (!d) return set_error(NX_ERR_INVALID, "null descriptor");
+    for (int a = 0; a < 3; a++)
+        if (d->dim[a] < 1) return set_error(NX_ERR_INVALID, "grid dimensions must be >= 1");
+    const uint64_t n = (uint64_t)d->dim[0] * d->dim[1] * d->dim[2];
+    if (n >= (1ull << 32) - kWG) return set_error(NX_ERR_INVALID, "grid too large (>= 2^32 cells)");
+    if (d->storage < NX_F64 || d->storage > NX_F16DEV)
+        return set_error(NX_ERR_INVALID, "unknown storage type");
+    if (d->compute < NX_COMPUTE_F64 || d->compute > NX_COMPUTE_REF)
+        return set_error(NX_ERR_INVALID, "unknown compute mode");
+    if ((d->storage == NX_F32DEV || d->storage == NX_F16DEV) && d->compute != NX_COMPUTE_F32)
+        return set_error(NX_ERR_UNSUPPORTED, "deviation storage requires NX_COMPUTE_F32");
+    if (d->storage == NX_F64 && d->compute == NX_COMPUTE_F32)
+        return set_error(NX_ERR_UNSUPPORTED, "binary64 storage requires binary64 compute");
+    return NX_OK;
+}
+
+inline uint32_t nwg(const DGeom& g) { return (g.n + kWG - 1) / kWG; }
+
+template <int S, int CM, bool FORCE, bool PER>
+void launch_step(const void* fin, void* fout, const uint32_t* flags, const DGeom& g, const Par& p,
+                 hipStream_t s)
+{
+    using T = typename Stor<S>::T;
+    hipLaunchKernelGGL((k_step<S, CM, FORCE, PER>), dim3(nwg(g)), dim3(kWG), 0, s,
+                       (const T*)fin, (T*)fout, flags, g, p);
+}
+
+template <int S, int CM>
+void dispatch_step(bool force, bool per, const void* fin, void* fout, const uint32_t* flags,
+                   const DGeom& g, const Par& p, hipStream_t s)
+{
+    if (force) {
+        if (per) launch_step<S, CM, true, true>(fin, fout, flags, g, p, s);
+        else launch_step<S, CM, true, false>(fin, fout, flags, g, p, s);
+    } else {
+        if (per) launch_step<S, CM, false, true>(fin, fout, flags, g, p, s);
+        else launch_step<S, CM, false, false>(fin, fout, flags, g, p, s);
+    }
+}
+
+// Calls f(integral_constant<S>, integral_constant<CM>) for the (storage,
+// compute) pair of d.
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+template <typename F>
+int by_type(const nx_dense_desc* d, F&& f)
+{
+    switch (d->storage) {
+        case NX_F64: f(IC<NX_F64>{}, IC<NX_COMPUTE_F64>{}); return NX_OK;
+        case NX_F32:
+            if (d->compute == NX_COMPUTE_F64) f(IC<NX_F32>{}, IC<NX_COMPUTE_F64>{});
+            else if (d->compute == NX_COMPUTE_REF) f(IC<NX_F32>{}, IC<NX_COMPUTE_REF>{});
+            else f(IC<NX_F32>{}, IC<NX_COMPUTE_F32>{});
+            return NX_OK;
+        case NX_F32DEV: f(IC<NX_F32DEV>{}, IC<NX_COMPUTE_F32>{}); return NX_OK;
+        case NX_F16DEV: f(IC<NX_F16DEV>{}, IC<NX_COMPUTE_F32>{}); return NX_OK;
+    }
+    return set_error(NX_ERR_INVALID, "unknown storage type");
+}
+
+}  // namespace
+}  // namespace nx
+
+using namespace nx;
+
+extern "C" size_t nx_pop_bytes_dense(const nx_dense_desc* d)
+{
+    if (!d) return 0;
+    const size_t n = (size_t)d->dim[0] * d->dim[1] * d->dim[2] * 19;
+    switch (d->storage) {
+        case NX_F64: return n * 8;
+        case NX_F32:
+        case NX_F32DEV: return n * 4;
+        case NX_F16DEV: return n * 2;
+    }
+    return 0;
+}
+
+extern "C" int nx_cavity_flags_dense(const nx_dense_desc* d, uint32_t* flags, nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!flags) return set_error(NX_ERR_INVALID, "null flags");
+    const DGeom g = make_geom(d);
+    hipLaunchKernelGGL(k_cavity_flags, dim3(nwg(g)), dim3(kWG), 0, (hipStream_t)s, flags, g);
+    NX_CHECK_LAUNCH("k_cavity_flags");
+    return NX_OK;
+}
+
+
+extern "C" int nx_d3q19_fill_dense(const nx_dense_desc* d, void* pop, const uint32_t* flags,
+                                   double ulb, const double ulid[3], nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!pop || !flags || !ulid) return set_error(NX_ERR_INVALID, "null pointer");
+    const DGeom g = make_geom(d);
+    const hipStream_t st = (hipStream_t)s;
+    if (int e = by_type(d, [&](auto sc, auto) {
+            constexpr int S = decltype(sc)::value;
+            hipLaunchKernelGGL((k_fill<S>), dim3(nwg(g)), dim3(kWG), 0, st,
+                               (typename Stor<S>::T*)pop, flags, g, ulb, ulid[0], ulid[1], ulid[2]);
+        }))
+        return e;
+    NX_CHECK_LAUNCH("k_fill");
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_wall_mask_dense(const nx_dense_desc* d, const uint32_t* flags_in,
+                                        uint32_t* flags_out, nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!flags_in || !flags_out) return set_error(NX_ERR_INVALID, "null flags");
+    const DGeom g = make_geom(d);
+    hipLaunchKernelGGL(k_wall_mask, dim3(nwg(g)), dim3(kWG), 0, (hipStream_t)s, flags_in,
+                       flags_out, g);
+    NX_CHECK_LAUNCH("k_wall_mask");
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_step_dense(const nx_dense_desc* d, const void* fin, void* fout,
+                                   const uint32_t* flags, const nx_lbm_params* p, nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!fin || !fout || !flags || !p) return set_error(NX_ERR_INVALID, "null pointer");
+    if (fin == fout)
+        return set_error(NX_ERR_INVALID, "fin and fout must differ (two-population scheme)");
+    const DGeom g = make_geom(d);
+    const Par par{p->omega, {p->force[0], p->force[1], p->force[2]}};
+    const bool force = p->force[0] != 0. || p->force[1] != 0. || p->force[2] != 0.;
+    if (force && d->compute == NX_COMPUTE_REF)
+        return set_error(NX_ERR_UNSUPPORTED,
+                         "body force requires NX_COMPUTE_F32 or NX_COMPUTE_F64");
+    const bool per = d->periodic != 0;
+    const hipStream_t st = (hipStream_t)s;
+    if (int e = by_type(d, [&](auto sc, auto cc) {
+            constexpr int S = decltype(sc)::value;
+            constexpr int CM = decltype(cc)::value;
+            dispatch_step<S, CM>(force, per, fin, fout, flags, g, par, st);
+        }))
+        return e;
+    NX_CHECK_LAUNCH("k_step");
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_moments_dense(const nx_dense_desc* d, const void* fin,
+                                      const uint32_t* flags, void* rho, void* u, nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!fin || !flags || !rho || !u) return set_error(NX_ERR_INVALID, "null pointer");
+    const DGeom g = make_geom(d);
+    const hipStream_t st = (hipStream_t)s;
+    if (int e = by_type(d, [&](auto sc, auto cc) {
+            constexpr int S = decltype(sc)::value;
+            constexpr int CM = decltype(cc)::value;
+            using T = typename Stor<S>::T;
+            if (g.periodic)
+                hipLaunchKernelGGL((k_moments<S, CM, true>), dim3(nwg(g)), dim3(kWG), 0, st,
+                                   (const T*)fin, flags, rho, u, g);
+            else
+                hipLaunchKernelGGL((k_moments<S, CM, false>), dim3(nwg(g)), dim3(kWG), 0, st,
+                                   (const T*)fin, flags, rho, u, g);
+        }))
+        return e;
+    NX_CHECK_LAUNCH("k_moments");
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_mass_dense(const nx_dense_desc* d, const void* fin, const uint32_t* flags,
+                                   double* out, nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!fin || !flags || !out) return set_error(NX_ERR_INVALID, "null pointer");
+    const DGeom g = make_geom(d);
+    const int nb = (int)std::min<uint32_t>(nwg(g), kRedGrid);
+    void* part = nullptr;
+    if (int e = scratch(sizeof(double) * nb, &part)) return e;
+    const hipStream_t st = (hipStream_t)s;
+    if (int e = by_type(d, [&](auto sc, auto) {
+            constexpr int S = decltype(sc)::value;
+            hipLaunchKernelGGL((k_mass<S>), dim3(nb), dim3(kWG), 0, st,
+                               (const typename Stor<S>::T*)fin, flags, g, (double*)part);
+        }))
+        return e;
+    NX_CHECK_LAUNCH("k_mass");
+    return finish_reduce((const double*)part, nb, 1, out, st);
+}
+
+extern "C" int nx_d3q19_residual_dense(const nx_dense_desc* d, const void* fin,
+                                       const uint32_t* flags, float* u_prev, double* out,
+                                       nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!fin || !flags || !u_prev || !out) return set_error(NX_ERR_INVALID, "null pointer");
+    const DGeom g = make_geom(d);
+    const int nb = (int)std::min<uint32_t>(nwg(g), kRedGrid);
+    void* part = nullptr;
+    if (int e = scratch(2 * sizeof(double) * nb, &part)) return e;
+    const hipStream_t st = (hipStream_t)s;
+    if (int e = by_type(d, [&](auto sc, auto cc) {
+            constexpr int S = decltype(sc)::value;
+            constexpr int CM = decltype(cc)::value;
+            using T = typename Stor<S>::T;
+            if (g.periodic)
+                hipLaunchKernelGGL((k_residual<S, CM, true>), dim3(nb), dim3(kWG), 0, st,
+                                   (const T*)fin, flags, u_prev, g, (double*)part);
+            else
+                hipLaunchKernelGGL((k_residual<S, CM, false>), dim3(nb), dim3(kWG), 0, st,
+                                   (const T*)fin, flags, u_prev, g, (double*)part);
+        }))
+        return e;
+    NX_CHECK_LAUNCH("k_residual");
+    return finish_reduce((const double*)part, nb, 2, out, st);
+}

@@ -1,0 +1,368 @@
+"""Python-side plumbing over the C ABI for tests and the benchmark.
+
+Mirrors the objects of the reference's cavity driver
+(benchmarks/lbm-lid-driven-cavity-flow/src/RunCavityTwoPop.cu:17-276 and
+LbmIteration.h:17-101): two population fields that ping-pong, a cell-type
+field with the wall-neighbour mask, and moments / mass / residual read-backs.
+All compute goes through libneon_amd.so; nothing here evaluates the lattice
+Boltzmann step on the host.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import abi
+from .abi import check, lib
+
+_NP_STORAGE = {abi.NX_F64: np.float64, abi.NX_F32: np.float32, abi.NX_F32DEV: np.float32,
+               abi.NX_F16DEV: np.float16}
+
+_initialised: set[int] = set()
+
+
+def init(device: int = 0) -> None:
+    """Select the device (Neon::Backend with one device id)."""
+    if device not in _initialised:
+        check(lib.nx_init(device))
+        _initialised.add(device)
+
+
+def sync() -> None:
+    check(lib.nx_sync())
+
+
+def device_name() -> str:
+    buf = C.create_string_buffer(256)
+    check(lib.nx_device_name(buf, 256))
+    return buf.value.decode()
+
+
+class DeviceBuffer:
+    """Owning device allocation."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib.nx_alloc(self.nbytes, C.byref(p)))
+        self.ptr = p.value or 0
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray, stream=None) -> None:
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes, (a.nbytes, self.nbytes)
+        check(lib.nx_h2d(self.ptr, a.ctypes.data, a.nbytes, stream))
+        check(lib.nx_stream_sync(stream))
+
+    def download(self, dtype, count: int | None = None, stream=None) -> np.ndarray:
+        dtype = np.dtype(dtype)
+        n = self.nbytes // dtype.itemsize if count is None else int(count)
+        out = np.empty(n, dtype=dtype)
+        check(lib.nx_d2h(out.ctypes.data, self.ptr, out.nbytes, stream))
+        check(lib.nx_stream_sync(stream))
+        return out
+
+    def zero(self, stream=None) -> None:
+        check(lib.nx_memset(self.ptr, 0, self.nbytes, stream))
+
+    def free(self) -> None:
+        if self.ptr:
+            lib.nx_free(self.ptr)
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        check(lib.nx_stream_create(C.byref(s)))
+        self.handle = s.value
+
+    def sync(self) -> None:
+        check(lib.nx_stream_sync(self.handle))
+
+    def __del__(self):
+        try:
+            lib.nx_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        check(lib.nx_event_create(C.byref(e)))
+        self.handle = e.value
+
+    def record(self, stream=None) -> None:
+        check(lib.nx_event_record(self.handle, stream.handle if isinstance(stream, Stream) else stream))
+
+    def sync(self) -> None:
+        check(lib.nx_event_sync(self.handle))
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = C.c_float()
+        check(lib.nx_event_elapsed_ms(self.handle, later.handle, C.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            lib.nx_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def params(omega: float, force=(0.0, 0.0, 0.0)) -> abi.LbmParams:
+    p = abi.LbmParams()
+    p.omega = float(omega)
+    for a in range(3):
+        p.force[a] = float(force[a])
+    return p
+
+
+def _storage(s) -> int:
+    return abi.STORAGE_NAMES[s] if isinstance(s, str) else int(s)
+
+
+def _compute(c) -> int:
+    return abi.COMPUTE_NAMES[c] if isinstance(c, str) else int(c)
+
+
+def default_compute(storage: int) -> int:
+    return abi.NX_COMPUTE_F64 if storage == abi.NX_F64 else abi.NX_COMPUTE_F32
+
+
+class _Lattice:
+    """State shared by the dense and block lattices (LbmIterationD3Q19)."""
+
+    storage: int
+    compute: int
+    n_slots: int  # cells (dense) or voxel slots (block)
+
+    def _alloc(self):
+        nbytes = self._pop_bytes()
+        self.pop = [DeviceBuffer(nbytes), DeviceBuffer(nbytes)]
+        self.flags = DeviceBuffer(4 * self.n_slots)
+        self.parity = 0
+        self._out = DeviceBuffer(32)
+
+    # which field holds the current populations (LbmIteration.h:58-69)
+    @property
+    def fin(self) -> DeviceBuffer:
+        return self.pop[self.parity]
+
+    @property
+    def fout(self) -> DeviceBuffer:
+        return self.pop[self.parity ^ 1]
+
+    @property
+    def np_storage(self):
+        return _NP_STORAGE[self.storage]
+
+    @property
+    def moment_dtype(self):
+        return np.float64 if self.storage == abi.NX_F64 else np.float32
+
+    def populations(self) -> np.ndarray:
+        """Raw stored populations of the current field (storage encoding)."""
+        return self.fin.download(self.np_storage, 19 * self.n_slots)
+
+    def set_populations(self, arr: np.ndarray, which: int | None = None) -> None:
+        arr = np.ascontiguousarray(arr, dtype=self.np_storage).reshape(-1)
+        assert arr.size == 19 * self.n_slots
+        if which is None:
+            for b in self.pop:
+                b.upload(arr)
+        else:
+            self.pop[which].upload(arr)
+
+    def get_flags(self) -> np.ndarray:
+        return self.flags.download(np.uint32, self.n_slots)
+
+    def run(self, nsteps: int, omega: float, force=(0.0, 0.0, 0.0), stream=None) -> None:
+        p = params(omega, force)
+        for _ in range(int(nsteps)):
+            self._step(p, stream)
+            self.parity ^= 1
+
+    def mass(self, stream=None) -> float:
+        self._mass(stream)
+        return float(self._out.download(np.float64, 1, stream)[0])
+
+    def residual(self, u_prev: DeviceBuffer, stream=None) -> tuple[float, float]:
+        self._residual(u_prev, stream)
+        r = self._out.download(np.float64, 2, stream)
+        return float(r[0]), float(r[1])
+
+
+class DenseLattice(_Lattice):
+    """D3Q19 on a dense grid (one dGrid partition), SoA layout."""
+
+    def __init__(self, dim, storage="f32", compute=None, periodic: int = 0):
+        init()
+        self.dim = tuple(int(v) for v in dim)
+        self.storage = _storage(storage)
+        self.compute = default_compute(self.storage) if compute is None else _compute(compute)
+        self.periodic = int(periodic)
+        self.desc = abi.DenseDesc()
+        for a in range(3):
+            self.desc.dim[a] = self.dim[a]
+        self.desc.storage, self.desc.compute, self.desc.periodic = (
+            self.storage, self.compute, self.periodic)
+        self.n_slots = self.dim[0] * self.dim[1] * self.dim[2]
+        self._alloc()
+
+    def _pop_bytes(self) -> int:
+        return int(lib.nx_pop_bytes_dense(C.byref(self.desc)))
+
+    def setup_cavity(self, ulb: float, ulid=(1.0, 0.0, 0.0), stream=None) -> None:
+        """RunCavityTwoPop.cu:159-242: classification, populations, wall mask."""
+        check(lib.nx_cavity_flags_dense(C.byref(self.desc), self.flags.ptr, stream))
+        self._fill_and_mask(ulb, ulid, stream)
+
+    def set_classes(self, cls: np.ndarray, ulb: float = 0.0, ulid=(1.0, 0.0, 0.0),
+                    stream=None) -> None:
+        """Arbitrary classification (x fastest), then fill + wall mask."""
+        cls = np.ascontiguousarray(cls, dtype=np.uint32).reshape(-1)
+        assert cls.size == self.n_slots
+        self.flags.upload(cls << abi.NX_CLASS_SHIFT)
+        self._fill_and_mask(ulb, ulid, stream)
+
+    def _fill_and_mask(self, ulb, ulid, stream):
+        lid = (C.c_double * 3)(*[float(v) for v in ulid])
+        for b in self.pop:
+            check(lib.nx_d3q19_fill_dense(C.byref(self.desc), b.ptr, self.flags.ptr, float(ulb),
+                                          lid, stream))
+        check(lib.nx_d3q19_wall_mask_dense(C.byref(self.desc), self.flags.ptr, self.flags.ptr,
+                                           stream))
+        check(lib.nx_stream_sync(stream))
+        self.parity = 0
+
+    def _step(self, p, stream):
+        check(lib.nx_d3q19_step_dense(C.byref(self.desc), self.fin.ptr, self.fout.ptr,
+                                      self.flags.ptr, C.byref(p), stream))
+
+    def moments(self, stream=None):
+        n = self.n_slots
+        it = np.dtype(self.moment_dtype).itemsize
+        rho, u = DeviceBuffer(n * it), DeviceBuffer(3 * n * it)
+        check(lib.nx_d3q19_moments_dense(C.byref(self.desc), self.fin.ptr, self.flags.ptr,
+                                         rho.ptr, u.ptr, stream))
+        X, Y, Z = self.dim
+        r = rho.download(self.moment_dtype, n, stream).reshape(Z, Y, X)
+        uu = u.download(self.moment_dtype, 3 * n, stream).reshape(3, Z, Y, X)
+        return r, uu
+
+    def _mass(self, stream):
+        check(lib.nx_d3q19_mass_dense(C.byref(self.desc), self.fin.ptr, self.flags.ptr,
+                                      self._out.ptr, stream))
+
+    def _residual(self, u_prev, stream):
+        check(lib.nx_d3q19_residual_dense(C.byref(self.desc), self.fin.ptr, self.flags.ptr,
+                                          u_prev.ptr, self._out.ptr, stream))
+
+
+class BlockGrid:
+    """Host bGrid structure (nx_bgrid_create) plus its device tables."""
+
+    def __init__(self, dim, active: np.ndarray, periodic: int = 0):
+        init()
+        self.dim = tuple(int(v) for v in dim)
+        act = np.ascontiguousarray(active, dtype=np.uint8).reshape(-1)
+        assert act.size == self.dim[0] * self.dim[1] * self.dim[2]
+        d = (C.c_int32 * 3)(*self.dim)
+        h = C.c_void_p()
+        check(lib.nx_bgrid_create(d, act.ctypes.data, int(periodic), C.byref(h)))
+        try:
+            nb, na = C.c_uint32(), C.c_uint64()
+            check(lib.nx_bgrid_info(h, C.byref(nb), C.byref(na)))
+            self.n_blocks, self.n_active = int(nb.value), int(na.value)
+            self.origins = np.empty(3 * self.n_blocks, np.int32)
+            self.masks = np.empty(16 * self.n_blocks, np.uint32)
+            self.conn = np.empty(27 * self.n_blocks, np.uint32)
+            check(lib.nx_bgrid_tables(h, self.origins.ctypes.data, self.masks.ctypes.data,
+                                      self.conn.ctypes.data))
+        finally:
+            lib.nx_bgrid_destroy(h)
+        self.periodic = int(periodic)
+        self.d_origins = DeviceBuffer.from_array(self.origins)
+        self.d_masks = DeviceBuffer.from_array(self.masks)
+        self.d_conn = DeviceBuffer.from_array(self.conn)
+
+
+class BlockLattice(_Lattice):
+    """D3Q19 on the block-sparse grid (one bGrid<8,8,8> partition), AoSoA layout."""
+
+    def __init__(self, grid: BlockGrid, storage="f32", compute=None):
+        self.grid = grid
+        self.dim = grid.dim
+        self.storage = _storage(storage)
+        self.compute = default_compute(self.storage) if compute is None else _compute(compute)
+        self.desc = abi.BlockDesc()
+        for a in range(3):
+            self.desc.dim[a] = self.dim[a]
+        self.desc.storage, self.desc.compute, self.desc.periodic = (
+            self.storage, self.compute, grid.periodic)
+        self.desc.n_blocks = grid.n_blocks
+        self.desc.origins, self.desc.masks, self.desc.conn = (
+            grid.d_origins.ptr, grid.d_masks.ptr, grid.d_conn.ptr)
+        self.n_slots = grid.n_blocks * 512
+        self._alloc()
+
+    def _pop_bytes(self) -> int:
+        return int(lib.nx_pop_bytes_block(C.byref(self.desc)))
+
+    def setup_cavity(self, ulb: float, ulid=(1.0, 0.0, 0.0), stream=None) -> None:
+        check(lib.nx_cavity_flags_block(C.byref(self.desc), self.flags.ptr, stream))
+        self._fill_and_mask(ulb, ulid, stream)
+
+    def set_classes(self, cls_slots: np.ndarray, ulb: float = 0.0, ulid=(1.0, 0.0, 0.0),
+                    stream=None) -> None:
+        """Classification per voxel slot (b*512 + v), then fill + wall mask."""
+        cls = np.ascontiguousarray(cls_slots, dtype=np.uint32).reshape(-1)
+        assert cls.size == self.n_slots
+        self.flags.upload(cls << abi.NX_CLASS_SHIFT)
+        self._fill_and_mask(ulb, ulid, stream)
+
+    def _fill_and_mask(self, ulb, ulid, stream):
+        lid = (C.c_double * 3)(*[float(v) for v in ulid])
+        for b in self.pop:
+            check(lib.nx_d3q19_fill_block(C.byref(self.desc), b.ptr, self.flags.ptr, float(ulb),
+                                          lid, stream))
+        check(lib.nx_d3q19_wall_mask_block(C.byref(self.desc), self.flags.ptr, self.flags.ptr,
+                                           stream))
+        check(lib.nx_stream_sync(stream))
+        self.parity = 0
+
+    def _step(self, p, stream):
+        check(lib.nx_d3q19_step_block(C.byref(self.desc), self.fin.ptr, self.fout.ptr,
+                                      self.flags.ptr, C.byref(p), stream))
+
+    def moments(self, stream=None):
+        n = self.n_slots
+        it = np.dtype(self.moment_dtype).itemsize
+        rho, u = DeviceBuffer(n * it), DeviceBuffer(3 * n * it)
+        check(lib.nx_d3q19_moments_block(C.byref(self.desc), self.fin.ptr, self.flags.ptr,
+                                         rho.ptr, u.ptr, stream))
+        return rho.download(self.moment_dtype, n, stream), u.download(self.moment_dtype, 3 * n,
+                                                                       stream)
+
+    def _mass(self, stream):
+        check(lib.nx_d3q19_mass_block(C.byref(self.desc), self.fin.ptr, self.flags.ptr,
+                                      self._out.ptr, stream))
+
+    def _residual(self, u_prev, stream):
+        check(lib.nx_d3q19_residual_block(C.byref(self.desc), self.fin.ptr, self.flags.ptr,
+                                          u_prev.ptr, self._out.ptr, stream))

@@ -1,0 +1,159 @@
+"""GPU parity of the dense (dGrid) D3Q19 path against the CPU oracle.
+
+Populations are compared bit for bit: the kernels evaluate the oracle's
+expression tree with no FMA contraction and the same IEEE conversions, so
+every storage / compute combination is expected to agree exactly.  The
+north-star bound for binary64 (1e-12 relative after 1000 steps on the 64^3
+cavity) is asserted as well, and size-independent properties are checked at
+the benchmark size (256^3).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+COMBOS = [("f64", "f64", 0, 0), ("f32", "f32", 1, 1), ("f32", "ref", 1, 2), ("f32", "f64", 1, 0),
+          ("f32dev", "f32", 2, 1), ("f16dev", "f32", 3, 1)]
+
+
+def _pair(nx, orc, dim, st, cm, sto, cmo, periodic=0):
+    g = nx.DenseLattice(dim, st, cm, periodic=periodic)
+    o = orc.DenseOracle(dim, sto, cmo, periodic=periodic)
+    return g, o
+
+
+def _raw(o):
+    return o.fin.view(np.uint16) if o.d.storage == 3 else o.fin
+
+
+def _graw(g):
+    p = g.populations()
+    return p.view(np.uint16) if g.storage == 3 else p
+
+
+@pytest.mark.parametrize("st,cm,sto,cmo", COMBOS)
+def test_cavity_bit_exact(nx, orc, st, cm, sto, cmo):
+    dim = (13, 11, 12)
+    g, o = _pair(nx, orc, dim, st, cm, sto, cmo)
+    g.setup_cavity(0.05)
+    o.setup_cavity(0.05)
+    assert np.array_equal(g.get_flags(), o.flags)
+    assert np.array_equal(_graw(g), _raw(o))
+    g.run(17, 1.72)
+    o.run(17, 1.72)
+    assert np.array_equal(_graw(g), _raw(o))
+    rg, ug = g.moments()
+    ro, uo = o.moments()
+    assert np.array_equal(rg, ro) and np.array_equal(ug, uo)
+
+
+@pytest.mark.parametrize("st,cm,sto,cmo", [c for c in COMBOS if c[1] != "ref"])
+def test_obstacles_periodic_forced_bit_exact(nx, orc, st, cm, sto, cmo):
+    """Random solids (bounce-back + undefined), moving-wall cells inside the
+    domain, periodic y, bulk cells on the open x/z faces (NX_FLAG_OPEN links),
+    Guo body force."""
+    dim = (14, 9, 10)
+    X, Y, Z = dim
+    rng = np.random.default_rng(1)
+    cls = np.full((Z, Y, X), 2, np.uint32)
+    r = rng.random((Z, Y, X))
+    cls[r < 0.12] = 0
+    cls[(r >= 0.12) & (r < 0.16)] = 1
+    cls[(r >= 0.16) & (r < 0.2)] = 3
+    g, o = _pair(nx, orc, dim, st, cm, sto, cmo, periodic=2)
+    g.set_classes(cls.reshape(-1), ulb=0.03, ulid=(0.3, 1.0, -0.5))
+    o.set_classes(cls.reshape(-1), ulb=0.03, ulid=(0.3, 1.0, -0.5))
+    fl = g.get_flags()
+    assert np.array_equal(fl, o.flags)
+    assert ((fl & (1 << 28)) != 0).sum() > 10
+    force = (1e-4, -3e-5, 2e-5)
+    g.run(9, 1.3, force=force)
+    o.run(9, 1.3, force=force)
+    assert np.array_equal(_graw(g), _raw(o))
+    g.run(4, 0.9)
+    o.run(4, 0.9)
+    assert np.array_equal(_graw(g), _raw(o))
+
+
+def test_config1_64_f64_1000_steps(nx, orc):
+    """64^3 cavity incl. wall layer, FP64, lid u=(0.05,0,0), Re=100 with
+    nu = U N / Re, 1000 steps: all bulk populations and rho/u within 1e-12
+    relative of the oracle (bit-exact expected)."""
+    N, ulb, re, steps = 64, 0.05, 100.0, 1000
+    omega = 1.0 / (3.0 * ulb * N / re + 0.5)
+    g, o = _pair(nx, orc, (N, N, N), "f64", "f64", 0, 0)
+    g.setup_cavity(ulb)
+    o.setup_cavity(ulb)
+    g.run(steps, omega)
+    o.run(steps, omega)
+    pg, po = g.populations().reshape(19, -1), o.fin.reshape(19, -1)
+    bulk = ((o.flags >> 24) & 3) == 2
+    rel = np.abs(pg[:, bulk] - po[:, bulk]).max() / np.abs(po[:, bulk]).max()
+    print(f"config1: max rel diff populations {rel:.3e}, bit-exact {np.array_equal(pg, po)}")
+    assert rel <= 1e-12
+    rg, ug = g.moments()
+    ro, uo = o.moments()
+    assert np.abs(ug - uo).max() <= 1e-12 * np.abs(uo).max()
+    assert np.abs(rg - ro).max() <= 1e-12
+
+
+def test_mass_and_residual_match_oracle(nx, orc):
+    N = 24
+    g, o = _pair(nx, orc, (N, N, N), "f64", "f64", 0, 0)
+    g.setup_cavity(0.05)
+    o.setup_cavity(0.05)
+    g.run(30, 1.6)
+    o.run(30, 1.6)
+    assert abs(g.mass() - o.mass()) <= 1e-13 * o.mass()
+    up_g = nx.DeviceBuffer(3 * N ** 3 * 4)
+    up_g.zero()
+    up_o = np.zeros(3 * N ** 3, np.float32)
+    a = g.residual(up_g)
+    b = o.residual(up_o)
+    g.run(5, 1.6)
+    o.run(5, 1.6)
+    a = g.residual(up_g)
+    b = o.residual(up_o)
+    assert np.allclose(a, b, rtol=1e-12, atol=0)
+    assert np.array_equal(up_g.download(np.float32), up_o)
+
+
+def test_full_size_256_f32_properties(nx):
+    """256^3 FP32 (the benchmark case): mass is conserved and the z-mirror
+    symmetry of the cavity holds to FP32 round-off."""
+    N = 256
+    g = nx.DenseLattice((N, N, N), "f32", "f32")
+    g.setup_cavity(0.05)
+    m0 = g.mass()
+    omega = 1.0 / (3.0 * 0.05 * N / 1000.0 + 0.5)
+    g.run(200, omega)
+    m1 = g.mass()
+    assert abs(m1 - m0) / m0 < 2e-6, (m0, m1)
+    _, u = g.moments()
+    ux = u[0].astype(np.float64)
+    uz = u[2].astype(np.float64)
+    scale = np.abs(ux).max()
+    assert scale > 1e-3
+    assert np.abs(ux - ux[::-1]).max() < 1e-4 * scale
+    assert np.abs(uz + uz[::-1]).max() < 1e-4 * scale
+    assert np.isfinite(ux).all()
+
+
+def test_fp16_dev_velocity_error_vs_f64(nx):
+    """FP16 deviation storage stays within 1e-3 relative (of max |u|) of the
+    FP64 run on a 64^3 cavity after 2000 steps."""
+    N, steps = 64, 2000
+    omega = 1.0 / (3.0 * 0.05 * N / 100.0 + 0.5)
+    ref = nx.DenseLattice((N, N, N), "f64")
+    ref.setup_cavity(0.05)
+    ref.run(steps, omega)
+    _, u64 = ref.moments()
+    errs = {}
+    for st in ("f32", "f32dev", "f16dev"):
+        g = nx.DenseLattice((N, N, N), st, "f32")
+        g.setup_cavity(0.05)
+        g.run(steps, omega)
+        _, u = g.moments()
+        errs[st] = float(np.abs(u.astype(np.float64) - u64).max() / np.abs(u64).max())
+    print("velocity error vs FP64:", errs)
+    assert errs["f32"] < 1e-4 and errs["f32dev"] < 1e-4 and errs["f16dev"] < 1e-3

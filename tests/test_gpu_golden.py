@@ -1,0 +1,57 @@
+"""The GPU binary64 dense path against the reference's own output file.
+
+tests/golden/NeonUniformLBM_20000_Y.dat is the centre-line u_x(y) at
+x = z = N/2 that the reference's cavity benchmark wrote
+(benchmarks/lbm-lid-driven-cavity-flow/src/RunCavityTwoPop.cu:110-149) for
+N = 160, Re = 100, ulb = 0.04, double storage/compute, after 20000 steps,
+printed with 6 significant digits.
+"""
+import pathlib
+
+import numpy as np
+import pytest
+
+GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
+N, RE, ULB, STEPS = 160, 100.0, 0.04, 20000
+
+
+def load_dat():
+    d = np.loadtxt(GOLDEN / "NeonUniformLBM_20000_Y.dat")
+    assert d.shape == (N, 2)
+    return d
+
+
+def compare(y, ux, dat, rel=6e-6, abs_=1e-11):
+    assert np.allclose(y, dat[:, 0], rtol=1e-5, atol=0)
+    err = np.abs(ux - dat[:, 1])
+    tol = rel * np.abs(dat[:, 1]) + abs_
+    bad = np.nonzero(err > tol)[0]
+    return err, bad
+
+
+@pytest.mark.gpu
+def test_gpu_f64_matches_reference_centerline(nx):
+    nu = ULB * (N - 2) / RE
+    omega = 1.0 / (3.0 * nu + 0.5)
+    g = nx.DenseLattice((N, N, N), "f64", "f64")
+    g.setup_cavity(ULB)
+    g.run(STEPS, omega)
+    _, u = g.moments()
+    ux = u[0][N // 2, :, N // 2].astype(np.float64)
+    y = np.arange(N) / N
+    dat = load_dat()
+    err, bad = compare(y, ux, dat)
+    print(f"max |u - dat| = {err.max():.3e}; worst rel = "
+          f"{(err / np.maximum(np.abs(dat[:, 1]), 1e-30)).max():.3e}")
+    assert bad.size == 0, [(int(i), ux[i], dat[i, 1]) for i in bad[:5]]
+
+
+def test_oracle_matches_reference_centerline():
+    """The oracle's own 20000-step run (tests/golden/make_centerline.py)."""
+    p = GOLDEN / "oracle_centerline_160_20000.txt"
+    if not p.exists():
+        pytest.skip("oracle centre-line fixture not generated (tests/golden/make_centerline.py)")
+    o = np.loadtxt(p)
+    dat = load_dat()
+    err, bad = compare(o[:, 0], o[:, 1], dat)
+    assert bad.size == 0, [(int(i), o[i, 1], dat[i, 1]) for i in bad[:5]]
