@@ -69,6 +69,7 @@ extern "C" {
 typedef struct nx_stream_s* nx_stream;
 typedef struct nx_event_s* nx_event;
 typedef struct nx_bgrid_s* nx_bgrid;
+typedef struct nx_container_s* nx_container;
 
 /* Dense grid: one partition of a dGrid (dGrid_imp.h:7-185). */
 typedef struct {
@@ -97,6 +98,15 @@ typedef struct {
     double force[3]; /* body acceleration g (Guo forcing); 0 = none     */
 } nx_lbm_params;
 
+/* Launch configuration of the container's interior kernel. */
+typedef struct {
+    int32_t wg;          /* threads per workgroup: 256 or 512             */
+    int32_t cpt;         /* cells per thread: 1 or 2                      */
+    int32_t load_policy; /* 0: cached loads, 1: non-temporal loads        */
+    int32_t xcd_remap;   /* 1: contiguous slab of the grid per XCD        */
+    int32_t vx;          /* cells per lane along x: 1                     */
+} nx_tuning;
+
 /* ---- runtime (replaces libNeonSys GpuDevice / Backend, GpuDevice.h:151-190,
  *      src/set/Backend.cpp:46-55,404-419) -------------------------------- */
 int nx_init(int device);
@@ -121,6 +131,9 @@ int nx_h2d(void* dst, const void* src, size_t bytes, nx_stream s);
 int nx_d2h(void* dst, const void* src, size_t bytes, nx_stream s);
 int nx_d2d(void* dst, const void* src, size_t bytes, nx_stream s);
 int nx_host_alloc(size_t bytes, void** host); /* pinned */
+/* Plain device-to-device streaming copy kernel (16-byte aligned); no
+ * counterpart in the reference: the achievable-bandwidth yardstick. */
+int nx_copy_kernel(void* dst, const void* src, size_t bytes, nx_stream s);
 int nx_host_free(void* host);
 
 /* Bytes of one population field (19 cardinalities) for a storage type. */
@@ -150,6 +163,33 @@ int nx_d3q19_step_dense(const nx_dense_desc* d, const void* fin, void* fout,
                         const uint32_t* flags, const nx_lbm_params* p, nx_stream s);
 int nx_d3q19_moments_dense(const nx_dense_desc* d, const void* fin,
                            const uint32_t* flags, void* rho, void* u, nx_stream s);
+
+/* Container: LbmContainers::iteration (LbmTools.h:284-325) as a launch plan
+ * bound to (fin, fout, flags, params), run once per time step like
+ * Neon::set::Container::run (Containter.h:25-35).  Creation classifies the
+ * bulk cells once (interior bitmap, held by the container); the flags must
+ * not change while the container is alive.  A run is one kernel on the stream
+ * and produces exactly the populations of nx_d3q19_step_dense.  When both
+ * fields hold identical values in every non-bulk cell (as the reference's
+ * set-up leaves them, RunCavityTwoPop.cu:168-206) the kernel also writes
+ * those unchanged wall values into fout, so every cache line is written
+ * whole; info reports this as copies_walls.  The two fields of a ping-pong
+ * pair take one container each (LbmIteration.h:41-42). */
+int nx_d3q19_iteration_dense(const nx_dense_desc* d, const void* fin, void* fout,
+                             const uint32_t* flags, const nx_lbm_params* p, nx_container* c);
+int nx_container_run(nx_container c, nx_stream s);
+int nx_container_info(nx_container c, uint64_t* n_interior, uint64_t* n_boundary,
+                      int* copies_walls);
+int nx_container_destroy(nx_container c);
+
+/* Launch configuration of the container's interior kernel, per (storage,
+ * compute) pair, process-wide.  No counterpart in the reference (its launch
+ * shape is fixed: dGrid_imp.h:16,27).  autotune times every configuration
+ * on the container's buffers (fout is overwritten, fin only read), keeps the
+ * fastest and returns it with its time per step. */
+int nx_d3q19_get_tuning_dense(const nx_dense_desc* d, nx_tuning* t);
+int nx_d3q19_set_tuning_dense(const nx_dense_desc* d, const nx_tuning* t);
+int nx_container_autotune(nx_container c, nx_stream s, nx_tuning* best, float* best_ms);
 
 /* FP64 reductions on the GPU (wave64 DPP + two-level tree; deterministic).
  * mass: out[0] = sum over bulk cells of sum_q f_q  (device double*).

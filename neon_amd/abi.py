@@ -1,7 +1,7 @@
 """ctypes binding of the C ABI declared in include/neon_amd.h.
 
 The shared library neon_amd/lib/libneon_amd.so must have been built
-(``python -m neon_amd.build``); there is no fallback: importing this module
+(``python neon_amd/build.py``); there is no fallback: importing this module
 without it raises ImportError.
 """
 from __future__ import annotations
@@ -44,6 +44,14 @@ class BlockDesc(C.Structure):
                 ("origins", C.c_void_p), ("masks", C.c_void_p), ("conn", C.c_void_p)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("wg", C.c_int32), ("cpt", C.c_int32), ("load_policy", C.c_int32),
+                ("xcd_remap", C.c_int32), ("vx", C.c_int32)]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
 class LbmParams(C.Structure):
     _fields_ = [("omega", C.c_double), ("force", C.c_double * 3)]
 
@@ -81,6 +89,15 @@ SIGNATURES = {
     "nx_d2d": (_I, [_P, _P, _SZ, _P]),
     "nx_host_alloc": (_I, [_SZ, C.POINTER(_P)]),
     "nx_host_free": (_I, [_P]),
+    "nx_copy_kernel": (_I, [_P, _P, _SZ, _P]),
+    "nx_d3q19_get_tuning_dense": (_I, [_DP, C.POINTER(Tuning)]),
+    "nx_d3q19_set_tuning_dense": (_I, [_DP, C.POINTER(Tuning)]),
+    "nx_d3q19_iteration_dense": (_I, [_DP, _P, _P, _P, _PP, C.POINTER(_P)]),
+    "nx_container_run": (_I, [_P, _P]),
+    "nx_container_info": (_I, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                               C.POINTER(C.c_int)]),
+    "nx_container_destroy": (_I, [_P]),
+    "nx_container_autotune": (_I, [_P, _P, C.POINTER(Tuning), C.POINTER(C.c_float)]),
     "nx_pop_bytes_dense": (_SZ, [_DP]),
     "nx_pop_bytes_block": (_SZ, [_BP]),
     "nx_cavity_flags_dense": (_I, [_DP, _P, _P]),
@@ -108,7 +125,7 @@ SIGNATURES = {
 def _load() -> C.CDLL:
     if not LIB_PATH.exists():
         raise ImportError(
-            f"{LIB_PATH} is not built; run `python -m neon_amd.build` (HIP library for gfx950)")
+            f"{LIB_PATH} is not built; run `python neon_amd/build.py` (HIP library for gfx950)")
     lib = C.CDLL(str(LIB_PATH))
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -1,6 +1,6 @@
 """Build libneon_amd.so (HIP kernels for gfx950 + the C ABI) in-tree.
 
-    python -m neon_amd.build            # compile every source, link the library
+    python neon_amd/build.py            # compile every source, link the library
 
 Each translation unit is compiled separately (in parallel) with hipcc and the
 objects are linked into neon_amd/lib/libneon_amd.so.  Flags:

@@ -1,0 +1,431 @@
+// d3q19_container.hip -- the dense LBM step as a reusable launch plan.
+//
+// The counterpart of LbmContainers::iteration (benchmarks/
+// lbm-lid-driven-cavity-flow/src/LbmTools.h:284-325), which returns a
+// Neon::set::Container that each time step runs (Containter.h:25-35,
+// LbmIteration.h:56-61).  The container binds (fin, fout, flags, omega, g)
+// and, once at creation, builds a 1-bit-per-cell bitmap of the interior
+// cells -- bulk cells with no wall link whose 18 upstream cells need no
+// periodic wrap -- with a wave64 ballot per 64 cells.
+//
+// A step is one launch of k_fused on the caller's stream:
+//   phase 1  interior lanes: the 19 population loads are issued before the
+//            bitmap test (pinned ahead of the branch; their addresses do not
+//            depend on per-cell data), then collide and store.  Addresses are
+//            a uniform per-plane base plus a 32-bit lane offset.
+//   other    the remaining lanes of the same wavefront: flag word, then for
+//            bulk cells the full pull (wall links, moving-wall term, missing
+//            upstream cells, periodic wrap) and collide; wall cells take
+//            their own value from fin.  Its loads hit the cache lines the
+//            interior lanes of the wavefront fetched.
+// Every lane of a wavefront stores all 19 populations at once.  Leaving the
+// wall lanes unwritten makes the first and last cache line of every grid row
+// a partial write, which HBM serves as read-modify-write (measured on
+// MI355X: 27% slower at 256^3).  Wall cells are never changed by a step, so
+// copying fin's wall values is exact when both fields hold the same wall
+// values -- the reference initialises both fields identically
+// (RunCavityTwoPop.cu:168-206).  The container checks this once at
+// creation; if any wall value differs it leaves wall cells unwritten.
+// Both paths evaluate the same expression tree as nx_d3q19_step_dense, so
+// the populations agree bit for bit.
+#include <vector>
+
+#include "dense_common.hpp"
+
+struct nx_container_s {
+    nx_dense_desc d;
+    nx::dense::DGeom g;
+    const void* fin;
+    void* fout;
+    const uint32_t* flags;
+    nx::dense::Par p;
+    bool force;
+    uint32_t* interior = nullptr;  // device bitmap, ceil(n/32) words
+    uint64_t n_boundary = 0;       // bulk cells on the full-pull path
+    uint64_t n_interior = 0;
+    int copy_walls = 0;            // 1: fin and fout wall values are identical
+};
+
+namespace nx {
+namespace dense {
+namespace {
+
+constexpr int kCWG = 256;  // classification workgroup (4 wavefronts)
+
+__device__ __forceinline__ bool is_interior(const DGeom& g, uint32_t i, uint32_t fl)
+{
+    if (cls_of(fl) != NX_CELL_BULK || (fl & (NX_MASK_BITS | NX_FLAG_OPEN))) return false;
+    if (!g.periodic) return true;
+    int x, y, z;
+    coords(g, i, x, y, z);
+    if ((g.periodic & NX_PERIODIC_X) && (x == 0 || x == g.X - 1)) return false;
+    if ((g.periodic & NX_PERIODIC_Y) && (y == 0 || y == g.Y - 1)) return false;
+    if ((g.periodic & NX_PERIODIC_Z) && (z == 0 || z == g.Z - 1)) return false;
+    return true;
+}
+
+// Interior bitmap (one ballot word pair per wavefront), and the numbers of
+// interior and other bulk cells per workgroup.
+__global__ void __launch_bounds__(kCWG)
+    k_classify(const uint32_t* __restrict__ flags, DGeom g, uint32_t* __restrict__ bits,
+               uint32_t* __restrict__ wg_count, uint64_t* __restrict__ wg_interior)
+{
+    __shared__ uint32_t cnt[kCWG / 64];
+    __shared__ uint32_t icnt[kCWG / 64];
+    const uint32_t i = blockIdx.x * kCWG + threadIdx.x;
+    const uint32_t fl = i < g.n ? flags[i] : 0u;
+    const bool in = i < g.n && is_interior(g, i, fl);
+    const bool bd = i < g.n && cls_of(fl) == NX_CELL_BULK && !in;
+    const uint64_t bi = __ballot(in);
+    const uint64_t bb = __ballot(bd);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // wavefront w of workgroup b covers cells 64*(4b+w) .. +63: words 2*(4b+w), +1
+    const uint32_t word = 2u * (blockIdx.x * (kCWG / 64) + wave);
+    if (lane == 0 && word < (g.n + 31) / 32) bits[word] = (uint32_t)bi;
+    if (lane == 1 && word + 1 < (g.n + 31) / 32) bits[word + 1] = (uint32_t)(bi >> 32);
+    if (lane == 0) {
+        cnt[wave] = (uint32_t)__popcll(bb);
+        icnt[wave] = (uint32_t)__popcll(bi);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t c = 0, ic = 0;
+        for (int w = 0; w < kCWG / 64; w++) {
+            c += cnt[w];
+            ic += icnt[w];
+        }
+        wg_count[blockIdx.x] = c;
+        wg_interior[blockIdx.x] = ic;
+    }
+}
+
+// Load of element j of a population plane: uniform 64-bit plane base plus a
+// 32-bit byte offset per lane (global_load saddr form; planes < 4 GiB).
+template <int LP, typename T>
+__device__ __forceinline__ T ld_plane(const T* plane_base, uint32_t j)
+{
+    return ldp<LP>((const T*)((const char*)plane_base + (uint32_t)(j * (uint32_t)sizeof(T))));
+}
+
+template <typename T>
+__device__ __forceinline__ void st_plane(T* plane_base, uint32_t j, T v)
+{
+    __builtin_nontemporal_store(v, (T*)((char*)plane_base + (uint32_t)(j * (uint32_t)sizeof(T))));
+}
+
+// 1 in *mismatch when a non-bulk cell holds different values in a and b.
+template <typename T>
+__global__ void __launch_bounds__(kCWG)
+    k_walls_equal(const T* __restrict__ a, const T* __restrict__ b,
+                  const uint32_t* __restrict__ flags, DGeom g, uint32_t* mismatch)
+{
+    const uint32_t i = blockIdx.x * kCWG + threadIdx.x;
+    bool diff = false;
+    if (i < g.n && cls_of(flags[i]) != NX_CELL_BULK)
+        for (int q = 0; q < Q; q++) {
+            using U = typename std::conditional<
+                sizeof(T) == 8, uint64_t,
+                typename std::conditional<sizeof(T) == 4, uint32_t, uint16_t>::type>::type;
+            diff = diff || __builtin_bit_cast(U, a[q * g.plane + i]) !=
+                               __builtin_bit_cast(U, b[q * g.plane + i]);
+        }
+    if (__ballot(diff) && (threadIdx.x & 63) == 0) mismatch[blockIdx.x] = 1u;
+}
+
+// WG threads, CPT cells per thread (cells base + c*WG), LP = load policy
+// (1: non-temporal), copy_walls as above.
+template <int S, int CM, bool FORCE, bool PER, int WG, int CPT, int LP>
+__global__ void __launch_bounds__(WG)
+    k_fused(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
+            const uint32_t* __restrict__ flags, const uint32_t* __restrict__ bits, DGeom g, Par p,
+            int remap, int copy_walls)
+{
+    using T = typename Stor<S>::T;
+    using P = typename Stor<S>::P;
+    using R = typename Real<S, CM>::R;
+    const uint32_t wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t base = wg * (uint32_t)(WG * CPT) + threadIdx.x;
+    uint32_t word[CPT];
+    T v[CPT][Q];
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const uint32_t ic = min(base + (uint32_t)(c * WG), g.n - 1u);
+        word[c] = bits[ic >> 5];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
+            const uint32_t j = q == 9 ? ic : min(ic - (uint32_t)off, g.n - 1u);
+            v[c][q] = ld_plane<LP>(fin + q * g.plane, j);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < CPT; c++)
+#pragma unroll
+        for (int q = 0; q < Q; q++) pin(v[c][q]);
+#pragma unroll
+    for (int c = 0; c < CPT; c++) {
+        const uint32_t i = base + (uint32_t)(c * WG);
+        if (i >= g.n) continue;
+        bool store = true;
+        if ((word[c] >> (i & 31u)) & 1u) {
+            P f[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) f[q] = Stor<S>::bulk(v[c][q], q);
+            R out[Q];
+            collide<S, CM, FORCE>(f, p.omega, p.g, out);
+#pragma unroll
+            for (int q = 0; q < Q; q++) v[c][q] = Stor<S>::template enc<R>(out[q], q);
+        } else {
+            const uint32_t fl = flags[i];
+            if (cls_of(fl) == NX_CELL_BULK) {
+                P f[Q];
+                pull<S, PER>(fin, g, i, fl, f);
+                R out[Q];
+                collide<S, CM, FORCE>(f, p.omega, p.g, out);
+#pragma unroll
+                for (int q = 0; q < Q; q++) v[c][q] = Stor<S>::template enc<R>(out[q], q);
+            } else if (copy_walls) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) v[c][q] = fin[q * g.plane + i];
+            } else {
+                store = false;
+            }
+        }
+        if (store) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) st_plane(fout + q * g.plane, i, v[c][q]);
+        }
+    }
+}
+
+// Launch configuration of k_fused per (storage, compute), process-wide.
+nx_tuning g_tuning[4][3] = {
+    {{512, 1, 0, 0, 1}, {256, 1, 0, 0, 1}, {256, 1, 0, 0, 1}},
+    {{256, 1, 0, 0, 1}, {512, 1, 1, 0, 1}, {256, 1, 0, 0, 1}},
+    {{256, 1, 0, 0, 1}, {512, 1, 1, 0, 1}, {256, 1, 0, 0, 1}},
+    {{256, 1, 0, 0, 1}, {512, 1, 1, 0, 1}, {256, 1, 0, 0, 1}},
+};
+
+template <int S, int CM>
+constexpr bool tunable()
+{
+    return (S == NX_F64 && CM == NX_COMPUTE_F64) || (S != NX_F64 && CM == NX_COMPUTE_F32);
+}
+
+bool valid_tuning(const nx_tuning& t)
+{
+    return (t.wg == 256 || t.wg == 512) && (t.cpt == 1 || t.cpt == 2) &&
+           (t.load_policy == 0 || t.load_policy == 1) && (t.xcd_remap == 0 || t.xcd_remap == 1) &&
+           t.vx == 1;
+}
+
+template <int S, int CM, bool FORCE, bool PER>
+void launch_fused(const nx_tuning& t, const nx_container_s& c, hipStream_t s)
+{
+    using T = typename Stor<S>::T;
+    auto go = [&](auto wgc, auto cptc, auto lpc) {
+        constexpr int WG = decltype(wgc)::value, CPT = decltype(cptc)::value,
+                      LP = decltype(lpc)::value;
+        const uint32_t nb = (c.g.n + WG * CPT - 1) / (WG * CPT);
+        hipLaunchKernelGGL((k_fused<S, CM, FORCE, PER, WG, CPT, LP>), dim3(nb), dim3(WG), 0, s,
+                           (const T*)c.fin, (T*)c.fout, c.flags, c.interior, c.g, c.p,
+                           t.xcd_remap, c.copy_walls);
+    };
+    if constexpr (tunable<S, CM>() && !FORCE && !PER) {
+        auto by_lp = [&](auto wgc, auto cptc) {
+            if (t.load_policy) go(wgc, cptc, IC<1>{});
+            else go(wgc, cptc, IC<0>{});
+        };
+        auto by_cpt = [&](auto wgc) {
+            if (t.cpt == 2) by_lp(wgc, IC<2>{});
+            else by_lp(wgc, IC<1>{});
+        };
+        if (t.wg == 512) by_cpt(IC<512>{});
+        else by_cpt(IC<256>{});
+    } else {
+        go(IC<256>{}, IC<1>{}, IC<0>{});
+    }
+}
+
+int run_container(const nx_container_s& c, const nx_tuning& t, hipStream_t s)
+{
+    if (int e = by_type(&c.d, [&](auto sc, auto cc) {
+            constexpr int S = decltype(sc)::value;
+            constexpr int CM = decltype(cc)::value;
+            const bool per = c.g.periodic != 0;
+            if (c.force) {
+                if (per) launch_fused<S, CM, true, true>(t, c, s);
+                else launch_fused<S, CM, true, false>(t, c, s);
+            } else {
+                if (per) launch_fused<S, CM, false, true>(t, c, s);
+                else launch_fused<S, CM, false, false>(t, c, s);
+            }
+        }))
+        return e;
+    NX_CHECK_LAUNCH("k_fused");
+    return NX_OK;
+}
+
+}  // namespace
+}  // namespace dense
+}  // namespace nx
+
+using namespace nx;
+using namespace nx::dense;
+
+extern "C" int nx_d3q19_iteration_dense(const nx_dense_desc* d, const void* fin, void* fout,
+                                        const uint32_t* flags, const nx_lbm_params* p,
+                                        nx_container* out)
+{
+    if (int e = check(d)) return e;
+    if (!fin || !fout || !flags || !p || !out) return set_error(NX_ERR_INVALID, "null pointer");
+    if ((uint64_t)d->dim[0] * d->dim[1] * d->dim[2] * 8 >= (1ull << 32))
+        return set_error(NX_ERR_UNSUPPORTED, "container: population planes must be < 4 GiB");
+    if (fin == fout)
+        return set_error(NX_ERR_INVALID, "fin and fout must differ (two-population scheme)");
+    const bool force = p->force[0] != 0. || p->force[1] != 0. || p->force[2] != 0.;
+    if (force && d->compute == NX_COMPUTE_REF)
+        return set_error(NX_ERR_UNSUPPORTED,
+                         "body force requires NX_COMPUTE_F32 or NX_COMPUTE_F64");
+    auto* c = new nx_container_s;
+    c->d = *d;
+    c->g = make_geom(d);
+    c->fin = fin;
+    c->fout = fout;
+    c->flags = flags;
+    c->p = Par{p->omega, {p->force[0], p->force[1], p->force[2]}};
+    c->force = force;
+    const uint32_t nwg_c = (c->g.n + kCWG - 1) / kCWG;
+    const size_t words = ((size_t)c->g.n + 31) / 32 + 2;
+    uint32_t* d_cnt = nullptr;
+    uint64_t* d_icnt = nullptr;
+    auto fail = [&](int code) {
+        if (d_cnt) hipFree(d_cnt);
+        if (d_icnt) hipFree(d_icnt);
+        if (c->interior) hipFree(c->interior);
+        delete c;
+        return code;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&c->interior, words * 4)) != hipSuccess) return fail(hip_error(e, "hipMalloc"));
+    if ((e = hipMemset(c->interior, 0, words * 4)) != hipSuccess) return fail(hip_error(e, "hipMemset"));
+    if ((e = hipMalloc(&d_cnt, (size_t)nwg_c * 4)) != hipSuccess) return fail(hip_error(e, "hipMalloc"));
+    if ((e = hipMalloc(&d_icnt, (size_t)nwg_c * 8)) != hipSuccess) return fail(hip_error(e, "hipMalloc"));
+    hipLaunchKernelGGL(k_classify, dim3(nwg_c), dim3(kCWG), 0, 0, flags, c->g, c->interior, d_cnt,
+                       d_icnt);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(hip_error(e, "k_classify"));
+    std::vector<uint32_t> cnt(nwg_c);
+    std::vector<uint64_t> icnt(nwg_c);
+    if ((e = hipMemcpy(cnt.data(), d_cnt, (size_t)nwg_c * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+        return fail(hip_error(e, "hipMemcpy"));
+    if ((e = hipMemcpy(icnt.data(), d_icnt, (size_t)nwg_c * 8, hipMemcpyDeviceToHost)) != hipSuccess)
+        return fail(hip_error(e, "hipMemcpy"));
+    uint64_t total = 0, itotal = 0;
+    for (uint32_t k = 0; k < nwg_c; k++) {
+        total += cnt[k];
+        itotal += icnt[k];
+    }
+    c->n_boundary = total;
+    c->n_interior = itotal;
+    // do both fields hold the same wall values?
+    if ((e = hipMemset(d_cnt, 0, (size_t)nwg_c * 4)) != hipSuccess) return fail(hip_error(e, "hipMemset"));
+    by_type(d, [&](auto sc, auto) {
+        constexpr int S = decltype(sc)::value;
+        using T = typename Stor<S>::T;
+        hipLaunchKernelGGL((k_walls_equal<T>), dim3(nwg_c), dim3(kCWG), 0, 0, (const T*)fin,
+                           (const T*)fout, flags, c->g, d_cnt);
+    });
+    if ((e = hipGetLastError()) != hipSuccess) return fail(hip_error(e, "k_walls_equal"));
+    if ((e = hipMemcpy(cnt.data(), d_cnt, (size_t)nwg_c * 4, hipMemcpyDeviceToHost)) != hipSuccess)
+        return fail(hip_error(e, "hipMemcpy"));
+    c->copy_walls = 1;
+    for (uint32_t k = 0; k < nwg_c; k++)
+        if (cnt[k]) c->copy_walls = 0;
+    hipFree(d_cnt);
+    hipFree(d_icnt);
+    *out = c;
+    return NX_OK;
+}
+
+extern "C" int nx_container_run(nx_container c, nx_stream s)
+{
+    if (!c) return set_error(NX_ERR_INVALID, "null container");
+    return run_container(*c, g_tuning[c->d.storage][c->d.compute], (hipStream_t)s);
+}
+
+extern "C" int nx_container_info(nx_container c, uint64_t* n_interior, uint64_t* n_boundary,
+                                 int* copies_walls)
+{
+    if (!c) return set_error(NX_ERR_INVALID, "null container");
+    if (n_interior) *n_interior = c->n_interior;
+    if (n_boundary) *n_boundary = c->n_boundary;
+    if (copies_walls) *copies_walls = c->copy_walls;
+    return NX_OK;
+}
+
+extern "C" int nx_container_destroy(nx_container c)
+{
+    if (!c) return NX_OK;
+    if (c->interior) hipFree(c->interior);
+    delete c;
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_get_tuning_dense(const nx_dense_desc* d, nx_tuning* t)
+{
+    if (int e = check(d)) return e;
+    if (!t) return set_error(NX_ERR_INVALID, "null pointer");
+    *t = g_tuning[d->storage][d->compute];
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_set_tuning_dense(const nx_dense_desc* d, const nx_tuning* t)
+{
+    if (int e = check(d)) return e;
+    if (!t || !valid_tuning(*t))
+        return set_error(NX_ERR_INVALID,
+                         "tuning: wg in {256,512}, cpt in {1,2}, load_policy and xcd_remap in "
+                         "{0,1}, vx = 1");
+    g_tuning[d->storage][d->compute] = *t;
+    return NX_OK;
+}
+
+// Times every launch configuration of the container's step (fout is
+// overwritten, fin only read) and keeps the fastest for its (storage, compute).
+extern "C" int nx_container_autotune(nx_container c, nx_stream s, nx_tuning* best, float* best_ms)
+{
+    if (!c) return set_error(NX_ERR_INVALID, "null container");
+    nx_tuning& slot = g_tuning[c->d.storage][c->d.compute];
+    hipEvent_t e0, e1;
+    NX_CHECK_HIP(hipEventCreate(&e0));
+    NX_CHECK_HIP(hipEventCreate(&e1));
+    nx_tuning bt = slot;
+    float bms = 1e30f;
+    const int reps = 10;
+    for (int wg : {256, 512})
+        for (int cpt : {1, 2})
+            for (int lp : {0, 1})
+                for (int rm : {0, 1}) {
+                    const nx_tuning t{wg, cpt, lp, rm, 1};
+                    for (int r = 0; r < 3; r++)
+                        if (int e = run_container(*c, t, (hipStream_t)s)) return e;
+                    NX_CHECK_HIP(hipEventRecord(e0, (hipStream_t)s));
+                    for (int r = 0; r < reps; r++)
+                        if (int e = run_container(*c, t, (hipStream_t)s)) return e;
+                    NX_CHECK_HIP(hipEventRecord(e1, (hipStream_t)s));
+                    NX_CHECK_HIP(hipEventSynchronize(e1));
+                    float ms = 0;
+                    NX_CHECK_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    ms /= reps;
+                    if (ms < bms) {
+                        bms = ms;
+                        bt = t;
+                    }
+                }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    slot = bt;
+    if (best) *best = bt;
+    if (best_ms) *best_ms = bms;
+    return NX_OK;
+}

@@ -9,121 +9,13 @@
 // remapped so each XCD walks a contiguous slab (L2 keeps the row-boundary
 // lines).  Each population is read once and written once per step:
 // 2*19*sizeof(storage) bytes per cell update.
-#include <algorithm>
-#include <type_traits>
-
-#include "nx_common.hpp"
+#include "dense_common.hpp"
 
 namespace nx {
+namespace dense {
 namespace {
 
-constexpr int kWG = 256;
-
-struct DGeom {
-    int32_t X, Y, Z;
-    uint32_t XY;
-    uint32_t n;       // cells
-    uint64_t plane;   // cells per population plane
-    int32_t periodic;
-};
-
-DGeom make_geom(const nx_dense_desc* d)
-{
-    DGeom g;
-    g.X = d->dim[0];
-    g.Y = d->dim[1];
-    g.Z = d->dim[2];
-    g.XY = (uint32_t)g.X * (uint32_t)g.Y;
-    g.n = g.XY * (uint32_t)g.Z;
-    g.plane = g.n;
-    g.periodic = d->periodic;
-    return g;
-}
-
-struct Par {
-    double omega;
-    double g[3];
-};
-
-// Upstream index of population q of cell i (x - c_q), with periodic wrap.
-// Returns false when the upstream cell lies outside the grid.
-__device__ __forceinline__ bool upstream(const DGeom& g, int x, int y, int z, int q, uint32_t& j)
-{
-    int a = x - CX[q], b = y - CY[q], c = z - CZ[q];
-    if (g.periodic & NX_PERIODIC_X) a = a < 0 ? a + g.X : (a >= g.X ? a - g.X : a);
-    if (g.periodic & NX_PERIODIC_Y) b = b < 0 ? b + g.Y : (b >= g.Y ? b - g.Y : b);
-    if (g.periodic & NX_PERIODIC_Z) c = c < 0 ? c + g.Z : (c >= g.Z ? c - g.Z : c);
-    j = (uint32_t)a + (uint32_t)g.X * (uint32_t)b + g.XY * (uint32_t)c;
-    return a >= 0 && b >= 0 && c >= 0 && a < g.X && b < g.Y && c < g.Z;
-}
-
-__device__ __forceinline__ void coords(const DGeom& g, uint32_t i, int& x, int& y, int& z)
-{
-    z = (int)(i / g.XY);
-    const uint32_t r = i - (uint32_t)z * g.XY;
-    y = (int)(r / (uint32_t)g.X);
-    x = (int)(r - (uint32_t)y * (uint32_t)g.X);
-}
-
-// Pull stream of one bulk cell (LbmTools.h:78-168): for q != 9,
-//   wall bit q set:  f_q = fin(x, opp q) + fin(x - c_q, opp q)
-//   otherwise:       f_q = fin(x - c_q, q)
-// A masked link whose upstream cell does not exist contributes 0
-// (NX_FLAG_OPEN; the reference leaves that value undefined, NghData.h:13-16).
-template <int S, bool PERIODIC>
-__device__ __forceinline__ void pull(const typename Stor<S>::T* __restrict__ fin, const DGeom& g,
-                                     uint32_t i, uint32_t fl, typename Stor<S>::P f[Q])
-{
-    using T = typename Stor<S>::T;
-    using P = typename Stor<S>::P;
-    const uint32_t m = fl & NX_MASK_BITS;
-    uint32_t jj[Q];
-    uint32_t missing = 0;
-    if (PERIODIC || (fl & NX_FLAG_OPEN)) {
-        int x, y, z;
-        coords(g, i, x, y, z);
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            if (q == 9) continue;
-            uint32_t j;
-            const bool ok = upstream(g, x, y, z, q, j);
-            jj[q] = ok ? j : i;
-            missing |= ok ? 0u : (1u << q);
-        }
-    } else {
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            if (q == 9) continue;
-            const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
-            // flags from the wall-mask kernel keep this in range; the clamp
-            // keeps any caller-supplied flag word from addressing outside
-            jj[q] = min(i - (uint32_t)off, g.n - 1u);
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        if (q == 9) {
-            f[9] = Stor<S>::bulk(fin[9 * g.plane + i], 9);
-            continue;
-        }
-        const bool w = (m >> q) & 1u;
-        const uint64_t src = w ? (uint64_t)OPP[q] : (uint64_t)q;
-        T v = fin[src * g.plane + jj[q]];
-        if ((missing >> q) & 1u) v = T(0);
-        f[q] = w ? Stor<S>::raw(v) : Stor<S>::bulk(v, q);
-    }
-    if (m) {
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            if (q == 9) continue;
-            if ((m >> q) & 1u) {
-                const P own = Stor<S>::bulk(fin[(uint64_t)OPP[q] * g.plane + i], OPP[q]);
-                f[q] = own + f[q];
-            }
-        }
-    }
-}
-
+// Generic step (any boundary mode, including periodic wrap).
 template <int S, int CM, bool FORCE, bool PERIODIC>
 __global__ void __launch_bounds__(kWG)
     k_step(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
@@ -300,73 +192,35 @@ __global__ void __launch_bounds__(kWG)
     }
 }
 
-int check(const nx_dense_desc* d)
-{
-    if (!d) return set_error(NX_ERR_INVALID, "null descriptor");
-    for (int a = 0; a < 3; a++)
-        if (d->dim[a] < 1) return set_error(NX_ERR_INVALID, "grid dimensions must be >= 1");
-    const uint64_t n = (uint64_t)d->dim[0] * d->dim[1] * d->dim[2];
-    if (n >= (1ull << 32) - kWG) return set_error(NX_ERR_INVALID, "grid too large (>= 2^32 cells)");
-    if (d->storage < NX_F64 || d->storage > NX_F16DEV)
-        return set_error(NX_ERR_INVALID, "unknown storage type");
-    if (d->compute < NX_COMPUTE_F64 || d->compute > NX_COMPUTE_REF)
-        return set_error(NX_ERR_INVALID, "unknown compute mode");
-    if ((d->storage == NX_F32DEV || d->storage == NX_F16DEV) && d->compute != NX_COMPUTE_F32)
-        return set_error(NX_ERR_UNSUPPORTED, "deviation storage requires NX_COMPUTE_F32");
-    if (d->storage == NX_F64 && d->compute == NX_COMPUTE_F32)
-        return set_error(NX_ERR_UNSUPPORTED, "binary64 storage requires binary64 compute");
-    return NX_OK;
-}
-
-inline uint32_t nwg(const DGeom& g) { return (g.n + kWG - 1) / kWG; }
-
-template <int S, int CM, bool FORCE, bool PER>
-void launch_step(const void* fin, void* fout, const uint32_t* flags, const DGeom& g, const Par& p,
-                 hipStream_t s)
-{
-    using T = typename Stor<S>::T;
-    hipLaunchKernelGGL((k_step<S, CM, FORCE, PER>), dim3(nwg(g)), dim3(kWG), 0, s,
-                       (const T*)fin, (T*)fout, flags, g, p);
-}
-
 template <int S, int CM>
 void dispatch_step(bool force, bool per, const void* fin, void* fout, const uint32_t* flags,
                    const DGeom& g, const Par& p, hipStream_t s)
 {
+    using T = typename Stor<S>::T;
     if (force) {
-        if (per) launch_step<S, CM, true, true>(fin, fout, flags, g, p, s);
-        else launch_step<S, CM, true, false>(fin, fout, flags, g, p, s);
+        if (per)
+            hipLaunchKernelGGL((k_step<S, CM, true, true>), dim3(nwg(g)), dim3(kWG), 0, s,
+                               (const T*)fin, (T*)fout, flags, g, p);
+        else
+            hipLaunchKernelGGL((k_step<S, CM, true, false>), dim3(nwg(g)), dim3(kWG), 0, s,
+                               (const T*)fin, (T*)fout, flags, g, p);
     } else {
-        if (per) launch_step<S, CM, false, true>(fin, fout, flags, g, p, s);
-        else launch_step<S, CM, false, false>(fin, fout, flags, g, p, s);
+        if (per)
+            hipLaunchKernelGGL((k_step<S, CM, false, true>), dim3(nwg(g)), dim3(kWG), 0, s,
+                               (const T*)fin, (T*)fout, flags, g, p);
+        else
+            hipLaunchKernelGGL((k_step<S, CM, false, false>), dim3(nwg(g)), dim3(kWG), 0, s,
+                               (const T*)fin, (T*)fout, flags, g, p);
     }
-}
-
-// Calls f(integral_constant<S>, integral_constant<CM>) for the (storage,
-// compute) pair of d.
-template <int V>
-using IC = std::integral_constant<int, V>;
-
-template <typename F>
-int by_type(const nx_dense_desc* d, F&& f)
-{
-    switch (d->storage) {
-        case NX_F64: f(IC<NX_F64>{}, IC<NX_COMPUTE_F64>{}); return NX_OK;
-        case NX_F32:
-            if (d->compute == NX_COMPUTE_F64) f(IC<NX_F32>{}, IC<NX_COMPUTE_F64>{});
-            else if (d->compute == NX_COMPUTE_REF) f(IC<NX_F32>{}, IC<NX_COMPUTE_REF>{});
-            else f(IC<NX_F32>{}, IC<NX_COMPUTE_F32>{});
-            return NX_OK;
-        case NX_F32DEV: f(IC<NX_F32DEV>{}, IC<NX_COMPUTE_F32>{}); return NX_OK;
-        case NX_F16DEV: f(IC<NX_F16DEV>{}, IC<NX_COMPUTE_F32>{}); return NX_OK;
-    }
-    return set_error(NX_ERR_INVALID, "unknown storage type");
 }
 
 }  // namespace
+}  // namespace dense
 }  // namespace nx
 
+
 using namespace nx;
+using namespace nx::dense;
 
 extern "C" size_t nx_pop_bytes_dense(const nx_dense_desc* d)
 {
@@ -515,3 +369,4 @@ extern "C" int nx_d3q19_residual_dense(const nx_dense_desc* d, const void* fin,
     NX_CHECK_LAUNCH("k_residual");
     return finish_reduce((const double*)part, nb, 2, out, st);
 }
+

@@ -3,6 +3,7 @@
 // libNeonSys (GpuDevice.h:151-190 launch/error handling,
 // gpuDeviceMemory.cpp) and libNeonSet's Backend (src/set/Backend.cpp:46-55,
 // 404-419): one device, launches asynchronous on a caller stream.
+#include <algorithm>
 #include <mutex>
 #include <string>
 
@@ -62,6 +63,15 @@ __global__ void __launch_bounds__(256) k_finish(const double* part, int n, int n
     block_sum_f64<4>(acc, lds);
     if (threadIdx.x == 0)
         for (int k = 0; k < nv; k++) out[k] = acc[k];
+}
+
+// Plain streaming copy, 16 B per lane per iteration: the measure of
+// achievable HBM bandwidth (and of counter scaling) on the device.
+__global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                              size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
 }
 
 int finish_reduce(const double* partials, int n, int nv, double* out, hipStream_t s)
@@ -213,6 +223,19 @@ int nx_d2d(void* dst, const void* src, size_t bytes, nx_stream s)
 {
     if (!bytes) return NX_OK;
     NX_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)s));
+    return NX_OK;
+}
+
+int nx_copy_kernel(void* dst, const void* src, size_t bytes, nx_stream s)
+{
+    if (!dst || !src) return set_error(NX_ERR_INVALID, "null pointer");
+    if (bytes % 16 || ((uintptr_t)dst | (uintptr_t)src) % 16)
+        return set_error(NX_ERR_INVALID, "nx_copy_kernel: 16-byte aligned sizes and pointers only");
+    const size_t n = bytes / 16;
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+    hipLaunchKernelGGL(k_copy, dim3(nb ? nb : 1), dim3(256), 0, (hipStream_t)s, (const uint4*)src,
+                       (uint4*)dst, n);
+    NX_CHECK_LAUNCH("k_copy");
     return NX_OK;
 }
 

@@ -241,6 +241,7 @@ class DenseLattice(_Lattice):
         self._fill_and_mask(ulb, ulid, stream)
 
     def _fill_and_mask(self, ulb, ulid, stream):
+        self._drop_containers()
         lid = (C.c_double * 3)(*[float(v) for v in ulid])
         for b in self.pop:
             check(lib.nx_d3q19_fill_dense(C.byref(self.desc), b.ptr, self.flags.ptr, float(ulb),
@@ -264,6 +265,68 @@ class DenseLattice(_Lattice):
         r = rho.download(self.moment_dtype, n, stream).reshape(Z, Y, X)
         uu = u.download(self.moment_dtype, 3 * n, stream).reshape(3, Z, Y, X)
         return r, uu
+
+    def tuning(self) -> dict:
+        t = abi.Tuning()
+        check(lib.nx_d3q19_get_tuning_dense(C.byref(self.desc), C.byref(t)))
+        return t.as_dict()
+
+    def set_tuning(self, wg=256, cpt=1, load_policy=0, xcd_remap=0, vx=1) -> None:
+        t = abi.Tuning(int(wg), int(cpt), int(load_policy), int(xcd_remap), int(vx))
+        check(lib.nx_d3q19_set_tuning_dense(C.byref(self.desc), C.byref(t)))
+
+    # -- containers (LbmIterationD3Q19: one per ping-pong direction) ---------
+    def _containers(self, omega, force):
+        key = (float(omega), tuple(float(v) for v in force))
+        if getattr(self, "_ckey", None) != key:
+            self._drop_containers()
+            p = params(omega, force)
+            cs = []
+            for k in range(2):
+                h = C.c_void_p()
+                check(lib.nx_d3q19_iteration_dense(C.byref(self.desc), self.pop[k].ptr,
+                                                   self.pop[k ^ 1].ptr, self.flags.ptr,
+                                                   C.byref(p), C.byref(h)))
+                cs.append(h.value)
+            self._cont, self._ckey = cs, key
+        return self._cont
+
+    def _drop_containers(self):
+        for h in getattr(self, "_cont", []) or []:
+            lib.nx_container_destroy(h)
+        self._cont, self._ckey = [], None
+
+    def container_info(self, omega, force=(0.0, 0.0, 0.0)):
+        """(interior cells, other bulk cells, walls copied) of the container
+        that runs the next step."""
+        a, b, w = C.c_uint64(), C.c_uint64(), C.c_int()
+        check(lib.nx_container_info(self._containers(omega, force)[self.parity], C.byref(a),
+                                    C.byref(b), C.byref(w)))
+        return int(a.value), int(b.value), bool(w.value)
+
+    def run(self, nsteps: int, omega: float, force=(0.0, 0.0, 0.0), stream=None) -> None:
+        cs = self._containers(omega, force)
+        for _ in range(int(nsteps)):
+            check(lib.nx_container_run(cs[self.parity], stream))
+            self.parity ^= 1
+
+    def run_plain(self, nsteps: int, omega: float, force=(0.0, 0.0, 0.0), stream=None) -> None:
+        """Steps through nx_d3q19_step_dense (no container)."""
+        _Lattice.run(self, nsteps, omega, force, stream)
+
+    def autotune(self, omega: float, force=(0.0, 0.0, 0.0), stream=None):
+        """Time every launch configuration on this lattice's buffers and keep
+        the fastest (the output field is overwritten)."""
+        t, ms = abi.Tuning(), C.c_float()
+        h = self._containers(omega, force)[self.parity]
+        check(lib.nx_container_autotune(h, stream, C.byref(t), C.byref(ms)))
+        return t.as_dict(), float(ms.value)
+
+    def __del__(self):
+        try:
+            self._drop_containers()
+        except Exception:
+            pass
 
     def _mass(self, stream):
         check(lib.nx_d3q19_mass_dense(C.byref(self.desc), self.fin.ptr, self.flags.ptr,

@@ -157,3 +157,57 @@ def test_fp16_dev_velocity_error_vs_f64(nx):
         errs[st] = float(np.abs(u.astype(np.float64) - u64).max() / np.abs(u64).max())
     print("velocity error vs FP64:", errs)
     assert errs["f32"] < 1e-4 and errs["f32dev"] < 1e-4 and errs["f16dev"] < 1e-3
+
+
+@pytest.mark.parametrize("st,cm", [("f64", "f64"), ("f32", "f32"), ("f32", "ref"), ("f16dev", "f32")])
+def test_container_equals_plain_step(nx, orc, st, cm):
+    """The container (interior bitmap + boundary list, two kernels) and the
+    single-kernel nx_d3q19_step_dense give identical populations, on a grid
+    with periodic y, open x/z faces, walls and a body force."""
+    dim = (37, 12, 21)
+    X, Y, Z = dim
+    rng = np.random.default_rng(11)
+    cls = np.full((Z, Y, X), 2, np.uint32)
+    cls[rng.random((Z, Y, X)) < 0.05] = 0
+    cls[:, :, 0] = 1
+    a = nx.DenseLattice(dim, st, cm, periodic=2)
+    b = nx.DenseLattice(dim, st, cm, periodic=2)
+    for lat in (a, b):
+        lat.set_classes(cls.reshape(-1), ulb=0.04)
+    force = (0.0, 0.0, 0.0) if cm == "ref" else (2e-5, 1e-5, 0.0)
+    a.run(7, 1.5, force=force)
+    b.run_plain(7, 1.5, force=force)
+    assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
+    n_int, n_bd, walls = a.container_info(1.5, force)
+    fl = a.get_flags()
+    bulk = ((fl >> 24) & 3) == 2
+    assert n_int + n_bd == int(bulk.sum()) and n_bd > 0 and n_int > 0 and walls
+
+
+def test_container_cavity_counts(nx):
+    N = 40
+    g = nx.DenseLattice((N, N, N), "f32", "f32")
+    g.setup_cavity(0.05)
+    n_int, n_bd, walls = g.container_info(1.8)
+    assert n_int == (N - 4) ** 3 and n_bd == (N - 2) ** 3 - (N - 4) ** 3 and walls
+
+
+def test_container_leaves_differing_walls_unwritten(nx, orc):
+    """If the two fields hold different wall values the container must not
+    copy them (the reference never writes wall cells)."""
+    N = 12
+    g, o = _pair(nx, orc, (N, N, N), "f32", "f32", 1, 1)
+    g.setup_cavity(0.05)
+    o.setup_cavity(0.05)
+    p1 = o.pop[1].copy().reshape(19, -1)
+    wall = ((o.flags >> 24) & 3) != 2
+    p1[:, wall] += np.float32(0.25)
+    o.pop[1][:] = p1.reshape(-1)
+    g.set_populations(p1.reshape(-1), which=1)
+    assert g.container_info(1.7)[2] is False
+    g.run(5, 1.7)
+    o.run(5, 1.7)
+    assert np.array_equal(g.populations(), o.fin)
+    g.parity ^= 1
+    o.parity ^= 1
+    assert np.array_equal(g.populations(), o.fin)
