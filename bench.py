@@ -77,6 +77,25 @@ def cpu_baseline(N, storage, compute, omega, budget_s=12.0):
                       f"{el:.1f} s, CPU oracle (C restatement of LbmTools.h, OpenMP over z)"}
 
 
+def copy_calibration(nx):
+    """Plain device copy of 1 GiB, 10 launches (nx_copy_kernel): what
+    FETCH_SIZE / WRITE_SIZE read for a kernel that moves a known number of
+    bytes (1 GiB read + 1 GiB written per launch)."""
+    from neon_amd import abi
+    n = 1 << 30
+    a, b = nx.DeviceBuffer(n), nx.DeviceBuffer(n)
+    a.zero()
+    s = nx.Stream()
+    e0, e1 = nx.Event(), nx.Event()
+    e0.record(s)
+    for _ in range(10):
+        abi.check(abi.lib.nx_copy_kernel(b.ptr, a.ptr, n, s.handle))
+    e1.record(s)
+    e1.sync()
+    ms = e0.elapsed_ms(e1) / 10
+    print(json.dumps({"copy_bytes_per_launch": 2 * n, "ms": ms, "GBs": 2 * n / ms / 1e6}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +104,11 @@ def main():
     ap.add_argument("--workload", default="cavity256_f32", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 comparison run")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time every launch configuration first and keep the fastest")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the plain copy kernel (1 GiB, 10 launches): the "
+                         "reference for scaling FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -93,6 +117,10 @@ def main():
         sys.exit(2)
 
     import neon_amd as nx
+
+    if a.copy_calibration:
+        copy_calibration(nx)
+        return
 
     N, storage, compute, re, dsteps, dwarm = WORKLOADS[a.workload]
     steps = dsteps if a.steps is None else a.steps
@@ -105,6 +133,9 @@ def main():
     stream = nx.Stream()
     lat = nx.DenseLattice((N, N, N), storage, compute)
     lat.setup_cavity(ulb, stream=stream.handle)
+    if a.autotune:
+        lat.autotune(omega, stream=stream.handle)
+        lat.setup_cavity(ulb, stream=stream.handle)
     lat.run(warmup, omega, stream=stream.handle)
     stream.sync()
 
@@ -162,13 +193,13 @@ def main():
                                f"{storage} storage + {compute} collision, Re={re:g}, "
                                f"lid u=({ulb},0,0), {steps} timed steps after {warmup} warm-up",
                    "grid": "dGrid", "N": N, "storage": storage, "compute": compute,
-                   "omega": omega, "device": nx.device_name()},
+                   "omega": omega, "device": nx.device_name(), "launch": lat.tuning()},
         "mlups_ref_convention": round(mlups_ref, 1),
         "wall_s": round(wall, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": f"k_step<{storage},{compute}> ({b_alg} B/fluid cell)"},
+                     "kernel": f"k_fused<{storage},{compute}> ({b_alg} B/fluid cell)"},
         "velocity_error_vs_f64": err,
         "cpu_baseline": cpu,
     }
