@@ -41,6 +41,15 @@ WORKLOADS = {
     "cavity64_f64": (64, "f64", "f64", 100.0, 1000, 0),
 }
 
+# Porous medium on the block-sparse grid: 384^3, 8^3 blocks, solid = union of
+# 2000 seeded spheres with radius U[6,20), fully solid blocks inactive, body
+# force g = 1e-6 along x, periodic x, nu = 0.02.
+POROUS = {
+    # name: (N, storage, compute, default steps, default warmup)
+    "porous384_f32": (384, "f32", "f32", 200, 20),
+    "porous384_f16dev": (384, "f16dev", "f32", 200, 20),
+}
+
 
 def traffic_from_profiles(kernel_tag: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
@@ -96,12 +105,89 @@ def copy_calibration(nx):
     print(json.dumps({"copy_bytes_per_launch": 2 * n, "ms": ms, "GBs": 2 * n / ms / 1e6}))
 
 
+def porous(nx, a):
+    N, storage, compute, dsteps, dwarm = POROUS[a.workload]
+    steps = dsteps if a.steps is None else a.steps
+    warmup = dwarm if a.warmup is None else a.warmup
+    nu, g = 0.02, (1e-6, 0.0, 0.0)
+    omega = 1.0 / (3.0 * nu + 0.5)
+    nx.init(0)
+    dim = (N, N, N)
+    t0 = time.perf_counter()
+    solid = nx.spheres(dim, 2000, 6.0, 20.0, seed=2024, periodic=1)
+    grid = nx.BlockGrid(dim, 1 - solid, periodic=1)
+    t_setup = time.perf_counter() - t0
+    nb3 = (N // 8) ** 3
+    stream = nx.Stream()
+
+    def make(st, cm):
+        lat = nx.BlockLattice(grid, st, cm)
+        from neon_amd import abi
+        act = np.unpackbits(grid.masks.view(np.uint8), bitorder="little").astype(bool)
+        lat.set_classes(np.where(act, abi.NX_CELL_BULK, abi.NX_CELL_UNDEFINED).astype(np.uint32),
+                        stream=stream.handle)
+        return lat
+
+    lat = make(storage, compute)
+    lat.run(warmup, omega, force=g, stream=stream.handle)
+    stream.sync()
+    e0, e1 = nx.Event(), nx.Event()
+    nx.sync()
+    tw = time.perf_counter()
+    e0.record(stream)
+    lat.run(steps, omega, force=g, stream=stream.handle)
+    e1.record(stream)
+    e1.sync()
+    wall = time.perf_counter() - tw
+    ms = e0.elapsed_ms(e1)
+    ms_step = ms / steps
+    fluid = grid.n_active
+    mlups = fluid * steps / (ms * 1e-3) / 1e6
+    b_alg = 2 * 19 * STORAGE_BYTES[storage]
+    achieved = fluid * b_alg / (ms_step * 1e-3) / 1e9
+    err = None
+    if not a.no_error:
+        _, u = lat.moments(stream=stream.handle)
+        del lat
+        ref = make("f64", "f64")
+        ref.run(warmup + steps, omega, force=g)
+        _, u64 = ref.moments()
+        act = np.repeat(np.unpackbits(grid.masks.view(np.uint8), bitorder="little")
+                        .astype(bool).reshape(-1, 512), 3, axis=0).reshape(-1)
+        d = (u.astype(np.float64) - u64)[act]
+        err = {"max_rel": float(np.abs(d).max() / np.abs(u64[act]).max()),
+               "l2_rel": float(np.sqrt((d * d).sum() / (u64[act] ** 2).sum())),
+               "reference": "binary64 GPU run of the same case"}
+    out = {
+        "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": storage,
+        "data": "synthetic (seeded sphere pack, fluid at rest, f = w_q)",
+        "config": {"workload": f"porous medium D3Q19 BGK on bGrid {N}^3 (8^3 blocks), 2000 "
+                               f"spheres r~U[6,20) seed 2024, periodic x, g=1e-6 x, nu=0.02, "
+                               f"{storage} storage + {compute} collision, {steps} timed steps "
+                               f"after {warmup} warm-up",
+                   "grid": "bGrid", "N": N, "blocks": grid.n_blocks, "candidate_blocks": nb3,
+                   "active_block_fraction": round(grid.n_blocks / nb3, 4),
+                   "fluid_voxels": fluid, "fluid_fraction": round(fluid / N ** 3, 4),
+                   "setup_s": round(t_setup, 3), "device": nx.device_name()},
+        "wall_s": round(wall, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(f"d3q19_block_{storage}_{N}"),
+                     "kernel": f"k_step<block,{storage},{compute}> ({b_alg} B/fluid voxel)"},
+        "velocity_error_vs_f64": err,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="cavity256_f32", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="cavity256_f32", choices=sorted(WORKLOADS) + sorted(POROUS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--autotune", action="store_true",
@@ -121,6 +207,9 @@ def main():
     if a.copy_calibration:
         copy_calibration(nx)
         return
+
+    if a.workload in POROUS:
+        return porous(nx, a)
 
     N, storage, compute, re, dsteps, dwarm = WORKLOADS[a.workload]
     steps = dsteps if a.steps is None else a.steps

@@ -15,6 +15,8 @@
  *   bits 24..25  classification (NX_CELL_*)
  *   bit  28      NX_FLAG_OPEN: at least one masked link points outside the
  *                grid or at an inactive voxel (its bounce-back term is 0)
+ *   bit  29      NX_FLAG_MIXED: masked links of both kinds (some upstream
+ *                cells exist, some do not)
  *
  * Population layouts:
  *   dense (dGrid SoA, dField_imp.h:67-87):  f[q][z][y][x], index
@@ -61,6 +63,7 @@ extern "C" {
 #define NX_CLASS_SHIFT 24
 #define NX_MASK_BITS 0x7FDFFu
 #define NX_FLAG_OPEN (1u << 28)
+#define NX_FLAG_MIXED (1u << 29)
 
 #define NX_PERIODIC_X 1
 #define NX_PERIODIC_Y 2
@@ -225,6 +228,10 @@ int nx_d3q19_wall_mask_block(const nx_block_desc* d, const uint32_t* flags_in,
                              uint32_t* flags_out, nx_stream s);
 int nx_d3q19_step_block(const nx_block_desc* d, const void* fin, void* fout,
                         const uint32_t* flags, const nx_lbm_params* p, nx_stream s);
+/* Block-to-XCD assignment of nx_d3q19_step_block (1: each XCD walks one
+ * contiguous range of blocks, the default; 0: hardware round-robin).  No
+ * counterpart in the reference. */
+int nx_d3q19_set_block_remap(int on);
 int nx_d3q19_moments_block(const nx_block_desc* d, const void* fin,
                            const uint32_t* flags, void* rho, void* u, nx_stream s);
 int nx_d3q19_mass_block(const nx_block_desc* d, const void* fin,
@@ -232,6 +239,14 @@ int nx_d3q19_mass_block(const nx_block_desc* d, const void* fin,
 int nx_d3q19_residual_block(const nx_block_desc* d, const void* fin,
                             const uint32_t* flags, float* u_prev, double* out,
                             nx_stream s);
+
+/* ---- geometry (benchmark workloads; no counterpart in the reference) ------
+ * Union of n seeded spheres, centres uniform in the box, radii uniform in
+ * [rmin, rmax), SplitMix64 stream (draws per sphere: cx, cy, cz, r); voxel
+ * (x,y,z) is solid when |(x,y,z) - c| < r, periodic axes measured across
+ * the wrap.  `solid` is X*Y*Z bytes, x fastest. */
+int nx_geometry_spheres(const int32_t dim[3], uint32_t n_spheres, double rmin, double rmax,
+                        uint64_t seed, int32_t periodic, uint8_t* solid);
 
 #ifdef __cplusplus
 }

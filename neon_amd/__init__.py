@@ -5,4 +5,4 @@ the C ABI of include/neon_amd.h).  This package only binds it.
 """
 from . import abi  # noqa: F401  (raises ImportError when the library is not built)
 from .lattice import (BlockGrid, BlockLattice, DenseLattice, DeviceBuffer, Event,  # noqa: F401
-                      Stream, device_name, init, params, sync)
+                      Stream, device_name, init, params, spheres, sync)

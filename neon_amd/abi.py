@@ -18,6 +18,7 @@ NX_CELL_BOUNCEBACK, NX_CELL_MOVINGWALL, NX_CELL_BULK, NX_CELL_UNDEFINED = range(
 NX_CLASS_SHIFT = 24
 NX_MASK_BITS = 0x7FDFF
 NX_FLAG_OPEN = 1 << 28
+NX_FLAG_MIXED = 1 << 29
 NX_PERIODIC_X, NX_PERIODIC_Y, NX_PERIODIC_Z = 1, 2, 4
 
 STORAGE_BYTES = {NX_F64: 8, NX_F32: 4, NX_F32DEV: 4, NX_F16DEV: 2}
@@ -116,6 +117,8 @@ SIGNATURES = {
     "nx_d3q19_fill_block": (_I, [_BP, _P, _P, _D, C.POINTER(_D), _P]),
     "nx_d3q19_wall_mask_block": (_I, [_BP, _P, _P, _P]),
     "nx_d3q19_step_block": (_I, [_BP, _P, _P, _P, _PP, _P]),
+    "nx_d3q19_set_block_remap": (_I, [_I]),
+    "nx_geometry_spheres": (_I, [C.POINTER(_I32), C.c_uint32, _D, _D, C.c_uint64, _I32, _P]),
     "nx_d3q19_moments_block": (_I, [_BP, _P, _P, _P, _P, _P]),
     "nx_d3q19_mass_block": (_I, [_BP, _P, _P, _P, _P]),
     "nx_d3q19_residual_block": (_I, [_BP, _P, _P, _P, _P, _P]),

@@ -21,6 +21,21 @@
 
 #include "nx_common.hpp"
 
+// pins a loaded value ahead of the branch that follows (see dense_common.hpp)
+template <typename T>
+__device__ __forceinline__ void nx_pin(T& v)
+{
+    if constexpr (sizeof(T) <= 4) {
+        asm volatile("" : "+v"(v));
+    } else {
+        uint32_t w[sizeof(T) / 4];
+        __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+        for (unsigned k = 0; k < sizeof(T) / 4; k++) asm volatile("" : "+v"(w[k]));
+        __builtin_memcpy(&v, w, sizeof(T));
+    }
+}
+
 namespace nx {
 namespace {
 
@@ -47,23 +62,39 @@ struct Par {
     double g[3];
 };
 
+// The block's connectivity row, read unconditionally once per workgroup (a
+// uniform address: scalar loads into SGPRs).  Reading it inside a per-lane
+// select would make every entry a per-lane load that each population load
+// then waits for.
+struct Conn {
+    uint32_t c[27];
+};
+
+__device__ __forceinline__ Conn load_conn(const uint32_t* __restrict__ conn, uint32_t b)
+{
+    Conn r;
+#pragma unroll
+    for (int d = 0; d < 27; d++) r.c[d] = conn[(uint64_t)b * 27 + d];
+    return r;
+}
+
 // Neighbour voxel (block t, in-block index nv) of voxel (x,y,z) of block b at
 // offset -c_q (the upstream voxel of population q).
 template <int q>
-__device__ __forceinline__ void upstream(uint32_t b, const uint32_t* __restrict__ cb, int x, int y,
-                                         int z, uint32_t& t, uint32_t& nv)
+__device__ __forceinline__ void upstream(uint32_t b, const Conn& cn, int x, int y, int z,
+                                         uint32_t& t, uint32_t& nv)
 {
     constexpr int s0 = -CX[q], s1 = -CY[q], s2 = -CZ[q];
     const bool c0 = s0 != 0 && x == (s0 < 0 ? 0 : 7);
     const bool c1 = s1 != 0 && y == (s1 < 0 ? 0 : 7);
     const bool c2 = s2 != 0 && z == (s2 < 0 ? 0 : 7);
     t = b;
-    if constexpr (s0 != 0) t = c0 ? cb[13 + s0] : t;
-    if constexpr (s1 != 0) t = c1 ? cb[13 + 3 * s1] : t;
-    if constexpr (s2 != 0) t = c2 ? cb[13 + 9 * s2] : t;
-    if constexpr (s0 != 0 && s1 != 0) t = (c0 && c1) ? cb[13 + s0 + 3 * s1] : t;
-    if constexpr (s0 != 0 && s2 != 0) t = (c0 && c2) ? cb[13 + s0 + 9 * s2] : t;
-    if constexpr (s1 != 0 && s2 != 0) t = (c1 && c2) ? cb[13 + 3 * s1 + 9 * s2] : t;
+    if constexpr (s0 != 0) t = c0 ? cn.c[13 + s0] : t;
+    if constexpr (s1 != 0) t = c1 ? cn.c[13 + 3 * s1] : t;
+    if constexpr (s2 != 0) t = c2 ? cn.c[13 + 9 * s2] : t;
+    if constexpr (s0 != 0 && s1 != 0) t = (c0 && c1) ? cn.c[13 + s0 + 3 * s1] : t;
+    if constexpr (s0 != 0 && s2 != 0) t = (c0 && c2) ? cn.c[13 + s0 + 9 * s2] : t;
+    if constexpr (s1 != 0 && s2 != 0) t = (c1 && c2) ? cn.c[13 + 3 * s1 + 9 * s2] : t;
     nv = (uint32_t)(((x + s0) & 7) + 8 * ((y + s1) & 7) + 64 * ((z + s2) & 7));
 }
 
@@ -97,7 +128,7 @@ using AllQ = std::make_integer_sequence<int, Q>;
 // upstream voxel is absent or inactive contribute 0 (NX_FLAG_OPEN).
 template <int S>
 __device__ __forceinline__ void pull(const typename Stor<S>::T* __restrict__ fin, const BGeom& g,
-                                     uint32_t b, const uint32_t* __restrict__ cb,
+                                     uint32_t b, const Conn& cb,
                                      const uint32_t* smask, uint32_t v, uint32_t fl,
                                      typename Stor<S>::P f[Q])
 {
@@ -147,27 +178,92 @@ __device__ __forceinline__ bool stage(const BGeom& g, uint32_t b, uint32_t* smas
     return bit_of(smask, threadIdx.x);
 }
 
+// Step of one block per workgroup.  The 19 population loads of a voxel are
+// issued together with its flag and mask words (their addresses depend only
+// on the block's connectivity row, read with scalar loads), masked links are
+// fixed up afterwards in one batch of loads, and every lane stores, so no
+// cache line of fout is written partially: inactive voxels receive 0 (the
+// inactive value of the reference's population fields,
+// RunCavityTwoPop.cu:56-57), active non-bulk voxels are left untouched.
+// The flag bits select the wall-link case without per-link lookups:
+// NX_FLAG_OPEN without NX_FLAG_MIXED -- every masked link is missing (porous
+// media: the link contributes the voxel's own opposite population only);
+// no NX_FLAG_OPEN -- every masked link has an upstream voxel; NX_FLAG_MIXED
+// -- per-link validity from the neighbour block's active mask.
 template <int S, int CM, bool FORCE>
 __global__ void __launch_bounds__(kBV)
     k_step(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
-           const uint32_t* __restrict__ flags, BGeom g, Par p)
+           const uint32_t* __restrict__ flags, BGeom g, Par p, int remap)
 {
+    using T = typename Stor<S>::T;
+    using P = typename Stor<S>::P;
     using R = typename Real<S, CM>::R;
-    __shared__ uint32_t smask[16];
-    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x);
+    const uint32_t b = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t v = threadIdx.x;
-    if (!stage(g, b, smask)) return;
-    const uint32_t fl = __builtin_nontemporal_load(&flags[(uint64_t)b * kBV + v]);
-    if (cls_of(fl) != NX_CELL_BULK) return;
-    const uint32_t* cb = g.conn + (uint64_t)b * 27;
-    typename Stor<S>::P f[Q];
-    pull<S>(fin, g, b, cb, smask, v, fl, f);
-    R out[Q];
-    collide<S, CM, FORCE>(f, p.omega, p.g, out);
+    const int x = (int)(v & 7u), y = (int)((v >> 3) & 7u), z = (int)(v >> 6);
     const uint64_t own = (uint64_t)b * kBStride + v;
+    const Conn cb = load_conn(g.conn, b);
+    uint32_t mword = g.masks[(uint64_t)b * 16 + (v >> 5)];
+    uint32_t fl = flags[(uint64_t)b * kBV + v];
+    T val[Q];
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (q == 9) {
+            val[9] = fin[own + 9 * kBV];
+        } else {
+            uint32_t t, nv;
+            upstream<q>(b, cb, x, y, z, t, nv);
+            const uint64_t a = t != kInvalidBlock ? (uint64_t)t * kBStride + (uint32_t)q * kBV + nv : own;
+            val[q] = fin[a];
+        }
+    });
 #pragma unroll
-    for (int q = 0; q < Q; q++)
-        __builtin_nontemporal_store(Stor<S>::template enc<R>(out[q], q), &fout[own + q * kBV]);
+    for (int q = 0; q < Q; q++) nx_pin(val[q]);
+    nx_pin(fl);
+    nx_pin(mword);
+    const bool active = (mword >> (v & 31u)) & 1u;
+    if (active && cls_of(fl) == NX_CELL_BULK) {
+        P f[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) f[q] = Stor<S>::bulk(val[q], q);
+        const uint32_t m = fl & NX_MASK_BITS;
+        if (m) {
+            const bool open = (fl & NX_FLAG_OPEN) != 0, mixed = (fl & NX_FLAG_MIXED) != 0;
+            T ownv[Q], nbv[Q];
+            uint32_t okm = 0;
+            for_q(AllQ{}, [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if constexpr (q != 9) {
+                    uint32_t t, nv;
+                    upstream<q>(b, cb, x, y, z, t, nv);
+                    bool ok = !open && t != kInvalidBlock;
+                    if (mixed)
+                        ok = t != kInvalidBlock &&
+                             ((g.masks[(uint64_t)t * 16 + (nv >> 5)] >> (nv & 31u)) & 1u);
+                    const bool w = (m >> q) & 1u;
+                    okm |= (ok ? 1u : 0u) << q;
+                    const uint64_t oa = own + (uint64_t)OPP[q] * kBV;
+                    ownv[q] = fin[oa];
+                    nbv[q] = fin[(w && ok) ? (uint64_t)t * kBStride + (uint32_t)OPP[q] * kBV + nv : oa];
+                }
+            });
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                if (q == 9) continue;
+                const P up = ((okm >> q) & 1u) ? Stor<S>::raw(nbv[q]) : P(0);
+                const P fixed = Stor<S>::bulk(ownv[q], OPP[q]) + up;
+                f[q] = ((m >> q) & 1u) ? fixed : f[q];
+            }
+        }
+        R out[Q];
+        collide<S, CM, FORCE>(f, p.omega, p.g, out);
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            __builtin_nontemporal_store(Stor<S>::template enc<R>(out[q], q), &fout[own + q * kBV]);
+    } else if (!active) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) __builtin_nontemporal_store(T(0), &fout[own + q * kBV]);
+    }
 }
 
 __global__ void __launch_bounds__(kBV) k_cavity_flags(uint32_t* flags, BGeom g)
@@ -228,9 +324,9 @@ __global__ void __launch_bounds__(kBV)
         fout[i] = fl;
         return;
     }
-    const uint32_t* cb = g.conn + (uint64_t)b * 27;
+    const Conn cb = load_conn(g.conn, b);
     const int x = (int)(v & 7u), y = (int)((v >> 3) & 7u), z = (int)(v >> 6);
-    uint32_t m = 0, open = 0;
+    uint32_t m = 0, open = 0, closed = 0;
     for_q(AllQ{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
         if constexpr (q != 9) {
@@ -241,10 +337,12 @@ __global__ void __launch_bounds__(kBV)
                 open = NX_FLAG_OPEN;
             } else if (cls_of(fin[(uint64_t)t * kBV + nv]) != NX_CELL_BULK) {
                 m |= 1u << q;
+                closed = 1;
             }
         }
     });
-    fout[i] = ((uint32_t)NX_CELL_BULK << NX_CLASS_SHIFT) | m | open;
+    fout[i] = ((uint32_t)NX_CELL_BULK << NX_CLASS_SHIFT) | m | open |
+              (open && closed ? NX_FLAG_MIXED : 0u);
 }
 
 template <int S, int CM>
@@ -263,7 +361,7 @@ __global__ void __launch_bounds__(kBV)
 #pragma unroll
     for (int q = 0; q < Q; q++) f[q] = 0;
     const uint64_t own = (uint64_t)b * kBStride + v;
-    if (cls == NX_CELL_BULK) pull<S>(fin, g, b, g.conn + (uint64_t)b * 27, smask, v, fl, f);
+    if (cls == NX_CELL_BULK) pull<S>(fin, g, b, load_conn(g.conn, b), smask, v, fl, f);
     if (cls == NX_CELL_MOVINGWALL)
         for (int a = 0; a < 3; a++) f[a] = Stor<S>::raw(fin[own + a * kBV]);
     R r, uu[3];
@@ -314,7 +412,7 @@ __global__ void __launch_bounds__(kBV)
         const uint32_t fl = flags[(uint64_t)b * kBV + v];
         if (cls_of(fl) != NX_CELL_BULK) continue;
         typename Stor<S>::P f[Q];
-        pull<S>(fin, g, b, g.conn + (uint64_t)b * 27, smask, v, fl, f);
+        pull<S>(fin, g, b, load_conn(g.conn, b), smask, v, fl, f);
         R r, uu[3];
         cell_moments<S, CM>(NX_CELL_BULK, f, r, uu);
         for (int a = 0; a < 3; a++) {
@@ -353,6 +451,9 @@ int check(const nx_block_desc* d)
 
 template <int V>
 using IC = std::integral_constant<int, V>;
+
+// 1: contiguous range of blocks per XCD (nx_d3q19_set_block_remap)
+int g_block_remap = 1;
 
 template <typename F>
 int by_type(const nx_block_desc* d, F&& f)
@@ -446,10 +547,10 @@ extern "C" int nx_d3q19_step_block(const nx_block_desc* d, const void* fin, void
             using T = typename Stor<S>::T;
             if (force)
                 hipLaunchKernelGGL((k_step<S, CM, true>), dim3(g.n_blocks), dim3(kBV), 0, st,
-                                   (const T*)fin, (T*)fout, flags, g, par);
+                                   (const T*)fin, (T*)fout, flags, g, par, g_block_remap);
             else
                 hipLaunchKernelGGL((k_step<S, CM, false>), dim3(g.n_blocks), dim3(kBV), 0, st,
-                                   (const T*)fin, (T*)fout, flags, g, par);
+                                   (const T*)fin, (T*)fout, flags, g, par, g_block_remap);
         }))
         return e;
     NX_CHECK_LAUNCH("k_step(block)");
@@ -514,4 +615,10 @@ extern "C" int nx_d3q19_residual_block(const nx_block_desc* d, const void* fin,
         return e;
     NX_CHECK_LAUNCH("k_residual(block)");
     return finish_reduce((const double*)part, nb, 2, out, st);
+}
+
+extern "C" int nx_d3q19_set_block_remap(int on)
+{
+    g_block_remap = on ? 1 : 0;
+    return NX_OK;
 }

@@ -90,7 +90,7 @@ __global__ void __launch_bounds__(kWG)
     }
     int x, y, z;
     coords(g, i, x, y, z);
-    uint32_t m = 0, open = 0;
+    uint32_t m = 0, open = 0, closed = 0;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         if (q == 9) continue;
@@ -100,9 +100,11 @@ __global__ void __launch_bounds__(kWG)
             open = NX_FLAG_OPEN;
         } else if (cls_of(fin[j]) != NX_CELL_BULK) {
             m |= 1u << q;
+            closed = 1;
         }
     }
-    fout[i] = ((uint32_t)NX_CELL_BULK << NX_CLASS_SHIFT) | m | open;
+    fout[i] = ((uint32_t)NX_CELL_BULK << NX_CLASS_SHIFT) | m | open |
+              (open && closed ? NX_FLAG_MIXED : 0u);
 }
 
 template <typename M>

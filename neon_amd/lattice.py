@@ -39,6 +39,16 @@ def device_name() -> str:
     return buf.value.decode()
 
 
+def spheres(dim, n: int, rmin: float, rmax: float, seed: int, periodic: int = 0) -> np.ndarray:
+    """Solid mask (uint8, x fastest) of n seeded random spheres
+    (nx_geometry_spheres)."""
+    d = (C.c_int32 * 3)(*[int(v) for v in dim])
+    out = np.empty(int(dim[0]) * int(dim[1]) * int(dim[2]), np.uint8)
+    check(lib.nx_geometry_spheres(d, int(n), float(rmin), float(rmax), int(seed), int(periodic),
+                                  out.ctypes.data))
+    return out
+
+
 class DeviceBuffer:
     """Owning device allocation."""
 

@@ -376,7 +376,7 @@ void or_wall_mask_dense(const or_dense* d, const uint32_t* in, uint32_t* out)
                     tmp[i] = fl;
                     continue;
                 }
-                uint32_t m = 0, open = 0;
+                uint32_t m = 0, open = 0, closed = 0;
                 for (int q = 0; q < 19; q++) {
                     if (q == 9) continue;
                     size_t j;
@@ -385,9 +385,11 @@ void or_wall_mask_dense(const or_dense* d, const uint32_t* in, uint32_t* out)
                         open = OR_FLAG_OPEN;
                     } else if (cls_of(in[j]) != OR_BULK) {
                         m |= 1u << q;
+                        closed = 1;
                     }
                 }
-                tmp[i] = ((uint32_t)OR_BULK << OR_CLASS_SHIFT) | m | open;
+                tmp[i] = ((uint32_t)OR_BULK << OR_CLASS_SHIFT) | m | open |
+                         (open && closed ? OR_FLAG_MIXED : 0u);
             }
     memcpy(out, tmp, g.plane * sizeof(uint32_t));
     free(tmp);
@@ -723,7 +725,7 @@ void or_wall_mask_block(const or_block* d, const uint32_t* in, uint32_t* out)
                 tmp[i] = fl;
                 continue;
             }
-            uint32_t m = 0, open = 0;
+            uint32_t m = 0, open = 0, closed = 0;
             for (int q = 0; q < 19; q++) {
                 if (q == 9) continue;
                 uint32_t nb;
@@ -733,9 +735,11 @@ void or_wall_mask_block(const or_block* d, const uint32_t* in, uint32_t* out)
                     open = OR_FLAG_OPEN;
                 } else if (cls_of(in[(size_t)nb * 512 + nv]) != OR_BULK) {
                     m |= 1u << q;
+                    closed = 1;
                 }
             }
-            tmp[i] = ((uint32_t)OR_BULK << OR_CLASS_SHIFT) | m | open;
+            tmp[i] = ((uint32_t)OR_BULK << OR_CLASS_SHIFT) | m | open |
+                     (open && closed ? OR_FLAG_MIXED : 0u);
         }
     memcpy(out, tmp, n * sizeof(uint32_t));
     free(tmp);

@@ -28,6 +28,7 @@
 #define OR_CLASS_SHIFT 24
 #define OR_MASK_BITS 0x7FDFFu
 #define OR_FLAG_OPEN (1u << 28)
+#define OR_FLAG_MIXED (1u << 29)
 #define OR_INVALID_BLOCK 0xFFFFFFFFu
 
 typedef struct {

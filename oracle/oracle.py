@@ -21,6 +21,7 @@ BOUNCEBACK, MOVINGWALL, BULK, UNDEFINED = range(4)
 CLASS_SHIFT = 24
 MASK_BITS = 0x7FDFF
 FLAG_OPEN = 1 << 28
+FLAG_MIXED = 1 << 29
 NP_STORAGE = {F64: np.float64, F32: np.float32, F32DEV: np.float32, F16DEV: np.uint16}
 
 

@@ -65,11 +65,16 @@ def test_block_porous_periodic_forced_bit_exact(nx, orc, st, sto):
     assert grid.n_blocks == o.n_blocks and np.array_equal(grid.conn, o.conn)
     keep = o.active_slots()
     cls = np.where(keep, 2, 3).astype(np.uint32)
+    # a few active bounce-back voxels next to inactive ones: voxels with both
+    # kinds of wall link (NX_FLAG_MIXED)
+    rng = np.random.default_rng(9)
+    cls[keep & (rng.random(keep.size) < 0.03)] = 0
     g.set_classes(cls)
     o.set_classes(cls)
     fl = g.get_flags()
     assert np.array_equal(fl[keep], o.flags[keep])
     assert ((fl[keep] & (1 << 28)) != 0).sum() > 100
+    assert ((fl[keep] & (1 << 29)) != 0).sum() > 10
     g.run(11, 1.6, force=(3e-5, 0.0, 0.0))
     o.run(11, 1.6, force=(3e-5, 0.0, 0.0))
     pg = _raw(g.populations(), g.storage).reshape(-1, 19, 512)
