@@ -73,6 +73,7 @@ typedef struct nx_stream_s* nx_stream;
 typedef struct nx_event_s* nx_event;
 typedef struct nx_bgrid_s* nx_bgrid;
 typedef struct nx_container_s* nx_container;
+typedef struct nx_batch_s* nx_batch;
 
 /* Dense grid: one partition of a dGrid (dGrid_imp.h:7-185). */
 typedef struct {
@@ -240,6 +241,35 @@ int nx_d3q19_residual_block(const nx_block_desc* d, const void* fin,
                             const uint32_t* flags, float* u_prev, double* out,
                             nx_stream s);
 
+/* ---- batched independent cavities (no counterpart in the reference) ------
+ * n members, each a dense grid dims[3k..3k+2] with classification classes
+ * (uint8 NX_CELL_*, concatenated in member order, x fastest), relaxation
+ * omega[k], lid ulb[k] * ulid[3k..3k+2] (fill of RunCavityTwoPop.cu:168-206).
+ * One arena per population field; member k's populations are its dGrid SoA
+ * layout.  Steps run as one persistent launch for all members; nx_batch_run
+ * captures check_every steps plus the residual test in a hipGraph and
+ * replays it until every member's residual
+ *   r = sqrt(sum |u_t - u_{t-K}|^2 / sum |u_t|^2)   (bulk cells, FP64 sums)
+ * is below tol (converged members stop stepping) or max_steps are taken.
+ * nx_batch_init sets bulk populations to the equilibrium of (rho, u)
+ * (device arrays in arena cell order; NULL: rest).  nx_batch_member returns a
+ * member's dense view (arena 0 holds the latest populations after an even
+ * number of steps). */
+int nx_batch_create(uint32_t n, const int32_t* dims, const uint8_t* classes,
+                    const double* omega, const double* ulb, const double* ulid,
+                    int32_t storage, nx_batch* b);
+int nx_batch_info(nx_batch b, uint32_t* n_members, uint64_t* n_cells, uint32_t* n_tiles);
+int nx_batch_member(nx_batch b, uint32_t k, int which, nx_dense_desc* d, void** pop,
+                    uint32_t** flags);
+int nx_batch_init(nx_batch b, const float* rho, const float* u, nx_stream s);
+int nx_batch_step(nx_batch b, int32_t k_steps, nx_stream s);
+int nx_batch_run(nx_batch b, int32_t max_steps, int32_t check_every, double tol,
+                 int32_t* steps, double* residual, uint32_t* n_unconverged);
+int nx_batch_set_launch(nx_batch b, int32_t wg, int32_t wgs_per_cu);
+int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int32_t* best_wgs_per_cu,
+                      float* best_ms);
+int nx_batch_destroy(nx_batch b);
+
 /* ---- geometry (benchmark workloads; no counterpart in the reference) ------
  * Union of n seeded spheres, centres uniform in the box, radii uniform in
  * [rmin, rmax), SplitMix64 stream (draws per sphere: cx, cy, cz, r); voxel
@@ -247,6 +277,12 @@ int nx_d3q19_residual_block(const nx_block_desc* d, const void* fin,
  * the wrap.  `solid` is X*Y*Z bytes, x fastest. */
 int nx_geometry_spheres(const int32_t dim[3], uint32_t n_spheres, double rmin, double rmax,
                         uint64_t seed, int32_t periodic, uint8_t* solid);
+/* Lid-driven cavity classes (RunCavityTwoPop.cu:208-224) plus n_boxes
+ * bounce-back boxes, edges uniform in {emin..emax}, lower corner uniform in
+ * {1 .. dim-1-edge} per axis (draws per box: ex, ey, ez, x0, y0, z0),
+ * clipped to the interior.  `cls` is X*Y*Z bytes of NX_CELL_*. */
+int nx_geometry_cavity(const int32_t dim[3], uint32_t n_boxes, int32_t emin, int32_t emax,
+                       uint64_t seed, uint8_t* cls);
 
 #ifdef __cplusplus
 }

@@ -119,6 +119,16 @@ SIGNATURES = {
     "nx_d3q19_step_block": (_I, [_BP, _P, _P, _P, _PP, _P]),
     "nx_d3q19_set_block_remap": (_I, [_I]),
     "nx_geometry_spheres": (_I, [C.POINTER(_I32), C.c_uint32, _D, _D, C.c_uint64, _I32, _P]),
+    "nx_geometry_cavity": (_I, [C.POINTER(_I32), C.c_uint32, _I32, _I32, C.c_uint64, _P]),
+    "nx_batch_create": (_I, [C.c_uint32, _P, _P, _P, _P, _P, _I32, C.POINTER(_P)]),
+    "nx_batch_info": (_I, [_P, C.POINTER(_U32), C.POINTER(C.c_uint64), C.POINTER(_U32)]),
+    "nx_batch_member": (_I, [_P, _U32, _I, _DP, C.POINTER(_P), C.POINTER(_P)]),
+    "nx_batch_init": (_I, [_P, _P, _P, _P]),
+    "nx_batch_step": (_I, [_P, _I32, _P]),
+    "nx_batch_run": (_I, [_P, _I32, _I32, _D, _P, _P, C.POINTER(_U32)]),
+    "nx_batch_set_launch": (_I, [_P, _I32, _I32]),
+    "nx_batch_autotune": (_I, [_P, _P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(C.c_float)]),
+    "nx_batch_destroy": (_I, [_P]),
     "nx_d3q19_moments_block": (_I, [_BP, _P, _P, _P, _P, _P]),
     "nx_d3q19_mass_block": (_I, [_BP, _P, _P, _P, _P]),
     "nx_d3q19_residual_block": (_I, [_BP, _P, _P, _P, _P, _P]),

@@ -178,13 +178,14 @@ __device__ __forceinline__ bool stage(const BGeom& g, uint32_t b, uint32_t* smas
     return bit_of(smask, threadIdx.x);
 }
 
-// Step of one block per workgroup.  The 19 population loads of a voxel are
-// issued together with its flag and mask words (their addresses depend only
-// on the block's connectivity row, read with scalar loads), masked links are
-// fixed up afterwards in one batch of loads, and every lane stores, so no
-// cache line of fout is written partially: inactive voxels receive 0 (the
-// inactive value of the reference's population fields,
-// RunCavityTwoPop.cu:56-57), active non-bulk voxels are left untouched.
+// Step of one block per workgroup.  The 19 population loads of an active
+// voxel are issued together with its flag word (their addresses depend only
+// on the block's connectivity row, read with scalar loads); inactive voxels
+// load nothing.  Masked links are fixed up afterwards in one batch of loads.
+// No cache line of fout is written partially: an inactive voxel whose
+// 128-byte line also holds active voxels receives 0 (the inactive value of
+// the reference's population fields, RunCavityTwoPop.cu:56-57); lines with
+// no active voxel are not written; active non-bulk voxels are untouched.
 // The flag bits select the wall-link case without per-link lookups:
 // NX_FLAG_OPEN without NX_FLAG_MIXED -- every masked link is missing (porous
 // media: the link contributes the voxel's own opposite population only);
@@ -202,26 +203,37 @@ __global__ void __launch_bounds__(kBV)
     const uint32_t v = threadIdx.x;
     const int x = (int)(v & 7u), y = (int)((v >> 3) & 7u), z = (int)(v >> 6);
     const uint64_t own = (uint64_t)b * kBStride + v;
+    // active bits of this voxel's z-plane (64 voxels) and the connectivity
+    // row: both uniform per wavefront, loaded before the population loads
+    const uint64_t plane_bits =
+        (uint64_t)g.masks[(uint64_t)b * 16 + 2 * (uint32_t)z] |
+        ((uint64_t)g.masks[(uint64_t)b * 16 + 2 * (uint32_t)z + 1] << 32);
     const Conn cb = load_conn(g.conn, b);
-    uint32_t mword = g.masks[(uint64_t)b * 16 + (v >> 5)];
-    uint32_t fl = flags[(uint64_t)b * kBV + v];
+    const bool active = (plane_bits >> (v & 63u)) & 1u;
+    // voxels sharing this voxel's 128-byte line of each population plane
+    constexpr uint32_t kLine = 128u / sizeof(T);
+    const uint64_t line_bits = kLine >= 64 ? ~0ull : ((1ull << kLine) - 1ull) << ((v & 63u) & ~(kLine - 1u));
+    const bool line_any = (plane_bits & line_bits) != 0;
+    uint32_t fl = 0;
     T val[Q];
-    for_q(AllQ{}, [&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        if constexpr (q == 9) {
-            val[9] = fin[own + 9 * kBV];
-        } else {
-            uint32_t t, nv;
-            upstream<q>(b, cb, x, y, z, t, nv);
-            const uint64_t a = t != kInvalidBlock ? (uint64_t)t * kBStride + (uint32_t)q * kBV + nv : own;
-            val[q] = fin[a];
-        }
-    });
+    if (active) {
+        fl = flags[(uint64_t)b * kBV + v];
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (q == 9) {
+                val[9] = fin[own + 9 * kBV];
+            } else {
+                uint32_t t, nv;
+                upstream<q>(b, cb, x, y, z, t, nv);
+                const uint64_t a =
+                    t != kInvalidBlock ? (uint64_t)t * kBStride + (uint32_t)q * kBV + nv : own;
+                val[q] = fin[a];
+            }
+        });
 #pragma unroll
-    for (int q = 0; q < Q; q++) nx_pin(val[q]);
-    nx_pin(fl);
-    nx_pin(mword);
-    const bool active = (mword >> (v & 31u)) & 1u;
+        for (int q = 0; q < Q; q++) nx_pin(val[q]);
+        nx_pin(fl);
+    }
     if (active && cls_of(fl) == NX_CELL_BULK) {
         P f[Q];
 #pragma unroll
@@ -260,7 +272,9 @@ __global__ void __launch_bounds__(kBV)
 #pragma unroll
         for (int q = 0; q < Q; q++)
             __builtin_nontemporal_store(Stor<S>::template enc<R>(out[q], q), &fout[own + q * kBV]);
-    } else if (!active) {
+    } else if (!active && line_any) {
+        // inactive voxel in a line that also holds active voxels: write the
+        // inactive value so the line is written whole
 #pragma unroll
         for (int q = 0; q < Q; q++) __builtin_nontemporal_store(T(0), &fout[own + q * kBV]);
     }

@@ -87,3 +87,39 @@ extern "C" int nx_geometry_spheres(const int32_t dim[3], uint32_t n_spheres, dou
     for (auto& x : th) x.join();
     return NX_OK;
 }
+
+// Lid-driven cavity classes (RunCavityTwoPop.cu:208-224: outer layer
+// bounce-back, y = Y-1 moving wall) plus n_boxes axis-aligned bounce-back
+// boxes inside the fluid: per box, edges e_a uniform in {emin..emax} and a
+// lower corner uniform in {1 .. dim_a-1-e_a} (draw order ex, ey, ez, x0, y0,
+// z0), clipped to the interior.
+extern "C" int nx_geometry_cavity(const int32_t dim[3], uint32_t n_boxes, int32_t emin,
+                                  int32_t emax, uint64_t seed, uint8_t* cls)
+{
+    if (!dim || !cls) return nx::set_error(NX_ERR_INVALID, "null pointer");
+    if (dim[0] < 3 || dim[1] < 3 || dim[2] < 3 || emin < 1 || emax < emin)
+        return nx::set_error(NX_ERR_INVALID, "nx_geometry_cavity: bad dimensions or edges");
+    const int X = dim[0], Y = dim[1], Z = dim[2];
+    for (int z = 0; z < Z; z++)
+        for (int y = 0; y < Y; y++)
+            for (int x = 0; x < X; x++) {
+                uint8_t c = NX_CELL_BULK;
+                if (x == 0 || x == X - 1 || y == 0 || y == Y - 1 || z == 0 || z == Z - 1)
+                    c = y == Y - 1 ? NX_CELL_MOVINGWALL : NX_CELL_BOUNCEBACK;
+                cls[(size_t)x + (size_t)X * ((size_t)y + (size_t)Y * z)] = c;
+            }
+    SplitMix64 rng{seed};
+    auto pick = [&](int lo, int hi) {  // uniform integer in [lo, hi]
+        return lo + (int)(rng.uniform() * (double)(hi - lo + 1));
+    };
+    for (uint32_t k = 0; k < n_boxes; k++) {
+        int e[3], o[3];
+        for (int a = 0; a < 3; a++) e[a] = pick(emin, emax);
+        for (int a = 0; a < 3; a++) o[a] = pick(1, std::max(1, dim[a] - 1 - e[a]));
+        for (int z = o[2]; z < std::min(o[2] + e[2], Z - 1); z++)
+            for (int y = o[1]; y < std::min(o[1] + e[1], Y - 1); y++)
+                for (int x = o[0]; x < std::min(o[0] + e[0], X - 1); x++)
+                    cls[(size_t)x + (size_t)X * ((size_t)y + (size_t)Y * z)] = NX_CELL_BOUNCEBACK;
+    }
+    return NX_OK;
+}

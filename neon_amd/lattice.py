@@ -49,6 +49,16 @@ def spheres(dim, n: int, rmin: float, rmax: float, seed: int, periodic: int = 0)
     return out
 
 
+def cavity_classes(dim, n_boxes: int = 0, emin: int = 4, emax: int = 16, seed: int = 0) -> np.ndarray:
+    """Cavity classification (uint8 NX_CELL_*, x fastest) with seeded
+    bounce-back boxes (nx_geometry_cavity)."""
+    d = (C.c_int32 * 3)(*[int(v) for v in dim])
+    out = np.empty(int(dim[0]) * int(dim[1]) * int(dim[2]), np.uint8)
+    check(lib.nx_geometry_cavity(d, int(n_boxes), int(emin), int(emax), int(seed),
+                                 out.ctypes.data))
+    return out
+
+
 class DeviceBuffer:
     """Owning device allocation."""
 
@@ -439,3 +449,82 @@ class BlockLattice(_Lattice):
     def _residual(self, u_prev, stream):
         check(lib.nx_d3q19_residual_block(C.byref(self.desc), self.fin.ptr, self.flags.ptr,
                                           u_prev.ptr, self._out.ptr, stream))
+
+
+class CavityBatch:
+    """Many independent dense cavities stepped by one persistent launch
+    (nx_batch_*).  members: list of dicts with keys dim, classes (uint8
+    NX_CELL_*), omega, ulb, ulid."""
+
+    def __init__(self, members, storage="f32"):
+        init()
+        self.n = len(members)
+        self.storage = _storage(storage)
+        dims = np.array([m["dim"] for m in members], np.int32).reshape(-1)
+        classes = np.concatenate([np.ascontiguousarray(m["classes"], np.uint8).reshape(-1)
+                                  for m in members])
+        omega = np.array([m["omega"] for m in members], np.float64)
+        ulb = np.array([m.get("ulb", 0.05) for m in members], np.float64)
+        ulid = np.array([m.get("ulid", (1.0, 0.0, 0.0)) for m in members], np.float64).reshape(-1)
+        h = C.c_void_p()
+        check(lib.nx_batch_create(self.n, dims.ctypes.data, classes.ctypes.data,
+                                  omega.ctypes.data, ulb.ctypes.data, ulid.ctypes.data,
+                                  self.storage, C.byref(h)))
+        self.h = h.value
+        self.dims = [tuple(int(v) for v in m["dim"]) for m in members]
+        nm, nc, nt = C.c_uint32(), C.c_uint64(), C.c_uint32()
+        check(lib.nx_batch_info(self.h, C.byref(nm), C.byref(nc), C.byref(nt)))
+        self.n_cells, self.n_tiles = int(nc.value), int(nt.value)
+
+    def init(self, rho=None, u=None, stream=None):
+        """Equilibrium start from device buffers (arena cell order) or rest."""
+        check(lib.nx_batch_init(self.h, rho.ptr if rho is not None else None,
+                                u.ptr if u is not None else None, stream))
+
+    def step(self, k: int, stream=None):
+        check(lib.nx_batch_step(self.h, int(k), stream))
+
+    def run(self, max_steps: int, check_every: int, tol: float):
+        steps = np.zeros(self.n, np.int32)
+        res = np.zeros(self.n, np.float64)
+        left = C.c_uint32()
+        check(lib.nx_batch_run(self.h, int(max_steps), int(check_every), float(tol),
+                               steps.ctypes.data, res.ctypes.data, C.byref(left)))
+        return steps, res, int(left.value)
+
+    def set_launch(self, wg: int, wgs_per_cu: int):
+        check(lib.nx_batch_set_launch(self.h, int(wg), int(wgs_per_cu)))
+
+    def autotune(self, stream=None):
+        w, c, ms = C.c_int32(), C.c_int32(), C.c_float()
+        check(lib.nx_batch_autotune(self.h, stream, C.byref(w), C.byref(c), C.byref(ms)))
+        return {"wg": int(w.value), "wgs_per_cu": int(c.value)}, float(ms.value)
+
+    def member(self, k: int, which: int = 0):
+        """(desc, populations pointer, flags pointer) of member k."""
+        d, p, f = abi.DenseDesc(), C.c_void_p(), C.c_void_p()
+        check(lib.nx_batch_member(self.h, int(k), int(which), C.byref(d), C.byref(p), C.byref(f)))
+        return d, p.value, f.value
+
+    def populations(self, k: int) -> np.ndarray:
+        d, p, _ = self.member(k)
+        n = 19 * d.dim[0] * d.dim[1] * d.dim[2]
+        out = np.empty(n, _NP_STORAGE[self.storage])
+        check(lib.nx_d2h(out.ctypes.data, p, out.nbytes, None))
+        return out
+
+    def moments(self, k: int):
+        d, p, f = self.member(k)
+        n = d.dim[0] * d.dim[1] * d.dim[2]
+        mt = np.float64 if self.storage == abi.NX_F64 else np.float32
+        it = np.dtype(mt).itemsize
+        rho, u = DeviceBuffer(n * it), DeviceBuffer(3 * n * it)
+        check(lib.nx_d3q19_moments_dense(C.byref(d), p, f, rho.ptr, u.ptr, None))
+        X, Y, Z = d.dim[0], d.dim[1], d.dim[2]
+        return (rho.download(mt, n).reshape(Z, Y, X), u.download(mt, 3 * n).reshape(3, Z, Y, X))
+
+    def __del__(self):
+        try:
+            lib.nx_batch_destroy(self.h)
+        except Exception:
+            pass

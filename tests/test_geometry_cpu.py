@@ -50,3 +50,29 @@ def test_spheres_match_reference():
         ref = reference(dim, 25, 2.0, 6.0, 123, per)
         assert np.array_equal(got, ref), (dim, per, int((got != ref).sum()))
         assert 0 < got.sum() < got.size
+
+
+def cavity_reference(dim, n_boxes, emin, emax, seed):
+    X, Y, Z = dim
+    c = np.full((Z, Y, X), 2, np.uint8)
+    c[0, :, :] = c[-1, :, :] = 0
+    c[:, 0, :] = 0
+    c[:, :, 0] = c[:, :, -1] = 0
+    c[:, -1, :] = 1
+    g = splitmix(seed)
+    u = lambda: (next(g) >> 11) * (1.0 / 9007199254740992.0)
+    pick = lambda lo, hi: lo + int(u() * (hi - lo + 1))
+    for _ in range(n_boxes):
+        e = [pick(emin, emax) for _ in range(3)]
+        o = [pick(1, max(1, d - 1 - ea)) for d, ea in zip(dim, e)]
+        c[o[2]:min(o[2] + e[2], Z - 1), o[1]:min(o[1] + e[1], Y - 1),
+          o[0]:min(o[0] + e[0], X - 1)] = 0
+    return c.reshape(-1)
+
+
+def test_cavity_geometry_matches_reference():
+    import neon_amd as nx
+    for dim, nb, seed in [((32, 40, 24), 4, 7), ((20, 20, 20), 0, 1), ((17, 9, 33), 3, 99)]:
+        got = nx.cavity_classes(dim, nb, 4, 16, seed)
+        ref = cavity_reference(dim, nb, 4, 16, seed)
+        assert np.array_equal(got, ref), (dim, int((got != ref).sum()))
