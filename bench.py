@@ -51,6 +51,16 @@ POROUS = {
 }
 
 
+# Batched cavities: 512 independent lid-driven cubes, edge U{32..96} step 8,
+# Re U[50,1000], lid 0.05, 0-4 seeded boxes with edges U{4..16}; one
+# persistent launch steps every member.
+BATCH = {
+    # name: (members, storage, default steps, default warmup)
+    "batch512_f32": (512, "f32", 200, 20),
+    "batch512_f16dev": (512, "f16dev", 200, 20),
+}
+
+
 def traffic_from_profiles(kernel_tag: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, scaled by a copy kernel of known size), if committed."""
@@ -63,7 +73,7 @@ def traffic_from_profiles(kernel_tag: str):
         return None
 
 
-def cpu_baseline(N, storage, compute, omega, budget_s=12.0):
+def cpu_baseline(N, storage, compute, omega, budget_s=12.0, max_steps=50):
     """Oracle (the C restatement of the reference step) on the host, all
     threads, a few steps of the same grid."""
     from oracle import oracle as orc
@@ -77,7 +87,7 @@ def cpu_baseline(N, storage, compute, omega, budget_s=12.0):
         o.run(1, omega)
         steps += 1
         el = time.perf_counter() - t0
-        if el > budget_s or steps >= 50:
+        if el > budget_s or steps >= max_steps:
             break
     fluid = (N - 2) ** 3
     return {"value": fluid * steps / el / 1e6, "unit": "MLUPS", "cores": int(orc.lib.or_get_threads()),
@@ -182,12 +192,81 @@ def porous(nx, a):
     print(json.dumps(out), flush=True)
 
 
+def batched(nx, a):
+    n, storage, dsteps, dwarm = BATCH[a.workload]
+    steps = dsteps if a.steps is None else a.steps
+    warmup = dwarm if a.warmup is None else a.warmup
+    steps += steps % 2                    # the batch steps in pairs (arena ping-pong)
+    warmup += warmup % 2
+    ulb = 0.05
+    nx.init(0)
+    rng = np.random.default_rng(2025)
+    t0 = time.perf_counter()
+    members, fluid = [], 0
+    for k in range(n):
+        e = int(rng.choice(np.arange(32, 97, 8)))
+        re = float(rng.uniform(50.0, 1000.0))
+        nb = int(rng.integers(0, 5))
+        cls = nx.cavity_classes((e, e, e), nb, 4, 16, seed=int(rng.integers(0, 2 ** 63)))
+        fluid += int((cls == 2).sum())
+        nu = ulb * e / re
+        members.append({"dim": (e, e, e), "classes": cls, "omega": 1.0 / (3.0 * nu + 0.5),
+                        "ulb": ulb, "ulid": (1.0, 0.0, 0.0)})
+    b = nx.CavityBatch(members, storage)
+    t_setup = time.perf_counter() - t0
+    launch = None
+    if a.autotune:
+        launch, _ = b.autotune()
+    b.init()
+    stream = nx.Stream()
+    b.step(warmup, stream=stream.handle)
+    stream.sync()
+    e0, e1 = nx.Event(), nx.Event()
+    nx.sync()
+    tw = time.perf_counter()
+    e0.record(stream)
+    b.step(steps, stream=stream.handle)
+    e1.record(stream)
+    e1.sync()
+    wall = time.perf_counter() - tw
+    ms = e0.elapsed_ms(e1)
+    ms_step = ms / steps
+    cells = b.n_cells
+    mlups = fluid * steps / (ms * 1e-3) / 1e6
+    b_alg = 2 * 19 * STORAGE_BYTES[storage]
+    achieved = fluid * b_alg / (ms_step * 1e-3) / 1e9
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(64, storage, "f32", 1.0 / (3.0 * ulb * 64 / 500.0 + 0.5), max_steps=4000)
+        cpu["sample"] = "oracle step of one 64^3 member-sized cavity, " + cpu["sample"]
+    out = {
+        "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": storage,
+        "data": "synthetic (seeded cavities with box obstacles, initialised at rest)",
+        "config": {"workload": f"{n} batched lid-driven cavities, edge U{{32..96}} step 8, "
+                               f"Re U[50,1000], 0-4 boxes U{{4..16}}, {storage} storage, one "
+                               f"persistent launch per step, {steps} timed steps after {warmup}",
+                   "members": n, "cells": cells, "fluid_cells": fluid,
+                   "arena_GB": round(2 * 19 * STORAGE_BYTES[storage] * cells / 1e9, 2),
+                   "setup_s": round(t_setup, 2), "launch": launch, "device": nx.device_name()},
+        "wall_s": round(wall, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(f"d3q19_batch_{storage}_{n}"),
+                     "kernel": f"k_batch_step<{storage}> ({b_alg} B/fluid cell)"},
+        "velocity_error_vs_f64": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="cavity256_f32", choices=sorted(WORKLOADS) + sorted(POROUS))
+    ap.add_argument("--workload", default="cavity256_f32", choices=sorted(WORKLOADS) + sorted(POROUS) + sorted(BATCH))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--autotune", action="store_true",
@@ -210,6 +289,8 @@ def main():
 
     if a.workload in POROUS:
         return porous(nx, a)
+    if a.workload in BATCH:
+        return batched(nx, a)
 
     N, storage, compute, re, dsteps, dwarm = WORKLOADS[a.workload]
     steps = dsteps if a.steps is None else a.steps

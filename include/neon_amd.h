@@ -284,6 +284,68 @@ int nx_geometry_spheres(const int32_t dim[3], uint32_t n_spheres, double rmin, d
 int nx_geometry_cavity(const int32_t dim[3], uint32_t n_boxes, int32_t emin, int32_t emax,
                        uint64_t seed, uint8_t* cls);
 
+/* ---- BF16 3D convolution on the matrix cores (learned warm start; no
+ * counterpart in the reference) --------------------------------------------
+ * Channels-last volumes x[n][z][y][x][c] (bf16 bit patterns), weights
+ * w[co][tap][ci] (bf16), tap = (dz*k + dy)*k + dx over offsets
+ * -(k-1)/2..(k-1)/2, zero padding, stride 1; every product accumulated in
+ * FP32.  shape = {n, depth, height, width} (powers of two); cin, cout
+ * multiples of 16 in [16, 256]; ksize 1 or 3.  x, w, dy 16-byte aligned.
+ *   fwd:        y = conv(x, w) + bias (bias may be NULL), ReLU with
+ *               NX_CONV_RELU; y bf16, or float with NX_CONV_OUT_F32
+ *   bwd_data:   dx = gradient of the convolution w.r.t. x (bf16 or float)
+ *   bwd_weight: dw[co][tap][ci] (float), db[co] (float, may be NULL)
+ * bwd_data and bwd_weight use the library's per-device workspace: calls
+ * from several host threads on one device must not overlap. */
+#define NX_CONV_RELU 1
+#define NX_CONV_OUT_F32 2
+int nx_conv3d_fwd_bf16(const uint16_t* x, const uint16_t* w, const float* bias, void* y,
+                       const int32_t shape[4], int32_t cin, int32_t cout, int32_t ksize,
+                       int32_t flags, nx_stream s);
+int nx_conv3d_bwd_data_bf16(const uint16_t* dy, const uint16_t* w, void* dx,
+                            const int32_t shape[4], int32_t cin, int32_t cout, int32_t ksize,
+                            int32_t flags, nx_stream s);
+int nx_conv3d_bwd_weight_bf16(const uint16_t* x, const uint16_t* dy, float* dw, float* db,
+                              const int32_t shape[4], int32_t cin, int32_t cout,
+                              int32_t ksize, nx_stream s);
+
+/* ---- U-Net surrogate for the warm start (no counterpart in the reference) -
+ * levels 2..4, widths multiples of 16 (<= 128), input [n][size^3][in_ch]
+ * float (in_ch <= 16), output [n][size^3][out_ch] float (out_ch <= 16),
+ * cubes of side `size` (power of two).  Encoder: two 3^3 convolutions + ReLU
+ * per level and 2x2x2 average pooling; decoder: nearest upsampling, skip
+ * concatenation, two 3^3 convolutions + ReLU; a 1^3 output convolution.  BF16
+ * weights and activations, FP32 accumulation, FP32 master weights.  The
+ * parameter vector (float) holds, per convolution in the order encoder
+ * (level 0..L-1, 2 each), decoder (level 0..L-2, 2 each), output:
+ * W[co][tap][ci] then b[co] (first convolution cin = 16, output cout = 16,
+ * channels past in_ch / out_ch unused).
+ * nx_unet_train_step: forward, loss = sum_v weight[v] sum_c (y - target)^2
+ * (weight [n][size^3]), backward, one Adam step (beta 0.9/0.999, eps 1e-8);
+ * *loss (host, may be NULL: then no synchronisation) receives the loss
+ * before the step.  nx_unet_backward: parameter gradients (nx_unet_grads)
+ * of sum(y * dy) for the input of the last nx_unet_forward. */
+typedef struct nx_unet_s* nx_unet;
+typedef struct {
+    int32_t levels;
+    int32_t width[4];
+    int32_t size;
+    int32_t max_batch;
+    int32_t in_ch;
+    int32_t out_ch;
+    int32_t reserved;
+    uint64_t seed;      /* He-normal initialisation (SplitMix64 stream) */
+} nx_unet_desc;
+int nx_unet_create(const nx_unet_desc* d, nx_unet* u);
+int nx_unet_info(nx_unet u, uint64_t* n_params, uint32_t* n_convs);
+int nx_unet_params(nx_unet u, float* host, int32_t set);
+int nx_unet_grads(nx_unet u, float* host);
+int nx_unet_forward(nx_unet u, const float* x, int32_t batch, float* y, nx_stream s);
+int nx_unet_backward(nx_unet u, const float* dy, int32_t batch, nx_stream s);
+int nx_unet_train_step(nx_unet u, const float* x, const float* target, const float* weight,
+                       int32_t batch, float lr, double* loss, nx_stream s);
+int nx_unet_destroy(nx_unet u);
+
 #ifdef __cplusplus
 }
 #endif

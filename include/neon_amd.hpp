@@ -1,0 +1,631 @@
+// Neon-shaped C++ host API over the C ABI of neon_amd.h: Backend, dGrid /
+// bGrid, Field, Container, Skeleton and the D3Q19 LBM containers, with the
+// names, argument meanings and error behaviour of the reference's
+// lid-driven-cavity benchmark path:
+//   Neon::Backend                  libNeonSet/include/Neon/set/Backend.h
+//   Neon::dGrid / Neon::bGrid      libNeonDomain/include/Neon/domain/{dGrid,bGrid}.h
+//   Field::forEachActiveCell, updateDeviceData, updateHostData
+//                                  libNeonDomain/include/Neon/domain/interface/FieldBase.h
+//   Neon::set::Container::run      libNeonSet/include/Neon/set/Containter.h:25-35
+//   Neon::skeleton::Skeleton       libNeonSkeleton/include/Neon/skeleton/Skeleton.h:56-65
+//   LbmContainers, LbmIterationD3Q19
+//                                  benchmarks/lbm-lid-driven-cavity-flow/src/{LbmTools.h,LbmIteration.h}
+// Header-only, C++17; link with libneon_amd.so.  One process drives one GPU:
+// a Backend takes exactly one device id (the reference's multi-device
+// partitioning collapses to one partition, so there is no halo exchange and
+// the Occ / TransferMode / StencilSemantic options are accepted and have
+// nothing to overlap or exchange).  Errors throw neon_amd::NeonException
+// carrying nx_last_error(), where the reference throws Neon::NeonException.
+#pragma once
+
+#include "neon_amd.h"
+
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+namespace neon_amd {
+
+class NeonException : public std::runtime_error
+{
+   public:
+    using std::runtime_error::runtime_error;
+};
+
+inline void check(int code, const std::string& where)
+{
+    if (code != NX_OK) throw NeonException(where + ": " + nx_last_error());
+}
+
+struct index_3d
+{
+    int32_t x = 0, y = 0, z = 0;
+    index_3d() = default;
+    index_3d(int32_t x_, int32_t y_, int32_t z_) : x(x_), y(y_), z(z_) {}
+    size_t rMulTyped() const { return size_t(x) * size_t(y) * size_t(z); }
+};
+
+enum class Runtime { stream };
+enum class Occ { none, standard };
+enum class TransferMode { get, put };
+enum class StencilSemantic { standard, streaming };
+
+// ---- Backend ---------------------------------------------------------------
+
+class Backend
+{
+    struct State
+    {
+        int                    device = 0;
+        std::vector<nx_stream> streams;
+        ~State()
+        {
+            for (nx_stream s : streams) nx_stream_destroy(s);
+        }
+    };
+    std::shared_ptr<State> s_;
+
+   public:
+    static constexpr int mainStreamIdx = 0;
+
+    Backend() = default;
+    explicit Backend(const std::vector<int>& devIds, Runtime = Runtime::stream, int nStreams = 1)
+    {
+        if (devIds.size() != 1)
+            throw NeonException("Backend: one process drives one GPU; pass exactly one device id");
+        check(nx_init(devIds[0]), "Backend");
+        s_ = std::make_shared<State>();
+        s_->device = devIds[0];
+        for (int i = 0; i < nStreams; i++) {
+            nx_stream st = nullptr;
+            check(nx_stream_create(&st), "Backend: stream");
+            s_->streams.push_back(st);
+        }
+    }
+    nx_stream stream(int idx = mainStreamIdx) const
+    {
+        if (!s_ || idx < 0 || idx >= int(s_->streams.size())) throw NeonException("Backend: bad stream index");
+        return s_->streams[idx];
+    }
+    void sync(int idx) const { check(nx_stream_sync(stream(idx)), "Backend::sync"); }
+    void syncAll() const { check(nx_sync(), "Backend::syncAll"); }
+    int  deviceId() const { return s_ ? s_->device : -1; }
+    std::string toString() const
+    {
+        char name[256] = {0};
+        nx_device_name(name, sizeof(name));
+        return std::string("Backend gpu:") + std::to_string(deviceId()) + " " + name;
+    }
+};
+
+// ---- cell classification field element --------------------------------------
+
+// CellType (benchmarks/lbm-lid-driven-cavity-flow/src/CellType.h): packed on
+// the device into one 32-bit word, classification in bits 24..25, the wall
+// neighbour bit flags (and the library's flag bits 28, 29) in the rest.
+struct CellType
+{
+    enum Classification : uint32_t { bounceBack = 0, movingWall = 1, bulk = 2, undefined = 3 };
+    Classification classification = undefined;
+    uint32_t       wallNghBitflag = 0;
+
+    uint32_t pack() const
+    {
+        return (uint32_t(classification) << NX_CLASS_SHIFT) | (wallNghBitflag & ~(3u << NX_CLASS_SHIFT));
+    }
+    static CellType unpack(uint32_t w)
+    {
+        CellType c;
+        c.classification = Classification((w >> NX_CLASS_SHIFT) & 3u);
+        c.wallNghBitflag = w & ~(3u << NX_CLASS_SHIFT);
+        return c;
+    }
+};
+
+// Deviation storage types (populations stored as f - w_q; no counterpart in
+// the reference): host mirrors hold the stored encoding.
+struct F32Dev { float v = 0.f; };
+struct F16Dev { uint16_t bits = 0; };
+
+template <typename T> struct StorageOf;
+template <> struct StorageOf<double> { static constexpr int value = NX_F64; };
+template <> struct StorageOf<float> { static constexpr int value = NX_F32; };
+template <> struct StorageOf<F32Dev> { static constexpr int value = NX_F32DEV; };
+template <> struct StorageOf<F16Dev> { static constexpr int value = NX_F16DEV; };
+
+// Compute types: double; float with the reference's expression semantics
+// (its double literals promote parts of the float collision); FastFloat is
+// float throughout (the bandwidth-bound fast path).
+struct FastFloat {};
+template <typename T> struct ComputeOf;
+template <> struct ComputeOf<double> { static constexpr int value = NX_COMPUTE_F64; };
+template <> struct ComputeOf<float> { static constexpr int value = NX_COMPUTE_REF; };
+template <> struct ComputeOf<FastFloat> { static constexpr int value = NX_COMPUTE_F32; };
+
+template <typename Grid, typename T, int C>
+class Field;
+
+// ---- dGrid -------------------------------------------------------------------
+
+// Dense grid of one partition (dGrid_imp.h:7-185): every cell of the box is
+// allocated; field layout SoA, slot = card * n + (z * Y + y) * X + x.
+class dGrid
+{
+    struct State
+    {
+        Backend  bk;
+        index_3d dim;
+        int      periodic = 0;
+    };
+    std::shared_ptr<State> s_;
+
+   public:
+    template <typename T, int C>
+    using Field = neon_amd::Field<dGrid, T, C>;
+
+    dGrid() = default;
+    dGrid(const Backend& bk, index_3d dim, const std::function<bool(const index_3d&)>& = {},
+          const void* /*stencil*/ = nullptr, int periodic = 0)
+    {
+        if (dim.x < 1 || dim.y < 1 || dim.z < 1) throw NeonException("dGrid: empty dimension");
+        s_ = std::make_shared<State>();
+        s_->bk = bk;
+        s_->dim = dim;
+        s_->periodic = periodic;
+    }
+    template <typename T, int C = 0>
+    Field<T, C> newField(const std::string& name, int cardinality, T inactiveValue) const
+    {
+        return Field<T, C>(*this, name, cardinality, inactiveValue);
+    }
+    index_3d       getDimension() const { return s_->dim; }
+    const Backend& getBackend() const { return s_->bk; }
+    int            periodic() const { return s_->periodic; }
+    size_t         nSlots() const { return s_->dim.rMulTyped(); }
+    size_t         slot(size_t cell, int card, int) const { return size_t(card) * nSlots() + cell; }
+    template <class F>
+    void forEachActive(F&& f) const  // f(index_3d, cell)
+    {
+        const index_3d d = s_->dim;
+        size_t         cell = 0;
+        for (int z = 0; z < d.z; z++)
+            for (int y = 0; y < d.y; y++)
+                for (int x = 0; x < d.x; x++) f(index_3d(x, y, z), cell++);
+    }
+    nx_dense_desc desc(int storage, int compute) const
+    {
+        nx_dense_desc d{};
+        d.dim[0] = s_->dim.x;
+        d.dim[1] = s_->dim.y;
+        d.dim[2] = s_->dim.z;
+        d.storage = storage;
+        d.compute = compute;
+        d.periodic = s_->periodic;
+        return d;
+    }
+};
+
+// ---- bGrid -------------------------------------------------------------------
+
+// Block-sparse grid of one partition, 8^3 blocks (bGrid_imp.h:19-246): blocks
+// holding an active voxel; field layout [block][card][512], voxel x+8y+64z.
+class bGrid
+{
+    struct State
+    {
+        Backend               bk;
+        index_3d              dim;
+        int                   periodic = 0;
+        nx_bgrid              g = nullptr;
+        uint32_t              nBlocks = 0;
+        uint64_t              nActive = 0;
+        std::vector<int32_t>  origins;
+        std::vector<uint32_t> masks, conn;
+        void *                dOrigins = nullptr, *dMasks = nullptr, *dConn = nullptr;
+        ~State()
+        {
+            nx_free(dOrigins);
+            nx_free(dMasks);
+            nx_free(dConn);
+            nx_bgrid_destroy(g);
+        }
+    };
+    std::shared_ptr<State> s_;
+
+    static int trampoline(int32_t x, int32_t y, int32_t z, void* user)
+    {
+        return (*static_cast<const std::function<bool(const index_3d&)>*>(user))(index_3d(x, y, z)) ? 1 : 0;
+    }
+
+   public:
+    template <typename T, int C>
+    using Field = neon_amd::Field<bGrid, T, C>;
+    static constexpr int kBlockVoxels = 512;
+
+    bGrid() = default;
+    bGrid(const Backend& bk, index_3d dim, std::function<bool(const index_3d&)> activeCell,
+          const void* /*stencil*/ = nullptr, int periodic = 0)
+    {
+        s_ = std::make_shared<State>();
+        s_->bk = bk;
+        s_->dim = dim;
+        s_->periodic = periodic;
+        if (!activeCell) activeCell = [](const index_3d&) { return true; };
+        const int32_t d[3] = {dim.x, dim.y, dim.z};
+        check(nx_bgrid_create_cb(d, &trampoline, &activeCell, periodic, &s_->g), "bGrid");
+        check(nx_bgrid_info(s_->g, &s_->nBlocks, &s_->nActive), "bGrid");
+        const size_t n = s_->nBlocks;
+        s_->origins.resize(3 * n);
+        s_->masks.resize(16 * n);
+        s_->conn.resize(27 * n);
+        check(nx_bgrid_tables(s_->g, s_->origins.data(), s_->masks.data(), s_->conn.data()), "bGrid");
+        auto up = [&](const void* h, size_t bytes, void** dptr) {
+            check(nx_alloc(bytes ? bytes : 4, dptr), "bGrid");
+            if (bytes) check(nx_h2d(*dptr, h, bytes, nullptr), "bGrid");
+        };
+        up(s_->origins.data(), s_->origins.size() * 4, &s_->dOrigins);
+        up(s_->masks.data(), s_->masks.size() * 4, &s_->dMasks);
+        up(s_->conn.data(), s_->conn.size() * 4, &s_->dConn);
+        check(nx_sync(), "bGrid");
+    }
+    template <typename T, int C = 0>
+    Field<T, C> newField(const std::string& name, int cardinality, T inactiveValue) const
+    {
+        return Field<T, C>(*this, name, cardinality, inactiveValue);
+    }
+    index_3d       getDimension() const { return s_->dim; }
+    const Backend& getBackend() const { return s_->bk; }
+    uint32_t       nBlocks() const { return s_->nBlocks; }
+    uint64_t       nActiveCells() const { return s_->nActive; }
+    size_t         nSlots() const { return size_t(s_->nBlocks) * kBlockVoxels; }
+    size_t         slot(size_t cell, int card, int cardinality) const
+    {
+        const size_t b = cell / kBlockVoxels, v = cell % kBlockVoxels;
+        return (b * size_t(cardinality) + size_t(card)) * kBlockVoxels + v;
+    }
+    bool isActive(size_t cell) const
+    {
+        const size_t b = cell / kBlockVoxels, v = cell % kBlockVoxels;
+        return (s_->masks[b * 16 + v / 32] >> (v % 32)) & 1u;
+    }
+    template <class F>
+    void forEachActive(F&& f) const  // f(index_3d, cell = block * 512 + voxel)
+    {
+        for (uint32_t b = 0; b < s_->nBlocks; b++)
+            for (int v = 0; v < kBlockVoxels; v++) {
+                if (!((s_->masks[size_t(b) * 16 + v / 32] >> (v % 32)) & 1u)) continue;
+                const index_3d o(s_->origins[3 * b], s_->origins[3 * b + 1], s_->origins[3 * b + 2]);
+                f(index_3d(o.x + (v & 7), o.y + ((v >> 3) & 7), o.z + (v >> 6)), size_t(b) * kBlockVoxels + v);
+            }
+    }
+    nx_block_desc desc(int storage, int compute) const
+    {
+        nx_block_desc d{};
+        d.dim[0] = s_->dim.x;
+        d.dim[1] = s_->dim.y;
+        d.dim[2] = s_->dim.z;
+        d.storage = storage;
+        d.compute = compute;
+        d.periodic = s_->periodic;
+        d.n_blocks = s_->nBlocks;
+        d.origins = static_cast<const int32_t*>(s_->dOrigins);
+        d.masks = static_cast<const uint32_t*>(s_->dMasks);
+        d.conn = static_cast<const uint32_t*>(s_->dConn);
+        return d;
+    }
+};
+
+// ---- Field -------------------------------------------------------------------
+
+template <typename T> struct DeviceRepr { using type = T; };
+template <> struct DeviceRepr<CellType> { using type = uint32_t; };
+
+// A field of `cardinality` values per cell with a host mirror (the
+// reference's MemMirror): forEachActiveCell edits the host copy,
+// updateDeviceData / updateHostData move it (asynchronously on the stream).
+template <typename GridT, typename T, int C>
+class Field
+{
+    using D = typename DeviceRepr<T>::type;
+    struct State
+    {
+        GridT          grid;
+        std::string    name;
+        int            card = 1;
+        T              inactive{};
+        std::vector<T> host;
+        std::vector<D> staging;
+        void*          dev = nullptr;
+        ~State() { nx_free(dev); }
+    };
+    std::shared_ptr<State> s_;
+
+   public:
+    using Type = T;
+    using Grid = GridT;
+
+    Field() = default;
+    Field(const GridT& g, const std::string& name, int cardinality, T inactiveValue)
+    {
+        if (C != 0 && cardinality != C) throw NeonException("newField: cardinality differs from the template's");
+        s_ = std::make_shared<State>();
+        s_->grid = g;
+        s_->name = name;
+        s_->card = cardinality;
+        s_->inactive = inactiveValue;
+        const size_t n = g.nSlots() * size_t(cardinality);
+        s_->host.assign(n, inactiveValue);
+        check(nx_alloc(n * sizeof(D), &s_->dev), "newField " + name);
+        updateDeviceData(Backend::mainStreamIdx);
+        g.getBackend().sync(Backend::mainStreamIdx);
+    }
+    bool           valid() const { return bool(s_); }
+    const GridT&   getGrid() const { return s_->grid; }
+    const Backend& getBackend() const { return s_->grid.getBackend(); }
+    int            getCardinality() const { return s_->card; }
+    const std::string& getName() const { return s_->name; }
+    void*          devPtr() const { return s_->dev; }
+
+    template <class F>
+    void forEachActiveCell(F&& f)  // f(const index_3d&, const int& card, T& value)
+    {
+        const int card = s_->card;
+        s_->grid.forEachActive([&](const index_3d& idx, size_t cell) {
+            for (int k = 0; k < card; k++) f(idx, k, s_->host[s_->grid.slot(cell, k, card)]);
+        });
+    }
+    T& hostRef(size_t cell, int card) { return s_->host[s_->grid.slot(cell, card, s_->card)]; }
+
+    void updateDeviceData(int streamIdx)
+    {
+        const size_t n = s_->host.size();
+        const void*  src = s_->host.data();
+        if constexpr (!std::is_same_v<T, D>) {
+            s_->staging.resize(n);
+            for (size_t i = 0; i < n; i++) s_->staging[i] = s_->host[i].pack();
+            src = s_->staging.data();
+        }
+        check(nx_h2d(s_->dev, src, n * sizeof(D), getBackend().stream(streamIdx)), "updateDeviceData " + s_->name);
+    }
+    void updateHostData(int streamIdx)
+    {
+        const size_t n = s_->host.size();
+        if constexpr (std::is_same_v<T, D>) {
+            check(nx_d2h(s_->host.data(), s_->dev, n * sizeof(D), getBackend().stream(streamIdx)),
+                  "updateHostData " + s_->name);
+        } else {
+            s_->staging.resize(n);
+            check(nx_d2h(s_->staging.data(), s_->dev, n * sizeof(D), getBackend().stream(streamIdx)),
+                  "updateHostData " + s_->name);
+            getBackend().sync(streamIdx);
+            for (size_t i = 0; i < n; i++) s_->host[i] = T::unpack(s_->staging[i]);
+        }
+    }
+};
+
+// ---- Container / Skeleton --------------------------------------------------
+
+class Container
+{
+    std::string                     name_;
+    Backend                         bk_;
+    std::function<void(nx_stream)>  body_;
+    std::shared_ptr<void>           keep_;  // resources the body uses
+
+   public:
+    Container() = default;
+    Container(std::string name, const Backend& bk, std::function<void(nx_stream)> body,
+              std::shared_ptr<void> keep = {})
+        : name_(std::move(name)), bk_(bk), body_(std::move(body)), keep_(std::move(keep)) {}
+    void run(int streamIdx = Backend::mainStreamIdx) const { body_(bk_.stream(streamIdx)); }
+    const std::string& getName() const { return name_; }
+};
+
+struct SkeletonOptions
+{
+    Occ          occ = Occ::none;
+    TransferMode transfer = TransferMode::get;
+    SkeletonOptions() = default;
+    SkeletonOptions(Occ o, TransferMode t) : occ(o), transfer(t) {}
+};
+
+// A sequence of containers run in order on the main stream (with one
+// partition there is no halo update to insert between them).
+class Skeleton
+{
+    Backend                bk_;
+    std::vector<Container> seq_;
+    std::string            name_;
+
+   public:
+    Skeleton() = default;
+    explicit Skeleton(const Backend& bk) : bk_(bk) {}
+    void sequence(const std::vector<Container>& ops, const std::string& name, SkeletonOptions = {})
+    {
+        seq_ = ops;
+        name_ = name;
+    }
+    void run() const
+    {
+        for (const Container& c : seq_) c.run(Backend::mainStreamIdx);
+    }
+    const std::string& getName() const { return name_; }
+};
+
+// ---- D3Q19 lattice-Boltzmann ------------------------------------------------
+
+namespace lbm {
+
+// D3Q19 velocities in the reference's order (D3Q19.h:22-42): opposite of q is
+// q +- 10 (9 is the rest direction); weights 1/3, 1/18, 1/36.
+struct D3Q19
+{
+    static constexpr int Q = 19;
+    static constexpr int centerDirection = 9;
+    static constexpr int goBackOffset = 10;
+    std::vector<index_3d> c_vect;
+    std::vector<double>   t_vect;
+    D3Q19()
+    {
+        const int cx[Q] = {-1, 0, 0, -1, -1, -1, -1, 0, 0, 0, 1, 0, 0, 1, 1, 1, 1, 0, 0};
+        const int cy[Q] = {0, -1, 0, -1, 1, 0, 0, -1, -1, 0, 0, 1, 0, 1, -1, 0, 0, 1, 1};
+        const int cz[Q] = {0, 0, -1, 0, 0, -1, 1, -1, 1, 0, 0, 0, 1, 0, 0, 1, -1, 1, -1};
+        for (int q = 0; q < Q; q++) {
+            c_vect.emplace_back(cx[q], cy[q], cz[q]);
+            const int l1 = std::abs(cx[q]) + std::abs(cy[q]) + std::abs(cz[q]);
+            t_vect.push_back(l1 == 0 ? 1. / 3. : (l1 == 1 ? 1. / 18. : 1. / 36.));
+        }
+    }
+};
+
+namespace detail {
+template <typename G> struct IsDense : std::false_type {};
+template <> struct IsDense<dGrid> : std::true_type {};
+}  // namespace detail
+
+// LbmContainers<Lattice, PopulationField, LbmComputeType> (LbmTools.h:22-437):
+// the step, the wall-neighbour mask and the moments as containers.
+template <typename PopulationField, typename ComputeType>
+struct LbmContainers
+{
+    using Grid = typename PopulationField::Grid;
+    using Store = typename PopulationField::Type;
+    using CellTypeField = Field<Grid, CellType, 1>;
+    static constexpr int kStorage = StorageOf<Store>::value;
+    static constexpr int kCompute = ComputeOf<ComputeType>::value;
+
+    template <typename Omega>
+    static Container iteration(StencilSemantic, const PopulationField& fIn, const CellTypeField& cellTypeField,
+                               Omega omega, PopulationField& fOut, const double force[3] = nullptr)
+    {
+        const Grid&   g = fIn.getGrid();
+        nx_lbm_params p{};
+        p.omega = double(omega);
+        for (int a = 0; a < 3; a++) p.force[a] = force ? force[a] : 0.0;
+        const void*     fin = fIn.devPtr();
+        void*           fout = fOut.devPtr();
+        const uint32_t* flags = static_cast<const uint32_t*>(cellTypeField.devPtr());
+        if constexpr (detail::IsDense<Grid>::value) {
+            // The library container classifies the cells when it is created,
+            // so it is bound at the first run: the reference builds its
+            // iteration before the cell types are set (RunCavityTwoPop.cu:77-84
+            // vs 208-241).  Cell types must not change after that run.
+            struct Lazy
+            {
+                nx_dense_desc   d;
+                const void*     fin;
+                void*           fout;
+                const uint32_t* flags;
+                nx_lbm_params   p;
+                nx_container    c = nullptr;
+                ~Lazy() { nx_container_destroy(c); }
+            };
+            auto lazy = std::make_shared<Lazy>(Lazy{g.desc(kStorage, kCompute), fin, fout, flags, p});
+            return Container("LbmIteration", g.getBackend(),
+                             [lazy](nx_stream s) {
+                                 if (!lazy->c) {
+                                     check(nx_stream_sync(s), "LbmIteration");
+                                     check(nx_d3q19_iteration_dense(&lazy->d, lazy->fin, lazy->fout, lazy->flags,
+                                                                    &lazy->p, &lazy->c),
+                                           "LbmIteration");
+                                 }
+                                 check(nx_container_run(lazy->c, s), "LbmIteration");
+                             },
+                             lazy);
+        } else {
+            const nx_block_desc d = g.desc(kStorage, kCompute);
+            return Container("LbmIteration", g.getBackend(), [d, fin, fout, flags, p](nx_stream s) {
+                check(nx_d3q19_step_block(&d, fin, fout, flags, &p, s), "LbmIteration");
+            });
+        }
+    }
+
+    static Container computeWallNghMask(const CellTypeField& infoInField, CellTypeField& infoOutField)
+    {
+        const Grid&     g = infoInField.getGrid();
+        const uint32_t* in = static_cast<const uint32_t*>(infoInField.devPtr());
+        uint32_t*       out = static_cast<uint32_t*>(infoOutField.devPtr());
+        if constexpr (detail::IsDense<Grid>::value) {
+            const nx_dense_desc d = g.desc(kStorage, kCompute);
+            return Container("LBM_computeWallNghMask", g.getBackend(), [d, in, out](nx_stream s) {
+                check(nx_d3q19_wall_mask_dense(&d, in, out, s), "computeWallNghMask");
+            });
+        } else {
+            const nx_block_desc d = g.desc(kStorage, kCompute);
+            return Container("LBM_computeWallNghMask", g.getBackend(), [d, in, out](nx_stream s) {
+                check(nx_d3q19_wall_mask_block(&d, in, out, s), "computeWallNghMask");
+            });
+        }
+    }
+
+    // rho [1], u [3]: double fields for double storage, float otherwise
+    template <typename RhoField, typename UField>
+    static Container computeRhoAndU(const PopulationField& fInField, const CellTypeField& cellTypeField,
+                                    RhoField& rhoField, UField& uField)
+    {
+        using M = std::conditional_t<kStorage == NX_F64, double, float>;
+        static_assert(std::is_same_v<typename RhoField::Type, M> && std::is_same_v<typename UField::Type, M>,
+                      "rho and u fields: double for double storage, float otherwise");
+        const Grid&     g = fInField.getGrid();
+        const void*     fin = fInField.devPtr();
+        const uint32_t* flags = static_cast<const uint32_t*>(cellTypeField.devPtr());
+        void*           rho = rhoField.devPtr();
+        void*           u = uField.devPtr();
+        if constexpr (detail::IsDense<Grid>::value) {
+            const nx_dense_desc d = g.desc(kStorage, kCompute);
+            return Container("LBM_computeRhoAndU", g.getBackend(), [d, fin, flags, rho, u](nx_stream s) {
+                check(nx_d3q19_moments_dense(&d, fin, flags, rho, u, s), "computeRhoAndU");
+            });
+        } else {
+            const nx_block_desc d = g.desc(kStorage, kCompute);
+            return Container("LBM_computeRhoAndU", g.getBackend(), [d, fin, flags, rho, u](nx_stream s) {
+                check(nx_d3q19_moments_block(&d, fin, flags, rho, u, s), "computeRhoAndU");
+            });
+        }
+    }
+};
+
+// LbmIterationD3Q19 (LbmIteration.h:17-101): two skeletons, one per
+// direction of the population ping-pong, alternated by run().
+template <typename PopulationField, typename ComputeType>
+struct LbmIterationD3Q19
+{
+    using Tools = LbmContainers<PopulationField, ComputeType>;
+    using CellTypeField = typename Tools::CellTypeField;
+
+    template <typename Omega>
+    LbmIterationD3Q19(StencilSemantic stencilSemantic, Occ occ, TransferMode transfer, PopulationField& fIn,
+                      PopulationField& fOut, CellTypeField& cellTypeField, Omega omega,
+                      const double force[3] = nullptr)
+    {
+        pop[0] = fIn;
+        pop[1] = fOut;
+        for (int t = 0; t < 2; t++) {
+            lbmTwoPop[t] = Skeleton(fIn.getBackend());
+            lbmTwoPop[t].sequence({Tools::iteration(stencilSemantic, pop[t], cellTypeField, omega, pop[1 - t], force)},
+                                  "LBM_iteration_" + std::to_string(t), SkeletonOptions(occ, transfer));
+        }
+        parity = 0;
+    }
+    PopulationField& getInput() { return pop[parity]; }
+    PopulationField& getOutput() { return pop[parity ^ 1]; }
+    void             run()
+    {
+        lbmTwoPop[parity].run();
+        parity ^= 1;
+    }
+    void sync() { pop[0].getBackend().syncAll(); }
+
+   private:
+    Skeleton        lbmTwoPop[2];
+    PopulationField pop[2];
+    int             parity = 0;
+};
+
+}  // namespace lbm
+}  // namespace neon_amd

@@ -20,6 +20,7 @@ NX_MASK_BITS = 0x7FDFF
 NX_FLAG_OPEN = 1 << 28
 NX_FLAG_MIXED = 1 << 29
 NX_PERIODIC_X, NX_PERIODIC_Y, NX_PERIODIC_Z = 1, 2, 4
+NX_CONV_RELU, NX_CONV_OUT_F32 = 1, 2
 
 STORAGE_BYTES = {NX_F64: 8, NX_F32: 4, NX_F32DEV: 4, NX_F16DEV: 2}
 STORAGE_NAMES = {"f64": NX_F64, "f32": NX_F32, "f32dev": NX_F32DEV, "f16dev": NX_F16DEV}
@@ -55,6 +56,12 @@ class Tuning(C.Structure):
 
 class LbmParams(C.Structure):
     _fields_ = [("omega", C.c_double), ("force", C.c_double * 3)]
+
+
+class UNetDesc(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("width", C.c_int32 * 4), ("size", C.c_int32),
+                ("max_batch", C.c_int32), ("in_ch", C.c_int32), ("out_ch", C.c_int32),
+                ("reserved", C.c_int32), ("seed", C.c_uint64)]
 
 
 _P = C.c_void_p
@@ -129,6 +136,17 @@ SIGNATURES = {
     "nx_batch_set_launch": (_I, [_P, _I32, _I32]),
     "nx_batch_autotune": (_I, [_P, _P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(C.c_float)]),
     "nx_batch_destroy": (_I, [_P]),
+    "nx_conv3d_fwd_bf16": (_I, [_P, _P, _P, _P, C.POINTER(_I32), _I32, _I32, _I32, _I32, _P]),
+    "nx_conv3d_bwd_data_bf16": (_I, [_P, _P, _P, C.POINTER(_I32), _I32, _I32, _I32, _I32, _P]),
+    "nx_conv3d_bwd_weight_bf16": (_I, [_P, _P, _P, _P, C.POINTER(_I32), _I32, _I32, _I32, _P]),
+    "nx_unet_create": (_I, [C.POINTER(UNetDesc), C.POINTER(_P)]),
+    "nx_unet_info": (_I, [_P, C.POINTER(C.c_uint64), C.POINTER(_U32)]),
+    "nx_unet_params": (_I, [_P, _P, _I32]),
+    "nx_unet_grads": (_I, [_P, _P]),
+    "nx_unet_forward": (_I, [_P, _P, _I32, _P, _P]),
+    "nx_unet_backward": (_I, [_P, _P, _I32, _P]),
+    "nx_unet_train_step": (_I, [_P, _P, _P, _P, _I32, C.c_float, C.POINTER(_D), _P]),
+    "nx_unet_destroy": (_I, [_P]),
     "nx_d3q19_moments_block": (_I, [_BP, _P, _P, _P, _P, _P]),
     "nx_d3q19_mass_block": (_I, [_BP, _P, _P, _P, _P]),
     "nx_d3q19_residual_block": (_I, [_BP, _P, _P, _P, _P, _P]),

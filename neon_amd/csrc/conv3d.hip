@@ -1,0 +1,518 @@
+// BF16 3D convolution forward, backward-data and backward-weight on the gfx950
+// matrix cores, FP32 accumulation (the learned warm start's network).  No
+// counterpart in the reference.
+//
+// All three are implicit GEMMs fed straight from global memory (the 27-fold
+// reuse of every input voxel is served by L1/L2):
+//   forward      M = voxels, N = cout, K = taps*cin      (A: 8 channels of one
+//                tap per lane = one 16-byte load; B: 8 weights, 16 bytes)
+//   bwd-data     the forward on the flipped, transposed weights
+//   bwd-weight   per tap: M = cout, N = cin, K = voxels (split over
+//                workgroups, per-wave FP32 partials, fixed-order reduction)
+// MFMA 16x16x32 bf16 operand layout: lane l holds row (A) / column (B) l%16
+// and the 8 K values of group l/16; D register r of lane l is row 4*(l/16)+r,
+// column l%16.
+#include "conv3d.hpp"
+#include "nx_common.hpp"
+
+#include <algorithm>
+
+namespace nx::conv {
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+struct Geo {
+    uint32_t V;          // voxels = n*d*h*w
+    int D, H, W;         // powers of two
+    int lw, lh;          // log2 W, log2 H
+};
+
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);  // quiet NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                                            // nearest even
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ bf16x8 ld8(const uint16_t* p)
+{
+    return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
+
+__device__ __forceinline__ f32x4 mfma(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+template <int KS>
+__device__ __forceinline__ void tap_offset(int tap, int& dz, int& dy, int& dx)
+{
+    if constexpr (KS == 3) {
+        dz = tap / 9 - 1;
+        dy = (tap / 3) % 3 - 1;
+        dx = tap % 3 - 1;
+    } else {
+        dz = dy = dx = 0;
+    }
+}
+
+// One wave: 32 voxels x (16*NT) output channels; 4 waves per workgroup.
+// grid.y selects the block of 16*NT output channels.
+template <int NT, int KS>
+__global__ void __launch_bounds__(256)
+    k_fwd(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
+          const float* __restrict__ bias, void* __restrict__ y, Geo g, int cin, int cout, int flags)
+{
+    constexpr int T = KS * KS * KS;
+    const int K = T * cin, KC = (K + 31) / 32;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, grp = lane >> 4;
+    const uint32_t vbase = (blockIdx.x * 4u + (uint32_t)wave) * 32u;
+    if (vbase >= g.V) return;  // whole wave
+    const int co0 = (int)blockIdx.y * NT * 16;
+
+    int px[2], py[2], pz[2];
+    uint32_t pv[2];
+    bool pok[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const uint32_t v = vbase + 16u * t + (uint32_t)r16;
+        pok[t] = v < g.V;
+        pv[t] = v;
+        px[t] = (int)(v & (uint32_t)(g.W - 1));
+        py[t] = (int)((v >> g.lw) & (uint32_t)(g.H - 1));
+        pz[t] = (int)((v >> (g.lw + g.lh)) & (uint32_t)(g.D - 1));
+    }
+    const uint16_t* wrow[NT];
+#pragma unroll
+    for (int n = 0; n < NT; n++) wrow[n] = w + (size_t)(co0 + n * 16 + r16) * K;
+
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int t = 0; t < 2; t++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) acc[t][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // K position of this lane group: kk = 32*kc + 8*grp = tap*cin + ci
+    int tap = 0, ci = 8 * grp;
+    while (ci >= cin) { ci -= cin; ++tap; }
+    for (int kc = 0; kc < KC; ++kc) {
+        const int kk = kc * 32 + 8 * grp;
+        bf16x8 a[2], b[NT];
+        if (kk < K) {
+            int dz, dy, dx;
+            tap_offset<KS>(tap, dz, dy, dx);
+            const int off = (dz * g.H + dy) * g.W + dx;
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const bool ok = pok[t] && (unsigned)(px[t] + dx) < (unsigned)g.W &&
+                                (unsigned)(py[t] + dy) < (unsigned)g.H &&
+                                (unsigned)(pz[t] + dz) < (unsigned)g.D;
+                a[t] = ok ? ld8(x + (size_t)(int64_t)((int64_t)pv[t] + off) * cin + ci) : bf16x8{};
+            }
+#pragma unroll
+            for (int n = 0; n < NT; n++) b[n] = ld8(wrow[n] + kk);
+        } else {
+#pragma unroll
+            for (int t = 0; t < 2; t++) a[t] = bf16x8{};
+#pragma unroll
+            for (int n = 0; n < NT; n++) b[n] = bf16x8{};
+        }
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int n = 0; n < NT; n++) acc[t][n] = mfma(a[t], b[n], acc[t][n]);
+        ci += 32;
+        while (ci >= cin) { ci -= cin; ++tap; }
+    }
+
+    const bool relu = flags & kRelu, f32out = flags & kOutF32;
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+        const int co = co0 + n * 16 + r16;
+        const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint32_t v = vbase + 16u * t + 4u * grp + (uint32_t)r;
+                if (v >= g.V) continue;
+                float s = acc[t][n][r] + bv;
+                if (relu) s = s > 0.f ? s : 0.f;
+                if (f32out)
+                    static_cast<float*>(y)[(size_t)v * cout + co] = s;
+                else
+                    static_cast<uint16_t*>(y)[(size_t)v * cout + co] = f2bf(s);
+            }
+    }
+}
+
+// Weight gradient of one tap for a (16*NM) x (16*NN) block of (co, ci).
+// grid.x = T * splits (tap fastest), grid.y = channel blocks.  Each wave
+// takes every 4th 32-voxel chunk of its split and writes its own partial:
+// part[(split*4 + wave)][tap][co][ci].
+template <int NM, int NN, int KS>
+__global__ void __launch_bounds__(256)
+    k_bwd_w(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
+            float* __restrict__ part, Geo g, int cin, int cout, uint32_t chunks_per_split)
+{
+    constexpr int T = KS * KS * KS;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, grp = lane >> 4;
+    const int tap = (int)(blockIdx.x % T);
+    const uint32_t split = blockIdx.x / T;
+    const int mblocks = cout / (16 * NM);
+    const int co0 = (int)(blockIdx.y % mblocks) * NM * 16;
+    const int ci0 = (int)(blockIdx.y / mblocks) * NN * 16;
+    int dz, dyo, dx;
+    tap_offset<KS>(tap, dz, dyo, dx);
+    const int off = (dz * g.H + dyo) * g.W + dx;
+
+    f32x4 acc[NM][NN];
+#pragma unroll
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+        for (int n = 0; n < NN; n++) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const uint32_t n_chunks = (g.V + 31u) / 32u;
+    const uint32_t c0 = split * chunks_per_split;
+    const uint32_t c1 = min(n_chunks, c0 + chunks_per_split);
+    for (uint32_t c = c0 + (uint32_t)wave; c < c1; c += 4u) {
+        const uint32_t v0 = c * 32u + 8u * grp;
+        uint32_t vv[8], nbv[8];
+        bool vok[8], nok[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const uint32_t v = v0 + (uint32_t)e;
+            const int px = (int)(v & (uint32_t)(g.W - 1));
+            const int py = (int)((v >> g.lw) & (uint32_t)(g.H - 1));
+            const int pz = (int)((v >> (g.lw + g.lh)) & (uint32_t)(g.D - 1));
+            vok[e] = v < g.V;
+            nok[e] = vok[e] && (unsigned)(px + dx) < (unsigned)g.W &&
+                     (unsigned)(py + dyo) < (unsigned)g.H && (unsigned)(pz + dz) < (unsigned)g.D;
+            vv[e] = v;
+            nbv[e] = (uint32_t)((int64_t)v + off);
+        }
+        bf16x8 a[NM], b[NN];
+#pragma unroll
+        for (int m = 0; m < NM; m++) {
+            u16x8 t;
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                t[e] = vok[e] ? dy[(size_t)vv[e] * cout + co0 + m * 16 + r16] : (unsigned short)0;
+            a[m] = __builtin_bit_cast(bf16x8, t);
+        }
+#pragma unroll
+        for (int n = 0; n < NN; n++) {
+            u16x8 t;
+#pragma unroll
+            for (int e = 0; e < 8; e++)
+                t[e] = nok[e] ? x[(size_t)nbv[e] * cin + ci0 + n * 16 + r16] : (unsigned short)0;
+            b[n] = __builtin_bit_cast(bf16x8, t);
+        }
+#pragma unroll
+        for (int m = 0; m < NM; m++)
+#pragma unroll
+            for (int n = 0; n < NN; n++) acc[m][n] = mfma(a[m], b[n], acc[m][n]);
+    }
+
+    float* out = part + ((size_t)(split * 4u + (uint32_t)wave) * T + tap) * (size_t)cout * cin;
+#pragma unroll
+    for (int m = 0; m < NM; m++)
+#pragma unroll
+        for (int n = 0; n < NN; n++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                out[(size_t)(co0 + m * 16 + 4 * grp + r) * cin + ci0 + n * 16 + r16] = acc[m][n][r];
+}
+
+// dw[co][tap][ci] = sum_p part[p][tap][co][ci], p in order.
+__global__ void __launch_bounds__(256)
+    k_reduce_w(const float* __restrict__ part, int P, int T, int cin, int cout, float* __restrict__ dw)
+{
+    const size_t n = (size_t)cout * T * cin;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const int ci = (int)(i % cin);
+        const int tap = (int)((i / cin) % T);
+        const int co = (int)(i / ((size_t)cin * T));
+        const size_t stride = (size_t)T * cout * cin;
+        const float* p = part + ((size_t)tap * cout + co) * cin + ci;
+        float s = 0.f;
+        for (int k = 0; k < P; k++) s += p[(size_t)k * stride];
+        dw[i] = s;
+    }
+}
+
+// Per-channel sums of dy over voxel ranges: part[blockIdx.x][c].
+__global__ void __launch_bounds__(256)
+    k_colsum(const uint16_t* __restrict__ dy, uint32_t V, int C, uint32_t per_block,
+             float* __restrict__ part)
+{
+    __shared__ float lds[256];
+    const int rows = 256 / C;  // C <= 256; threads past rows*C idle
+    const int c = threadIdx.x % C, j = threadIdx.x / C;
+    const uint32_t v0 = blockIdx.x * per_block, v1 = min(V, v0 + per_block);
+    float s = 0.f;
+    if (j < rows)
+        for (uint32_t v = v0 + (uint32_t)j; v < v1; v += (uint32_t)rows)
+            s += __builtin_bit_cast(float, (uint32_t)dy[(size_t)v * C + c] << 16);
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    if (threadIdx.x < C) {
+        float t = 0.f;
+        for (int k = 0; k < rows; k++) t += lds[k * C + threadIdx.x];
+        part[(size_t)blockIdx.x * C + threadIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    k_reduce_cols(const float* __restrict__ part, int P, int C, float* __restrict__ out)
+{
+    const int c = (int)(blockIdx.x * 256 + threadIdx.x);
+    if (c >= C) return;
+    float s = 0.f;
+    for (int k = 0; k < P; k++) s += part[(size_t)k * C + c];
+    out[c] = s;
+}
+
+__global__ void __launch_bounds__(256)
+    k_flip(const uint16_t* __restrict__ w, uint16_t* __restrict__ wt, int T, int cin, int cout)
+{
+    const size_t n = (size_t)cout * T * cin;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        // i indexes wt[ci][tap][co]
+        const int co = (int)(i % cout);
+        const int tap = (int)((i / cout) % T);
+        const int ci = (int)(i / ((size_t)cout * T));
+        wt[i] = w[((size_t)co * T + (T - 1 - tap)) * cin + ci];
+    }
+}
+
+Geo geo_of(const Shape& s)
+{
+    Geo g;
+    g.V = (uint32_t)((uint64_t)s.n * s.d * s.h * s.w);
+    g.D = s.d;
+    g.H = s.h;
+    g.W = s.w;
+    g.lw = __builtin_ctz((unsigned)s.w);
+    g.lh = __builtin_ctz((unsigned)s.h);
+    return g;
+}
+
+bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+template <typename F>
+int by_ks(int ks, F&& f)
+{
+    if (ks == 3) { f(std::integral_constant<int, 3>{}); return NX_OK; }
+    if (ks == 1) { f(std::integral_constant<int, 1>{}); return NX_OK; }
+    return set_error(NX_ERR_INVALID, "conv3d: kernel size must be 1 or 3");
+}
+
+// largest of 8, 4, 2, 1 dividing c16 = channels/16, at most `cap`
+int tiles_of(int c16, int cap)
+{
+    for (int t = cap; t > 1; t >>= 1)
+        if (c16 % t == 0) return t;
+    return 1;
+}
+
+int launch_fwd(const Shape& s, const uint16_t* x, const uint16_t* w, const float* bias, void* y,
+               int flags, hipStream_t st)
+{
+    const Geo g = geo_of(s);
+    if (g.V == 0) return NX_OK;
+    const int nt = tiles_of(s.cout / 16, 8);
+    const dim3 grid((g.V + 127u) / 128u, (unsigned)(s.cout / (16 * nt)));
+    return by_ks(s.ks, [&](auto KS) {
+        switch (nt) {
+            case 8: hipLaunchKernelGGL((k_fwd<8, KS>), grid, dim3(256), 0, st, x, w, bias, y, g, s.cin, s.cout, flags); break;
+            case 4: hipLaunchKernelGGL((k_fwd<4, KS>), grid, dim3(256), 0, st, x, w, bias, y, g, s.cin, s.cout, flags); break;
+            case 2: hipLaunchKernelGGL((k_fwd<2, KS>), grid, dim3(256), 0, st, x, w, bias, y, g, s.cin, s.cout, flags); break;
+            default: hipLaunchKernelGGL((k_fwd<1, KS>), grid, dim3(256), 0, st, x, w, bias, y, g, s.cin, s.cout, flags); break;
+        }
+    });
+}
+
+struct WPlan {
+    int nm, nn, T;
+    uint32_t splits, chunks_per_split;
+    size_t part_floats;
+};
+
+WPlan plan_w(const Shape& s)
+{
+    WPlan p;
+    p.T = s.ks * s.ks * s.ks;
+    p.nm = tiles_of(s.cout / 16, 4);
+    p.nn = tiles_of(s.cin / 16, 4);
+    const uint64_t V = (uint64_t)s.n * s.d * s.h * s.w;
+    const uint32_t n_chunks = (uint32_t)((V + 31) / 32);
+    const uint32_t blocks = (uint32_t)(p.T * (s.cout / (16 * p.nm)) * (s.cin / (16 * p.nn)));
+    uint32_t splits = (2048u + blocks - 1) / blocks;
+    splits = std::max(1u, std::min({splits, 64u, (n_chunks + 3u) / 4u}));
+    p.chunks_per_split = (n_chunks + splits - 1) / splits;
+    p.splits = (n_chunks + p.chunks_per_split - 1) / p.chunks_per_split;
+    if (p.splits == 0) p.splits = 1;
+    p.part_floats = (size_t)p.splits * 4 * p.T * s.cout * s.cin;
+    return p;
+}
+
+constexpr uint32_t kColBlocks = 64;
+
+}  // namespace
+
+int check(const Shape& s)
+{
+    if (s.n < 1 || !pow2(s.d) || !pow2(s.h) || !pow2(s.w))
+        return set_error(NX_ERR_INVALID, "conv3d: batch >= 1 and power-of-two depth/height/width required");
+    if (s.cin < 16 || s.cin % 16 || s.cout < 16 || s.cout % 16 || s.cin > 256 || s.cout > 256)
+        return set_error(NX_ERR_INVALID, "conv3d: channel counts must be multiples of 16 in [16, 256]");
+    if (s.ks != 1 && s.ks != 3) return set_error(NX_ERR_INVALID, "conv3d: kernel size must be 1 or 3");
+    if ((uint64_t)s.n * s.d * s.h * s.w * std::max(s.cin, s.cout) >= (1ull << 31))
+        return set_error(NX_ERR_INVALID, "conv3d: tensor too large (2^31 elements)");
+    return NX_OK;
+}
+
+int fwd(const Shape& s, const uint16_t* x, const uint16_t* w, const float* bias, void* y, int flags,
+        hipStream_t st)
+{
+    if (int e = launch_fwd(s, x, w, bias, y, flags, st)) return e;
+    NX_CHECK_LAUNCH("k_fwd");
+    return NX_OK;
+}
+
+int flip(const Shape& s, const uint16_t* w, uint16_t* wt, hipStream_t st)
+{
+    const int T = s.ks * s.ks * s.ks;
+    const size_t n = (size_t)s.cout * T * s.cin;
+    hipLaunchKernelGGL(k_flip, dim3((unsigned)std::min<size_t>((n + 255) / 256, 4096)), dim3(256), 0,
+                       st, w, wt, T, s.cin, s.cout);
+    NX_CHECK_LAUNCH("k_flip");
+    return NX_OK;
+}
+
+int bwd_data(const Shape& s, const uint16_t* dy, const uint16_t* wt, void* dx, int flags,
+             hipStream_t st)
+{
+    Shape t = s;
+    t.cin = s.cout;
+    t.cout = s.cin;
+    if (int e = launch_fwd(t, dy, wt, nullptr, dx, flags & kOutF32, st)) return e;
+    NX_CHECK_LAUNCH("k_fwd (bwd-data)");
+    return NX_OK;
+}
+
+size_t bwd_weight_ws(const Shape& s)
+{
+    const WPlan p = plan_w(s);
+    const size_t cols = (size_t)kColBlocks * s.cout;
+    return (std::max(p.part_floats, cols)) * sizeof(float);
+}
+
+int bwd_weight(const Shape& s, const uint16_t* x, const uint16_t* dy, float* dw, float* db, void* ws,
+               hipStream_t st)
+{
+    const Geo g = geo_of(s);
+    const WPlan p = plan_w(s);
+    float* part = static_cast<float*>(ws);
+    const dim3 grid(p.T * p.splits, (unsigned)((s.cout / (16 * p.nm)) * (s.cin / (16 * p.nn))));
+    int err = by_ks(s.ks, [&](auto KS) {
+        auto go = [&](auto NM, auto NN) {
+            hipLaunchKernelGGL((k_bwd_w<decltype(NM)::value, decltype(NN)::value, KS>), grid,
+                               dim3(256), 0, st, x, dy, part, g, s.cin, s.cout, p.chunks_per_split);
+        };
+        auto by_nn = [&](auto NM) {
+            switch (p.nn) {
+                case 4: go(NM, std::integral_constant<int, 4>{}); break;
+                case 2: go(NM, std::integral_constant<int, 2>{}); break;
+                default: go(NM, std::integral_constant<int, 1>{}); break;
+            }
+        };
+        switch (p.nm) {
+            case 4: by_nn(std::integral_constant<int, 4>{}); break;
+            case 2: by_nn(std::integral_constant<int, 2>{}); break;
+            default: by_nn(std::integral_constant<int, 1>{}); break;
+        }
+    });
+    if (err) return err;
+    NX_CHECK_LAUNCH("k_bwd_w");
+    const size_t nw = (size_t)s.cout * p.T * s.cin;
+    hipLaunchKernelGGL(k_reduce_w, dim3((unsigned)std::min<size_t>((nw + 255) / 256, 4096)), dim3(256),
+                       0, st, part, (int)(p.splits * 4), p.T, s.cin, s.cout, dw);
+    NX_CHECK_LAUNCH("k_reduce_w");
+    if (db) {
+        // the column partials reuse the workspace after the weight reduction
+        const uint32_t per = (g.V + kColBlocks - 1) / kColBlocks;
+        hipLaunchKernelGGL(k_colsum, dim3(kColBlocks), dim3(256), 0, st, dy, g.V, s.cout, per, part);
+        NX_CHECK_LAUNCH("k_colsum");
+        hipLaunchKernelGGL(k_reduce_cols, dim3((s.cout + 255) / 256), dim3(256), 0, st, part,
+                           (int)kColBlocks, s.cout, db);
+        NX_CHECK_LAUNCH("k_reduce_cols");
+    }
+    return NX_OK;
+}
+
+}  // namespace nx::conv
+
+using namespace nx;
+
+namespace {
+
+int shape_of(const int32_t shape[4], int cin, int cout, int ks, conv::Shape& s)
+{
+    if (!shape) return set_error(NX_ERR_INVALID, "conv3d: null shape");
+    s = conv::Shape{shape[0], shape[1], shape[2], shape[3], cin, cout, ks};
+    return conv::check(s);
+}
+
+bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int nx_conv3d_fwd_bf16(const uint16_t* x, const uint16_t* w, const float* bias, void* y,
+                       const int32_t shape[4], int32_t cin, int32_t cout, int32_t ksize, int32_t flags,
+                       nx_stream s)
+{
+    conv::Shape sh;
+    if (int e = shape_of(shape, cin, cout, ksize, sh)) return e;
+    if (!x || !w || !y) return set_error(NX_ERR_INVALID, "nx_conv3d_fwd_bf16: null pointer");
+    if (!aligned16(x) || !aligned16(w))
+        return set_error(NX_ERR_INVALID, "nx_conv3d_fwd_bf16: x and w must be 16-byte aligned");
+    return conv::fwd(sh, x, w, bias, y, flags & (conv::kRelu | conv::kOutF32), (hipStream_t)s);
+}
+
+int nx_conv3d_bwd_data_bf16(const uint16_t* dy, const uint16_t* w, void* dx, const int32_t shape[4],
+                            int32_t cin, int32_t cout, int32_t ksize, int32_t flags, nx_stream s)
+{
+    conv::Shape sh;
+    if (int e = shape_of(shape, cin, cout, ksize, sh)) return e;
+    if (!dy || !w || !dx) return set_error(NX_ERR_INVALID, "nx_conv3d_bwd_data_bf16: null pointer");
+    if (!aligned16(dy)) return set_error(NX_ERR_INVALID, "nx_conv3d_bwd_data_bf16: dy must be 16-byte aligned");
+    void* ws = nullptr;
+    const size_t nw = (size_t)cout * ksize * ksize * ksize * cin;
+    if (int e = scratch(nw * sizeof(uint16_t), &ws)) return e;
+    if (int e = conv::flip(sh, w, static_cast<uint16_t*>(ws), (hipStream_t)s)) return e;
+    return conv::bwd_data(sh, dy, static_cast<uint16_t*>(ws), dx, flags & conv::kOutF32, (hipStream_t)s);
+}
+
+int nx_conv3d_bwd_weight_bf16(const uint16_t* x, const uint16_t* dy, float* dw, float* db,
+                              const int32_t shape[4], int32_t cin, int32_t cout, int32_t ksize,
+                              nx_stream s)
+{
+    conv::Shape sh;
+    if (int e = shape_of(shape, cin, cout, ksize, sh)) return e;
+    if (!x || !dy || !dw) return set_error(NX_ERR_INVALID, "nx_conv3d_bwd_weight_bf16: null pointer");
+    void* ws = nullptr;
+    if (int e = scratch(conv::bwd_weight_ws(sh), &ws)) return e;
+    return conv::bwd_weight(sh, x, dy, dw, db, ws, (hipStream_t)s);
+}
+
+}  // extern "C"

@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(kTile)
             for (int q = 0; q < Q; q++) {
                 const double cu = CX[q] * u[0] + CY[q] * u[1] + CZ[q] * u[2];
                 const double feq = rho * wq(q) * (1. + 3. * cu + 4.5 * cu * cu - usqr);
-                const auto v = Stor<S>::template enc<double>(feq, q);
+                const auto v = Stor<S>::enc_full(feq, q);
                 p0[M.pop_base + (uint64_t)q * M.n + i] = v;
                 p1[M.pop_base + (uint64_t)q * M.n + i] = v;
             }

@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(kBV)
     for (int q = 0; q < Q; q++) {
         typename Stor<S>::T val;
         if (cls == NX_CELL_BULK) {
-            val = Stor<S>::template enc<double>(wq(q), q);
+            val = Stor<S>::enc_full(wq(q), q);
         } else if (cls == NX_CELL_MOVINGWALL) {
             const double w = -6. * wq(q) * ulb * (CX[q] * l0 + CY[q] * l1 + CZ[q] * l2);
             val = (typename Stor<S>::T)(typename Stor<S>::P)w;
@@ -402,7 +402,7 @@ __global__ void __launch_bounds__(kBV)
         const uint64_t own = (uint64_t)b * kBStride + v;
         double r = 0;
 #pragma unroll
-        for (int q = 0; q < Q; q++) r += (double)Stor<S>::bulk(fin[own + q * kBV], q);
+        for (int q = 0; q < Q; q++) r += Stor<S>::full(fin[own + q * kBV], q);
         acc[0] += r;
     }
     block_sum_f64<1>(acc, lds);

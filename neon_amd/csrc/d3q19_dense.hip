@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(kWG)
     for (int q = 0; q < Q; q++) {
         typename Stor<S>::T v;
         if (cls == NX_CELL_BULK) {
-            v = Stor<S>::template enc<double>(wq(q), q);
+            v = Stor<S>::enc_full(wq(q), q);
         } else if (cls == NX_CELL_MOVINGWALL) {
             const double val = -6. * wq(q) * ulb * (CX[q] * l0 + CY[q] * l1 + CZ[q] * l2);
             // binary16 goes through binary32 (the oracle's rounding path)
@@ -155,7 +155,7 @@ __global__ void __launch_bounds__(kWG)
         if (cls_of(flags[i]) != NX_CELL_BULK) continue;
         double r = 0;
 #pragma unroll
-        for (int q = 0; q < Q; q++) r += (double)Stor<S>::bulk(fin[q * g.plane + i], q);
+        for (int q = 0; q < Q; q++) r += Stor<S>::full(fin[q * g.plane + i], q);
         acc[0] += r;
     }
     block_sum_f64<1>(acc, lds);

@@ -36,6 +36,17 @@ __device__ __forceinline__ int cls_of(uint32_t fl) { return (int)((fl >> NX_CLAS
 // ---- storage codecs -------------------------------------------------------
 // P is the precision in which the pulled value is formed (the reference adds
 // own + neighbour on a wall link in LbmStoreType, LbmTools.h:83-84).
+// Bulk populations are handled in a working representation: f itself for
+// F64 / F32 storage, the deviation d = f - w_q for the deviation storages
+// (whose collision is written in d, so that FP32 arithmetic rounds relative
+// to the deviations rather than to w_q; project-defined).  Wall cells always
+// hold f.  A wall link adds own + wall in either representation (w_q = w_opp).
+//   bulk(v, q)    stored bulk value -> working value
+//   raw(v)        stored wall value -> P
+//   full(v, q)    stored bulk value -> f in binary64 (mass sums)
+//   enc(v, q)     working value -> stored
+//   enc_full(f,q) f -> stored bulk value (fill, equilibrium start); the
+//                 deviation is formed in binary64 and rounded through binary32
 template <int S>
 struct Stor;
 
@@ -43,53 +54,71 @@ template <>
 struct Stor<NX_F64> {
     using T = double;
     using P = double;
+    static constexpr bool kDev = false;
     static __device__ __forceinline__ P bulk(T v, int) { return v; }
     static __device__ __forceinline__ P raw(T v) { return v; }
+    static __device__ __forceinline__ double full(T v, int) { return v; }
     template <typename R>
     static __device__ __forceinline__ T enc(R v, int)
     {
         return (T)v;
     }
+    static __device__ __forceinline__ T enc_full(double f, int) { return f; }
 };
 
 template <>
 struct Stor<NX_F32> {
     using T = float;
     using P = float;
+    static constexpr bool kDev = false;
     static __device__ __forceinline__ P bulk(T v, int) { return v; }
     static __device__ __forceinline__ P raw(T v) { return v; }
+    static __device__ __forceinline__ double full(T v, int) { return (double)v; }
     template <typename R>
     static __device__ __forceinline__ T enc(R v, int)
     {
         return (T)v;
     }
+    static __device__ __forceinline__ T enc_full(double f, int) { return (float)f; }
 };
 
-// f - w_q in binary32 for bulk cells, f for wall cells
+// d = f - w_q in binary32 for bulk cells, f for wall cells
 template <>
 struct Stor<NX_F32DEV> {
     using T = float;
     using P = float;
-    static __device__ __forceinline__ P bulk(T v, int q) { return (float)wq(q) + v; }
+    static constexpr bool kDev = true;
+    static __device__ __forceinline__ P bulk(T v, int) { return v; }
     static __device__ __forceinline__ P raw(T v) { return v; }
+    static __device__ __forceinline__ double full(T v, int q) { return (double)(float)wq(q) + (double)v; }
     template <typename R>
-    static __device__ __forceinline__ T enc(R v, int q)
+    static __device__ __forceinline__ T enc(R v, int)
     {
-        return (float)v - (float)wq(q);
+        return (T)v;
     }
+    static __device__ __forceinline__ T enc_full(double f, int q) { return (float)(f - wq(q)); }
 };
 
-// f - w_q in binary16 (v_cvt_f16_f32, round to nearest even) for bulk cells
+// d = f - w_q in binary16 (v_cvt_f16_f32, round to nearest even) for bulk cells
 template <>
 struct Stor<NX_F16DEV> {
     using T = _Float16;
     using P = float;
-    static __device__ __forceinline__ P bulk(T v, int q) { return (float)wq(q) + (float)v; }
+    static constexpr bool kDev = true;
+    static __device__ __forceinline__ P bulk(T v, int) { return (float)v; }
     static __device__ __forceinline__ P raw(T v) { return (float)v; }
-    template <typename R>
-    static __device__ __forceinline__ T enc(R v, int q)
+    static __device__ __forceinline__ double full(T v, int q)
     {
-        return (_Float16)((float)v - (float)wq(q));
+        return (double)(float)wq(q) + (double)(float)v;
+    }
+    template <typename R>
+    static __device__ __forceinline__ T enc(R v, int)
+    {
+        return (_Float16)(float)v;
+    }
+    static __device__ __forceinline__ T enc_full(double f, int q)
+    {
+        return (_Float16)(float)(f - wq(q));
     }
 };
 
@@ -191,6 +220,68 @@ __device__ __forceinline__ void bgk_forced(const R f[Q], R omega, R g0, R g1, R 
     }
 }
 
+// Deviation form (project-defined): d = f - w_q.  rho = 1 + sum d,
+// u = sum c d / rho, and feq - w_q = w_q (rho - 1) + w_q rho (3 c.u
+// + 4.5 (c.u)^2 - 1.5 u^2): the same BGK as bgk<R>, every term small.
+template <typename R>
+__device__ __forceinline__ void macroscopic_dev(const R d[Q], R& drho, R& rho, R& u0, R& u1, R& u2)
+{
+    const R XM1 = d[0] + d[3] + d[4] + d[5] + d[6];
+    const R XP1 = d[10] + d[13] + d[14] + d[15] + d[16];
+    const R X0 = d[9] + d[1] + d[2] + d[7] + d[8] + d[11] + d[12] + d[17] + d[18];
+    const R YM1 = d[1] + d[3] + d[7] + d[8] + d[14];
+    const R YP1 = d[4] + d[11] + d[13] + d[17] + d[18];
+    const R ZM1 = d[2] + d[5] + d[7] + d[16] + d[18];
+    const R ZP1 = d[6] + d[8] + d[12] + d[15] + d[17];
+    drho = XM1 + XP1 + X0;
+    rho = R(1.) + drho;
+    u0 = (XP1 - XM1) / rho;
+    u1 = (YP1 - YM1) / rho;
+    u2 = (ZP1 - ZM1) / rho;
+}
+
+template <typename R>
+__device__ __forceinline__ void bgk_dev(const R d[Q], R omega, R out[Q])
+{
+    R drho, rho, u0, u1, u2;
+    macroscopic_dev<R>(d, drho, rho, u0, u1, u2);
+    const R usqr = R(1.5) * (u0 * u0 + u1 * u1 + u2 * u2);
+    const R ck[9] = {u0, u1, u2, u0 + u1, u0 - u1, u0 + u2, u0 - u2, u1 + u2, u1 - u2};
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const R wi = i < 3 ? R(1. / 18.) : R(1. / 36.);
+        const R rw = rho * wi;
+        const R deq = wi * drho + rw * (R(4.5) * ck[i] * ck[i] - R(3.) * ck[i] - usqr);
+        const R deqopp = deq + rw * R(6.) * ck[i];
+        out[i] = (R(1.) - omega) * d[i] + omega * deq;
+        out[i + 10] = (R(1.) - omega) * d[i + 10] + omega * deqopp;
+    }
+    const R deq9 = R(1. / 3.) * drho - rho * R(1. / 3.) * usqr;
+    out[9] = (R(1.) - omega) * d[9] + omega * deq9;
+}
+
+template <typename R>
+__device__ __forceinline__ void bgk_forced_dev(const R d[Q], R omega, R g0, R g1, R g2, R out[Q])
+{
+    R drho, rho, u0, u1, u2;
+    macroscopic_dev<R>(d, drho, rho, u0, u1, u2);
+    u0 += R(0.5) * g0;
+    u1 += R(0.5) * g1;
+    u2 += R(0.5) * g2;
+    const R usqr = R(1.5) * (u0 * u0 + u1 * u1 + u2 * u2);
+    const R ug = u0 * g0 + u1 * g1 + u2 * g2;
+    const R fpre = R(1) - R(0.5) * omega;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const R w = R(wq(q));
+        const R cu = R(CX[q]) * u0 + R(CY[q]) * u1 + R(CZ[q]) * u2;
+        const R cg = R(CX[q]) * g0 + R(CY[q]) * g1 + R(CZ[q]) * g2;
+        const R deq = w * drho + rho * w * (R(3) * cu + R(4.5) * cu * cu - usqr);
+        const R src = fpre * w * rho * (R(3) * (cg - ug) + R(9) * cu * cg);
+        out[q] = (R(1) - omega) * d[q] + omega * deq + src;
+    }
+}
+
 template <int S, int CM, bool FORCE>
 __device__ __forceinline__ void collide(const typename Stor<S>::P fp[Q], double omega,
                                         const double g[3], typename Real<S, CM>::R out[Q])
@@ -199,7 +290,12 @@ __device__ __forceinline__ void collide(const typename Stor<S>::P fp[Q], double 
     R f[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) f[q] = (R)fp[q];
-    if constexpr (FORCE) {
+    if constexpr (Stor<S>::kDev) {
+        if constexpr (FORCE)
+            bgk_forced_dev<R>(f, (R)omega, (R)g[0], (R)g[1], (R)g[2], out);
+        else
+            bgk_dev<R>(f, (R)omega, out);
+    } else if constexpr (FORCE) {
         bgk_forced<R>(f, (R)omega, (R)g[0], (R)g[1], (R)g[2], out);
     } else if constexpr (CM == NX_COMPUTE_REF && S != NX_F64) {
         bgk_ref(f, (float)omega, out);
@@ -220,7 +316,12 @@ __device__ __forceinline__ void cell_moments(int cls, const typename Stor<S>::P 
         R f[Q];
 #pragma unroll
         for (int q = 0; q < Q; q++) f[q] = (R)fp[q];
-        macroscopic<R>(f, rho, u[0], u[1], u[2]);
+        if constexpr (Stor<S>::kDev) {
+            R drho;
+            macroscopic_dev<R>(f, drho, rho, u[0], u[1], u[2]);
+        } else {
+            macroscopic<R>(f, rho, u[0], u[1], u[2]);
+        }
     } else if (cls == NX_CELL_MOVINGWALL) {
         rho = R(1.0);
         // fp[0..2] hold the cell's own populations 0..2

@@ -131,11 +131,22 @@ static inline double ld(const void* p, int s, size_t i)
     }
 }
 
-/* value of a bulk cell's population (deviation storage adds w_q in binary32) */
+static inline int is_dev(int s) { return s == OR_F32DEV || s == OR_F16DEV; }
+
+/* working value of a bulk cell's population: f for F64/F32 storage, the
+ * stored deviation d = f - w_q for the deviation storages (project-defined:
+ * their collision is written in d) */
 static inline double dec_bulk(const void* p, int s, size_t i, int q)
 {
-    if (s == OR_F64 || s == OR_F32) return ld(p, s, i);
-    return (double)(wf(q) + (float)ld(p, s, i));
+    (void)q;
+    return ld(p, s, i);
+}
+
+/* f of a bulk cell in binary64 (mass sums) */
+static inline double dec_full(const void* p, int s, size_t i, int q)
+{
+    if (!is_dev(s)) return ld(p, s, i);
+    return (double)wf(q) + ld(p, s, i);
 }
 
 /* own + neighbour on a wall link, added in the storage precision
@@ -146,16 +157,22 @@ static inline double add_p(int s, double a, double b)
     return (double)((float)a + (float)b);
 }
 
-static inline void st(void* p, int s, size_t i, int q, double v, int bulk)
+/* store a working value (bulk) or f (wall) */
+static inline void st(void* p, int s, size_t i, double v)
 {
     switch (s) {
         case OR_F64: ((double*)p)[i] = v; break;
-        case OR_F32: ((float*)p)[i] = (float)v; break;
-        case OR_F32DEV: ((float*)p)[i] = bulk ? (float)((float)v - wf(q)) : (float)v; break;
-        default:
-            ((uint16_t*)p)[i] = or_f32_to_f16(bulk ? (float)((float)v - wf(q)) : (float)v);
-            break;
+        case OR_F32:
+        case OR_F32DEV: ((float*)p)[i] = (float)v; break;
+        default: ((uint16_t*)p)[i] = or_f32_to_f16((float)v); break;
     }
+}
+
+/* store f of a bulk cell: deviation storages keep f - w_q, formed in
+ * binary64 and rounded through binary32 */
+static inline void st_full(void* p, int s, size_t i, int q, double f)
+{
+    st(p, s, i, is_dev(s) ? (double)(float)(f - W[q]) : f);
 }
 
 /* ---- macroscopic + BGK (LbmTools.h:171-282) ------------------------------ */
@@ -268,11 +285,82 @@ static void collide_ref(const float f[19], float omega, float out[19])
 DEF_FORCED(forced_d, double)
 DEF_FORCED(forced_f, float)
 
+/* Deviation form (project-defined), binary32: d = f - w_q, rho = 1 + sum d,
+ * u = sum c d / rho, feq - w_q = w_q (rho-1) + w_q rho (3 c.u + 4.5 (c.u)^2
+ * - 1.5 u^2); the expression trees of the device's bgk_dev / bgk_forced_dev. */
+static inline void macro_dev(const float d[19], float* drho, float* rho, float u[3])
+{
+    const float XM1 = d[0] + d[3] + d[4] + d[5] + d[6];
+    const float XP1 = d[10] + d[13] + d[14] + d[15] + d[16];
+    const float X0 = d[9] + d[1] + d[2] + d[7] + d[8] + d[11] + d[12] + d[17] + d[18];
+    const float YM1 = d[1] + d[3] + d[7] + d[8] + d[14];
+    const float YP1 = d[4] + d[11] + d[13] + d[17] + d[18];
+    const float ZM1 = d[2] + d[5] + d[7] + d[16] + d[18];
+    const float ZP1 = d[6] + d[8] + d[12] + d[15] + d[17];
+    *drho = XM1 + XP1 + X0;
+    *rho = 1.f + *drho;
+    u[0] = (XP1 - XM1) / *rho;
+    u[1] = (YP1 - YM1) / *rho;
+    u[2] = (ZP1 - ZM1) / *rho;
+}
+
+static void collide_dev(const float d[19], float omega, float out[19])
+{
+    float drho, rho, u[3];
+    macro_dev(d, &drho, &rho, u);
+    const float usqr = 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const float ck[9] = {u[0],        u[1],        u[2],        u[0] + u[1], u[0] - u[1],
+                         u[0] + u[2], u[0] - u[2], u[1] + u[2], u[1] - u[2]};
+    const float w18 = (float)(1. / 18.), w36 = (float)(1. / 36.), w3 = (float)(1. / 3.);
+    for (int i = 0; i < 9; i++) {
+        const float wi = i < 3 ? w18 : w36;
+        const float rw = rho * wi;
+        const float deq = wi * drho + rw * (4.5f * ck[i] * ck[i] - 3.f * ck[i] - usqr);
+        const float deqopp = deq + rw * 6.f * ck[i];
+        out[i] = (1.f - omega) * d[i] + omega * deq;
+        out[i + 10] = (1.f - omega) * d[i + 10] + omega * deqopp;
+    }
+    const float deq9 = w3 * drho - rho * w3 * usqr;
+    out[9] = (1.f - omega) * d[9] + omega * deq9;
+}
+
+static void forced_dev(const float d[19], float omega, const float g[3], float out[19])
+{
+    float drho, rho, u[3];
+    macro_dev(d, &drho, &rho, u);
+    u[0] += 0.5f * g[0];
+    u[1] += 0.5f * g[1];
+    u[2] += 0.5f * g[2];
+    const float usqr = 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const float ug = u[0] * g[0] + u[1] * g[1] + u[2] * g[2];
+    const float fpre = 1.f - 0.5f * omega;
+    for (int q = 0; q < 19; q++) {
+        const float wq = (float)W[q];
+        const float cu = (float)C[q][0] * u[0] + (float)C[q][1] * u[1] + (float)C[q][2] * u[2];
+        const float cg = (float)C[q][0] * g[0] + (float)C[q][1] * g[1] + (float)C[q][2] * g[2];
+        const float deq = wq * drho + rho * wq * (3.f * cu + 4.5f * cu * cu - usqr);
+        const float src = fpre * wq * rho * (3.f * (cg - ug) + 9.f * cu * cg);
+        out[q] = (1.f - omega) * d[q] + omega * deq + src;
+    }
+}
+
 /* collide one cell: f holds pulled values (exactly representable in the
  * pull precision), out receives compute-precision results as double */
 static void collide_any(int s, int cmode, const double f[19], double omega, const double g[3],
                         int forced, double out[19])
 {
+    float ff[19], of[19];
+    if (is_dev(s)) {
+        for (int q = 0; q < 19; q++) ff[q] = (float)f[q];
+        if (forced) {
+            const float gf[3] = {(float)g[0], (float)g[1], (float)g[2]};
+            forced_dev(ff, (float)omega, gf, of);
+        } else {
+            collide_dev(ff, (float)omega, of);
+        }
+        for (int q = 0; q < 19; q++) out[q] = (double)of[q];
+        return;
+    }
     if (cmode == OR_COMPUTE_F64 || s == OR_F64) {
         if (forced)
             forced_d(f, omega, g, out);
@@ -280,7 +368,6 @@ static void collide_any(int s, int cmode, const double f[19], double omega, cons
             collide_f64(f, omega, out);
         return;
     }
-    float ff[19], of[19];
     for (int q = 0; q < 19; q++) ff[q] = (float)f[q];
     if (forced) {
         const float gf[3] = {(float)g[0], (float)g[1], (float)g[2]};
@@ -353,11 +440,10 @@ void or_fill_dense(const or_dense* d, void* pop, const uint32_t* flags, double u
         const int cls = cls_of(flags[i]);
         for (int q = 0; q < 19; q++) {
             const double v = fill_value(cls, q, ulb, ulid);
-            /* deviation storage: bulk cells hold f - w = 0 exactly */
-            if (cls == OR_BULK && (d->storage == OR_F32DEV || d->storage == OR_F16DEV))
-                st(pop, d->storage, q * g.plane + i, q, (double)wf(q), 1);
+            if (cls == OR_BULK)
+                st_full(pop, d->storage, q * g.plane + i, q, v);
             else
-                st(pop, d->storage, q * g.plane + i, q, v, 0);
+                st(pop, d->storage, q * g.plane + i, v);
         }
     }
 }
@@ -459,7 +545,7 @@ void or_step_dense(const or_dense* d, const void* fin, void* fout, const uint32_
                 }
                 pull_dense(&g, d->storage, fin, fl, x, y, z, i, f);
                 collide_any(d->storage, d->compute, f, omega, gg, forced, o);
-                for (int q = 0; q < 19; q++) st(fout, d->storage, q * g.plane + i, q, o[q], 1);
+                for (int q = 0; q < 19; q++) st(fout, d->storage, q * g.plane + i, o[q]);
             }
 }
 
@@ -476,7 +562,13 @@ static void cell_moments(int s, int cmode, uint32_t fl, const double f[19], cons
 {
     const int cls = cls_of(fl);
     if (cls == OR_BULK) {
-        if (cmode == OR_COMPUTE_F64 || s == OR_F64) {
+        if (is_dev(s)) {
+            float ff[19], dr, r, uf[3];
+            for (int q = 0; q < 19; q++) ff[q] = (float)f[q];
+            macro_dev(ff, &dr, &r, uf);
+            *rho = r;
+            for (int a = 0; a < 3; a++) u[a] = uf[a];
+        } else if (cmode == OR_COMPUTE_F64 || s == OR_F64) {
             macro_d(f, rho, u);
         } else {
             float ff[19], r, uf[3];
@@ -533,7 +625,7 @@ double or_mass_dense(const or_dense* d, const void* fin, const uint32_t* flags)
     for (size_t i = 0; i < g.plane; i++) {
         if (cls_of(flags[i]) != OR_BULK) continue;
         double r = 0;
-        for (int q = 0; q < 19; q++) r += dec_bulk(fin, d->storage, q * g.plane + i, q);
+        for (int q = 0; q < 19; q++) r += dec_full(fin, d->storage, q * g.plane + i, q);
         m += r;
     }
     return m;
@@ -704,11 +796,11 @@ void or_fill_block(const or_block* d, void* pop, const uint32_t* flags, double u
         for (int v = 0; v < 512; v++) {
             const int cls = cls_of(flags[(size_t)b * 512 + v]);
             for (int q = 0; q < 19; q++) {
-                if (cls == OR_BULK && (d->storage == OR_F32DEV || d->storage == OR_F16DEV))
-                    st(pop, d->storage, bidx(b, q, v), q, (double)wf(q), 1);
+                if (cls == OR_BULK)
+                    st_full(pop, d->storage, bidx(b, q, v), q, W[q]);
                 else
-                    st(pop, d->storage, bidx(b, q, v), q,
-                       cls == OR_UNDEFINED ? 0. : fill_value(cls, q, ulb, ulid), 0);
+                    st(pop, d->storage, bidx(b, q, v),
+                       cls == OR_UNDEFINED ? 0. : fill_value(cls, q, ulb, ulid));
             }
         }
 }
@@ -783,7 +875,7 @@ void or_step_block(const or_block* d, const void* fin, void* fout, const uint32_
             double f[19], o[19];
             pull_block(d, fin, fl, (uint32_t)b, v, f);
             collide_any(d->storage, d->compute, f, omega, gg, forced, o);
-            for (int q = 0; q < 19; q++) st(fout, d->storage, bidx((uint32_t)b, q, v), q, o[q], 1);
+            for (int q = 0; q < 19; q++) st(fout, d->storage, bidx((uint32_t)b, q, v), o[q]);
         }
 }
 
@@ -819,7 +911,7 @@ double or_mass_block(const or_block* d, const void* fin, const uint32_t* flags)
         for (int v = 0; v < 512; v++) {
             if (!blk_active(d, b, v) || cls_of(flags[(size_t)b * 512 + v]) != OR_BULK) continue;
             double r = 0;
-            for (int q = 0; q < 19; q++) r += dec_bulk(fin, d->storage, bidx(b, q, v), q);
+            for (int q = 0; q < 19; q++) r += dec_full(fin, d->storage, bidx(b, q, v), q);
             m += r;
         }
     return m;

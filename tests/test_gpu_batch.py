@@ -8,12 +8,12 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def make_members(nx, seed=0, sizes=(12, 16, 20, 14)):
+def make_members(nx, seed=0, sizes=(12, 16, 20, 14), re_range=(50, 400)):
     rng = np.random.default_rng(seed)
     ms = []
     for k, n in enumerate(sizes):
         dim = (n, n + (k % 2) * 4, n - (k % 3))
-        re = rng.uniform(50, 400)
+        re = rng.uniform(*re_range)
         omega = 1.0 / (3.0 * 0.05 * dim[0] / re + 0.5)
         ms.append({"dim": dim, "classes": nx.cavity_classes(dim, k % 3, 2, 5, seed=100 + k),
                    "omega": omega, "ulb": 0.05, "ulid": (1.0, 0.0, 0.0)})
@@ -41,10 +41,11 @@ def test_batch_members_equal_standalone(nx, st, launch):
 
 
 def test_batch_run_retires_converged_members(nx):
-    ms = make_members(nx, seed=3, sizes=(10, 12, 14))
+    ms = make_members(nx, seed=3, sizes=(10, 12, 14), re_range=(5, 30))
     b = nx.CavityBatch(ms, "f64")
     b.init()
-    steps, res, left = b.run(max_steps=4000, check_every=20, tol=1e-5)
+    steps, res, left = b.run(max_steps=20000, check_every=20, tol=1e-5)
+    print("steps", steps, "residual", res)
     assert left == 0 and (res < 1e-5).all() and (steps > 0).all()
     assert len(set(steps.tolist())) > 1          # different members stop at different times
     for k, m in enumerate(ms):
@@ -52,15 +53,18 @@ def test_batch_run_retires_converged_members(nx):
         assert np.array_equal(b.populations(k), ref.populations()), k
     # graph replays after the eager steps above: run again from rest
     b.init()
-    steps2, res2, left2 = b.run(max_steps=4000, check_every=20, tol=1e-5)
+    steps2, res2, left2 = b.run(max_steps=20000, check_every=20, tol=1e-5)
     assert np.array_equal(steps, steps2) and np.array_equal(res, res2) and left2 == 0
 
 
 def test_batch_warm_start_from_converged_fields(nx):
-    ms = make_members(nx, seed=5, sizes=(12, 12))
-    b = nx.CavityBatch(ms, "f32")
+    # FP32 deviation storage: its residual floor (~2e-7 here) is below the
+    # tolerance; raw FP32 populations stall near 1e-5 on such small cavities
+    ms = make_members(nx, seed=5, sizes=(12, 12), re_range=(5, 30))
+    b = nx.CavityBatch(ms, "f32dev")
     b.init()
-    steps, _, _ = b.run(max_steps=6000, check_every=20, tol=1e-5)
+    steps, _, left0 = b.run(max_steps=20000, check_every=20, tol=1e-6)
+    assert left0 == 0
     # gather converged (rho, u) in arena order and restart from them
     rho = []
     u = []
@@ -71,6 +75,7 @@ def test_batch_warm_start_from_converged_fields(nx):
     drho = nx.DeviceBuffer.from_array(np.concatenate(rho).astype(np.float32))
     du = nx.DeviceBuffer.from_array(np.concatenate(u).astype(np.float32))
     b.init(drho, du)
-    steps_w, res_w, left = b.run(max_steps=6000, check_every=20, tol=1e-5)
+    steps_w, res_w, left = b.run(max_steps=20000, check_every=20, tol=1e-6)
+    print("cold", steps, "warm", steps_w)
     assert left == 0
     assert (steps_w < steps / 2).all(), (steps, steps_w)
