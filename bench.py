@@ -271,6 +271,8 @@ def main():
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--autotune", action="store_true",
                     help="time every launch configuration first and keep the fastest")
+    ap.add_argument("--vx", type=int, default=None,
+                    help="cells per lane of the dense step (1, or 16 bytes of cells: 4 f32, 2 f64)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 10 launches): the "
                          "reference for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -303,6 +305,10 @@ def main():
     stream = nx.Stream()
     lat = nx.DenseLattice((N, N, N), storage, compute)
     lat.setup_cavity(ulb, stream=stream.handle)
+    if a.vx is not None:
+        t = lat.tuning()
+        t["vx"] = a.vx
+        lat.set_tuning(**t)
     if a.autotune:
         lat.autotune(omega, stream=stream.handle)
         lat.setup_cavity(ulb, stream=stream.handle)

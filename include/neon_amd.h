@@ -24,7 +24,9 @@
  *   block (bGrid AoSoA, bPartition_imp.h:95-107): index
  *          (b*19 + q)*512 + x + 8*y + 64*z   (8^3 blocks)
  * Storage types: NX_F64, NX_F32 hold f; NX_F32DEV / NX_F16DEV hold f - w_q
- * for bulk cells and f itself for non-bulk (wall) cells.
+ * for bulk cells and f itself for non-bulk (wall) cells, and collide in
+ * binary32 on the deviations (rho = 1 + sum d, feq - w_q formed term by term)
+ * so that rounding is relative to f - w_q rather than to w_q.
  */
 #ifndef NEON_AMD_H
 #define NEON_AMD_H
@@ -262,6 +264,16 @@ int nx_batch_info(nx_batch b, uint32_t* n_members, uint64_t* n_cells, uint32_t* 
 int nx_batch_member(nx_batch b, uint32_t k, int which, nx_dense_desc* d, void** pop,
                     uint32_t** flags);
 int nx_batch_init(nx_batch b, const float* rho, const float* u, nx_stream s);
+/* After nx_batch_init from a predicted field: `iterations` times, two steps
+ * then every bulk cell's equilibrium part is moved to the prescribed u,
+ * f <- f - feq(rho, u(f)) + feq(rho, u) (consistent initial conditions after
+ * Mei, Luo, Lallemand & d'Humieres 2006): density and non-equilibrium parts
+ * adapt to the field before the free iteration starts.  Only the members
+ * whose byte in `members` (host, n_members; NULL: all) is non-zero step and
+ * are constrained.  Step counters are not advanced (the caller accounts for
+ * the 2 * iterations steps); every member is stepping afterwards. */
+int nx_batch_constrain(nx_batch b, const float* u, int32_t iterations, const uint8_t* members,
+                       nx_stream s);
 int nx_batch_step(nx_batch b, int32_t k_steps, nx_stream s);
 int nx_batch_run(nx_batch b, int32_t max_steps, int32_t check_every, double tol,
                  int32_t* steps, double* residual, uint32_t* n_unconverged);

@@ -3,7 +3,9 @@
     python neon_amd/build.py            # compile every source, link the library
 
 Each translation unit is compiled separately (in parallel) with hipcc and the
-objects are linked into neon_amd/lib/libneon_amd.so.  Flags:
+objects are linked into neon_amd/lib/libneon_amd.so; the C++ programs under
+neon_amd/apps/ (host code on include/neon_amd.hpp) are then built with g++
+into neon_amd/lib/.  Flags:
   * --offload-arch=gfx950       MI355X only
   * -ffp-contract=off           no FMA contraction: binary64 kernels match a CPU
                                 evaluation of the same expression tree bit for bit
@@ -24,6 +26,7 @@ REPO = ROOT.parent
 CSRC = ROOT / "csrc"
 OBJ = ROOT / "_build"
 LIB = ROOT / "lib" / "libneon_amd.so"
+APPS = ROOT / "apps"     # C++ programs on include/neon_amd.hpp, linked against LIB
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -68,6 +71,16 @@ def build(verbose: bool = True) -> pathlib.Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    for app in sorted(APPS.glob("*.cpp")):
+        exe = LIB.parent / app.stem
+        cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{REPO / 'include'}", str(app), "-o", str(exe) + ".tmp",
+               f"-L{LIB.parent}", "-lneon_amd", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"app build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        os.replace(str(exe) + ".tmp", exe)
+        if verbose:
+            print(f"built {exe}")
     return LIB
 
 

@@ -275,6 +275,53 @@ __global__ void __launch_bounds__(kTile)
     }
 }
 
+// Consistent initialisation after an equilibrium start (after Mei, Luo,
+// Lallemand & d'Humieres 2006): the equilibrium part of every bulk cell is
+// moved to the prescribed velocity u*, keeping the cell's density and its
+// non-equilibrium part,  f <- f - feq(rho, u(f)) + feq(rho, u*), in binary64
+// (the difference is the same for f and for d = f - w_q).
+template <int S>
+__global__ void __launch_bounds__(kTile)
+    k_batch_constrain(typename Stor<S>::T* __restrict__ p, const uint32_t* __restrict__ flags,
+                      const Member* __restrict__ members, const uint32_t* __restrict__ tile_member,
+                      const uint32_t* __restrict__ active, const float* __restrict__ u_in,
+                      uint32_t n_tiles, int tile)
+{
+    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const uint32_t m = tile_member[t];
+        if (!active[m]) continue;
+        const Member M = members[m];
+        for (int cc = threadIdx.x; cc < tile; cc += kTile) {
+            const uint32_t i = (t - M.tile_first) * (uint32_t)tile + (uint32_t)cc;
+            if (i >= M.n) break;
+            const uint64_t c = M.cell_base + (uint64_t)i;
+            if (cls_of(flags[c]) != NX_CELL_BULK) continue;
+            double f[Q], rho = 0.0, j[3] = {0.0, 0.0, 0.0};
+            for (int q = 0; q < Q; q++) {
+                f[q] = (double)Stor<S>::bulk(p[M.pop_base + (uint64_t)q * M.n + i], q);
+                rho += f[q];
+                j[0] += CX[q] * f[q];
+                j[1] += CY[q] * f[q];
+                j[2] += CZ[q] * f[q];
+            }
+            if constexpr (Stor<S>::kDev) rho += 1.0;
+            double u[3], us[3];
+            for (int a = 0; a < 3; a++) {
+                u[a] = j[a] / rho;
+                us[a] = (double)u_in[3ull * M.cell_base + (uint64_t)a * M.n + i];
+            }
+            const double uu = 1.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+            const double uus = 1.5 * (us[0] * us[0] + us[1] * us[1] + us[2] * us[2]);
+            for (int q = 0; q < Q; q++) {
+                const double cu = CX[q] * u[0] + CY[q] * u[1] + CZ[q] * u[2];
+                const double cus = CX[q] * us[0] + CY[q] * us[1] + CZ[q] * us[2];
+                const double d = rho * wq(q) * (3. * (cus - cu) + 4.5 * (cus * cus - cu * cu) - (uus - uu));
+                p[M.pop_base + (uint64_t)q * M.n + i] = Stor<S>::template enc<double>(f[q] + d, q);
+            }
+        }
+    }
+}
+
 template <int V>
 using IC = std::integral_constant<int, V>;
 
@@ -534,6 +581,37 @@ extern "C" int nx_batch_init(nx_batch b, const float* rho, const float* u, nx_st
     std::vector<uint32_t> ones(b->n_members, 1u), zeros(b->n_members, 0u);
     NX_CHECK_HIP(hipMemcpyAsync(b->active, ones.data(), b->n_members * 4, hipMemcpyHostToDevice, st));
     NX_CHECK_HIP(hipMemcpyAsync(b->steps_done, zeros.data(), b->n_members * 4, hipMemcpyHostToDevice, st));
+    NX_CHECK_HIP(hipStreamSynchronize(st));
+    return NX_OK;
+}
+
+// Consistent start: `iterations` times, two steps then k_batch_constrain on
+// the latest populations with the prescribed velocity u (layout of
+// nx_batch_init), for the members selected by `members` (host bytes, NULL:
+// all); the others do not step.  Every member is stepping again afterwards;
+// step counters and residual state are left untouched.
+extern "C" int nx_batch_constrain(nx_batch b, const float* u, int32_t iterations,
+                                  const uint8_t* members, nx_stream s)
+{
+    if (!b || !u || iterations < 0) return set_error(NX_ERR_INVALID, "nx_batch_constrain: bad arguments");
+    const hipStream_t st = (hipStream_t)s;
+    std::vector<uint32_t> sel(b->n_members, 1u), ones(b->n_members, 1u);
+    if (members)
+        for (uint32_t k = 0; k < b->n_members; k++) sel[k] = members[k] ? 1u : 0u;
+    NX_CHECK_HIP(hipMemcpyAsync(b->active, sel.data(), b->n_members * 4, hipMemcpyHostToDevice, st));
+    for (int it = 0; it < iterations; it++) {
+        if (int e = launch_steps(*b, 2, 0, st)) return e;
+        if (int e = by_storage(b->storage, [&](auto sc) {
+                constexpr int S = decltype(sc)::value;
+                using T = typename Stor<S>::T;
+                hipLaunchKernelGGL((k_batch_constrain<S>), dim3(std::min<uint32_t>(b->n_tiles, 8192)),
+                                   dim3(kTile), 0, st, (T*)b->pop[0], b->flags, b->d_members,
+                                   b->d_tile_member, b->active, u, b->n_tiles, b->wg);
+            }))
+            return e;
+        NX_CHECK_LAUNCH("k_batch_constrain");
+    }
+    NX_CHECK_HIP(hipMemcpyAsync(b->active, ones.data(), b->n_members * 4, hipMemcpyHostToDevice, st));
     NX_CHECK_HIP(hipStreamSynchronize(st));
     return NX_OK;
 }

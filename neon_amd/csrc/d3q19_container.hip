@@ -198,6 +198,105 @@ __global__ void __launch_bounds__(WG)
     }
 }
 
+// Vector variant: each lane owns VX consecutive cells of a row (X % VX == 0,
+// VX * sizeof(T) = 16 bytes), so every population is one 16-byte load at
+// i - off_q (dword aligned, which multi-dword global loads accept) and one
+// aligned 16-byte store at i.  A lane whose VX cells are
+// all interior takes the vector path; any other lane runs the per-cell path
+// of k_fused for each of its cells and stores per cell.
+template <int S, int CM, bool FORCE, bool PER, int WG, int VX, int LP>
+__global__ void __launch_bounds__(WG)
+    k_fused_vx(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
+               const uint32_t* __restrict__ flags, const uint32_t* __restrict__ bits, DGeom g, Par p,
+               int remap, int copy_walls)
+{
+    using T = typename Stor<S>::T;
+    using P = typename Stor<S>::P;
+    using R = typename Real<S, CM>::R;
+    typedef T VL __attribute__((ext_vector_type(VX), aligned(sizeof(T))));  // dword-aligned load
+    typedef T VS __attribute__((ext_vector_type(VX)));                      // aligned store
+    static_assert(VX * sizeof(T) == 16 && sizeof(T) >= 4, "16-byte lanes of dword elements");
+    const uint32_t wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t i0 = (wg * (uint32_t)WG + threadIdx.x) * (uint32_t)VX;
+    if (i0 >= g.n) return;
+    const uint32_t word = bits[i0 >> 5];
+    VL v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
+        // clamped into the plane: only lanes whose cells are all interior
+        // (so every upstream cell exists) use these values
+        const int64_t jr = (int64_t)i0 - off;
+        const uint32_t jj = jr < 0 ? 0u : (jr > (int64_t)(g.n - VX) ? g.n - (uint32_t)VX : (uint32_t)jr);
+        const VL* src = (const VL*)((const char*)(fin + q * g.plane) + (uint32_t)(jj * (uint32_t)sizeof(T)));
+        v[q] = LP ? __builtin_nontemporal_load(src) : *src;
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+#pragma unroll
+        for (int c = 0; c < VX; c++) {
+            T t = v[q][c];
+            pin(t);
+            v[q][c] = t;
+        }
+    constexpr uint32_t kAll = (1u << VX) - 1u;
+    const uint32_t nib = (word >> (i0 & 31u)) & kAll;
+    if (nib == kAll) {
+#pragma unroll
+        for (int c = 0; c < VX; c++) {
+            P f[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) f[q] = Stor<S>::bulk(v[q][c], q);
+            R out[Q];
+            collide<S, CM, FORCE>(f, p.omega, p.g, out);
+#pragma unroll
+            for (int q = 0; q < Q; q++) v[q][c] = Stor<S>::template enc<R>(out[q], q);
+        }
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            VS o;
+#pragma unroll
+            for (int c = 0; c < VX; c++) o[c] = v[q][c];
+            __builtin_nontemporal_store(o, (VS*)(fout + q * g.plane + i0));
+        }
+        return;
+    }
+#pragma unroll 1
+    for (int c = 0; c < VX; c++) {
+        const uint32_t i = i0 + (uint32_t)c;
+        T o[Q];
+        bool store = true;
+        P f[Q];
+        bool bulk = false;
+        if ((nib >> c) & 1u) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) f[q] = Stor<S>::bulk(v[q][c], q);
+            bulk = true;
+        } else {
+            const uint32_t fl = flags[i];
+            if (cls_of(fl) == NX_CELL_BULK) {
+                pull<S, PER>(fin, g, i, fl, f);
+                bulk = true;
+            } else if (copy_walls) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) o[q] = fin[q * g.plane + i];
+            } else {
+                store = false;
+            }
+        }
+        if (bulk) {
+            R out[Q];
+            collide<S, CM, FORCE>(f, p.omega, p.g, out);
+#pragma unroll
+            for (int q = 0; q < Q; q++) o[q] = Stor<S>::template enc<R>(out[q], q);
+        }
+        if (store) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) st_plane(fout + q * g.plane, i, o[q]);
+        }
+    }
+}
+
 // Launch configuration of k_fused per (storage, compute), process-wide.
 // Defaults: the fastest configuration measured at 256^3 on MI355X.
 nx_tuning g_tuning[4][3] = {
@@ -213,11 +312,16 @@ constexpr bool tunable()
     return (S == NX_F64 && CM == NX_COMPUTE_F64) || (S != NX_F64 && CM == NX_COMPUTE_F32);
 }
 
-bool valid_tuning(const nx_tuning& t)
+// cells per lane of the vector variant for a storage type: 16-byte lanes of
+// 4- or 8-byte elements (always dword aligned); binary16 keeps one cell per
+// lane (a shifted run of halves is only 2-byte aligned)
+constexpr int vx_of(int storage) { return storage == NX_F64 ? 2 : (storage == NX_F16DEV ? 1 : 4); }
+
+bool valid_tuning(const nx_tuning& t, int storage)
 {
     return (t.wg == 256 || t.wg == 512) && (t.cpt == 1 || t.cpt == 2) &&
            (t.load_policy == 0 || t.load_policy == 1) && (t.xcd_remap == 0 || t.xcd_remap == 1) &&
-           t.vx == 1;
+           (t.vx == 1 || t.vx == vx_of(storage));
 }
 
 template <int S, int CM, bool FORCE, bool PER>
@@ -232,6 +336,30 @@ void launch_fused(const nx_tuning& t, const nx_container_s& c, hipStream_t s)
                            (const T*)c.fin, (T*)c.fout, c.flags, c.interior, c.g, c.p,
                            t.xcd_remap, c.copy_walls);
     };
+    constexpr int VX = vx_of(S);
+    if constexpr (VX > 1) {
+    if (t.vx == VX && c.g.X % VX == 0) {
+        auto go_vx = [&](auto wgc, auto lpc) {
+            constexpr int WG = decltype(wgc)::value, LP = decltype(lpc)::value;
+            const uint32_t lanes = c.g.n / VX;
+            const uint32_t nb = (lanes + WG - 1) / WG;
+            hipLaunchKernelGGL((k_fused_vx<S, CM, FORCE, PER, WG, VX, LP>), dim3(nb), dim3(WG), 0, s,
+                               (const T*)c.fin, (T*)c.fout, c.flags, c.interior, c.g, c.p,
+                               t.xcd_remap, c.copy_walls);
+        };
+        if constexpr (tunable<S, CM>() && !FORCE && !PER) {
+            auto by_lp = [&](auto wgc) {
+                if (t.load_policy) go_vx(wgc, IC<1>{});
+                else go_vx(wgc, IC<0>{});
+            };
+            if (t.wg == 512) by_lp(IC<512>{});
+            else by_lp(IC<256>{});
+        } else {
+            go_vx(IC<256>{}, IC<0>{});
+        }
+        return;
+    }
+    }
     if constexpr (tunable<S, CM>() && !FORCE && !PER) {
         auto by_lp = [&](auto wgc, auto cptc) {
             if (t.load_policy) go(wgc, cptc, IC<1>{});
@@ -383,10 +511,10 @@ extern "C" int nx_d3q19_get_tuning_dense(const nx_dense_desc* d, nx_tuning* t)
 extern "C" int nx_d3q19_set_tuning_dense(const nx_dense_desc* d, const nx_tuning* t)
 {
     if (int e = check(d)) return e;
-    if (!t || !valid_tuning(*t))
+    if (!t || !valid_tuning(*t, d->storage))
         return set_error(NX_ERR_INVALID,
                          "tuning: wg in {256,512}, cpt in {1,2}, load_policy and xcd_remap in "
-                         "{0,1}, vx = 1");
+                         "{0,1}, vx = 1 or 16 bytes of cells (2 f64, 4 f32, 8 f16)");
     g_tuning[d->storage][d->compute] = *t;
     return NX_OK;
 }
@@ -403,11 +531,15 @@ extern "C" int nx_container_autotune(nx_container c, nx_stream s, nx_tuning* bes
     nx_tuning bt = slot;
     float bms = 1e30f;
     const int reps = 10;
+    const int nvx = vx_of(c->d.storage) > 1 ? 2 : 1;
+    for (int kv = 0; kv < nvx; kv++)
     for (int wg : {256, 512})
         for (int cpt : {1, 2})
             for (int lp : {0, 1})
                 for (int rm : {0, 1}) {
-                    const nx_tuning t{wg, cpt, lp, rm, 1};
+                    const int vx = kv ? vx_of(c->d.storage) : 1;
+                    if (vx > 1 && cpt == 2) continue;  // the vector variant has one lane of VX cells
+                    const nx_tuning t{wg, cpt, lp, rm, vx};
                     for (int r = 0; r < 3; r++)
                         if (int e = run_container(*c, t, (hipStream_t)s)) return e;
                     NX_CHECK_HIP(hipEventRecord(e0, (hipStream_t)s));

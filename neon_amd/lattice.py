@@ -484,6 +484,17 @@ class CavityBatch:
     def step(self, k: int, stream=None):
         check(lib.nx_batch_step(self.h, int(k), stream))
 
+    def constrain(self, u, iterations: int, members=None, stream=None):
+        """Consistent start after init(rho, u): `iterations` x (2 steps + move
+        the equilibrium part back to the prescribed u), for the members whose
+        entry of the boolean mask `members` is set (None: all)."""
+        sel = None
+        if members is not None:
+            sel = np.ascontiguousarray(members, np.uint8)
+            assert sel.size == self.n
+        check(lib.nx_batch_constrain(self.h, u.ptr, int(iterations),
+                                     sel.ctypes.data if sel is not None else None, stream))
+
     def run(self, max_steps: int, check_every: int, tol: float):
         steps = np.zeros(self.n, np.int32)
         res = np.zeros(self.n, np.float64)

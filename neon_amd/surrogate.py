@@ -167,14 +167,30 @@ def fields_from_prediction(pred: np.ndarray, members):
     return (np.concatenate(rho).astype(np.float32), np.concatenate(u).astype(np.float32))
 
 
-def generate(members, tol=1e-6, max_steps=200000, check_every=50, storage="f32dev"):
-    """Step every member to the residual with one batch; returns
-    (batch, steps, residual, unconverged, seconds)."""
+def converge(b: CavityBatch, tol, max_steps, check_every, chunk=10000, log=None):
+    """Run an initialised batch until every member's residual is below tol
+    (or max_steps), in chunks so that progress can be reported; returns
+    (steps, residual, unconverged, seconds)."""
+    t0 = time.perf_counter()
+    done = 0
+    while True:
+        n = min(chunk, max_steps - done)
+        steps, res, left = b.run(max_steps=n, check_every=check_every, tol=tol)
+        done += n
+        if log:
+            log(f"  {done} steps, {left} of {b.n} unconverged")
+        if left == 0 or done >= max_steps:
+            return steps, res, left, time.perf_counter() - t0
+
+
+def generate(members, tol=1e-6, max_steps=200000, check_every=50, storage="f32dev", log=None):
+    """Step every member to the residual with one batch (FP32 deviation
+    storage by default: raw FP32 populations stall near a residual of 1e-5
+    on small cavities); returns (batch, steps, residual, unconverged, seconds)."""
     b = CavityBatch(members, storage)
     b.init()
-    t0 = time.perf_counter()
-    steps, res, left = b.run(max_steps=max_steps, check_every=check_every, tol=tol)
-    return b, steps, res, left, time.perf_counter() - t0
+    steps, res, left, secs = converge(b, tol, max_steps, check_every, log=log)
+    return b, steps, res, left, secs
 
 
 # ---- training -----------------------------------------------------------------
@@ -244,16 +260,25 @@ def loss_of(pred: np.ndarray, y: np.ndarray, w: np.ndarray) -> float:
     return float((w[..., None] * (pred - y) ** 2).sum() / pred.shape[0])
 
 
-def warm_start(batch: CavityBatch, rho: np.ndarray, u: np.ndarray, check_every: int):
-    """Initialise `batch` from the predicted (rho, u), falling back to the
-    rest start for every member whose residual after one check interval is
-    not below the rest start's.  Leaves the batch initialised (0 steps taken)
-    and returns the boolean fallback mask."""
+CONSTRAIN_ITERATIONS = 10
+
+
+def warm_start(batch: CavityBatch, rho: np.ndarray, u: np.ndarray, check_every: int,
+               iterations: int = CONSTRAIN_ITERATIONS):
+    """Initialise `batch` from the predicted (rho, u): equilibrium populations,
+    then `iterations` constrained double steps (nx_batch_constrain) so that
+    density and non-equilibrium parts match the field.  A member whose
+    residual after one further check interval is not below the rest start's
+    falls back to the rest start.  Leaves the batch initialised (0 steps
+    counted); returns the boolean fallback mask.  Cost beyond the returned
+    state: 4 * iterations + 2 * check_every steps of every member."""
     dr, du = DeviceBuffer.from_array(rho), DeviceBuffer.from_array(u)
+    never = 1e-150            # no member retires during the probe intervals
     batch.init(dr, du)
-    _, res_w, _ = batch.run(max_steps=check_every, check_every=check_every, tol=0.0)
+    batch.constrain(du, iterations)
+    _, res_w, _ = batch.run(max_steps=check_every, check_every=check_every, tol=never)
     batch.init()
-    _, res_c, _ = batch.run(max_steps=check_every, check_every=check_every, tol=0.0)
+    _, res_c, _ = batch.run(max_steps=check_every, check_every=check_every, tol=never)
     fallback = ~(res_w < res_c)
     if fallback.any():
         rho = rho.copy()
@@ -268,4 +293,121 @@ def warm_start(batch: CavityBatch, rho: np.ndarray, u: np.ndarray, check_every: 
         dr.upload(rho)
         du.upload(u)
     batch.init(dr, du)
+    batch.constrain(du, iterations, members=~fallback)
     return fallback
+
+
+def warm_start_overhead(check_every: int, iterations: int = CONSTRAIN_ITERATIONS) -> int:
+    """Steps of every member spent by warm_start before the free iteration."""
+    return 4 * iterations + 2 * check_every
+
+
+def forward_flops(net: UNet) -> float:
+    """Multiply-add FLOPs of one forward pass per sample: sum over the
+    convolutions of 2 k^3 cin cout voxels."""
+    S, L, wd = net.size, net.levels, net.widths
+    f = 0.0
+    for l in range(L):
+        v = (S >> l) ** 3
+        cin = 16 if l == 0 else wd[l - 1]
+        f += 2 * 27 * v * (cin * wd[l] + wd[l] * wd[l])
+        if l < L - 1:
+            f += 2 * 27 * v * ((wd[l + 1] + wd[l]) * wd[l] + wd[l] * wd[l])
+    f += 2 * S ** 3 * wd[0] * 16
+    return f
+
+
+def main(argv=None):
+    """Warm-start experiment: generate steady cavities with the batched
+    solver, train the U-Net on the device, then iterate held-out cavities to
+    the residual from rest and from the predicted start.  One JSON line."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(prog="python -m neon_amd.surrogate")
+    ap.add_argument("--size", type=int, default=32)
+    ap.add_argument("--train", type=int, default=512)
+    ap.add_argument("--test", type=int, default=64)
+    ap.add_argument("--re", type=float, nargs=2, default=(20.0, 200.0))
+    ap.add_argument("--boxes", type=int, nargs=2, default=(0, 4))
+    ap.add_argument("--edges", type=int, nargs=2, default=(4, 16))
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--check-every", type=int, default=50)
+    ap.add_argument("--max-steps", type=int, default=200000)
+    ap.add_argument("--storage", default="f32dev")
+    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--widths", default="16,32,64,64")
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--save-params", default=None)
+    a = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
+    widths = tuple(int(v) for v in a.widths.split(","))
+    ms = make_cavities(a.train + a.test, a.size, a.seed, tuple(a.re), tuple(a.boxes), tuple(a.edges))
+    train_ms, test_ms = ms[:a.train], ms[a.train:]
+    log(f"generate {a.train} cavities {a.size}^3 to residual {a.tol:g}")
+    b, steps_g, _, left_g, t_gen = generate(train_ms, a.tol, a.max_steps, a.check_every, a.storage, log=log)
+    x = features(train_ms)
+    y, w = targets(b, train_ms)
+    cells_g = int(sum(np.prod(m["dim"]) for m in train_ms))
+    del b
+    net = UNet(a.size, a.levels, widths, max_batch=a.batch, seed=a.seed + 1)
+    data = Dataset(x, y, w)
+    log(f"train {net.n_params} parameters, {a.train_steps} steps of {a.batch}")
+    check(lib.nx_sync())
+    t0 = time.perf_counter()
+    hist = train(net, data, a.train_steps, a.batch, a.lr, seed=a.seed, log_every=200, log=log)
+    t_train = time.perf_counter() - t0
+    flops = 3.0 * forward_flops(net) * a.batch * a.train_steps
+    if a.save_params:
+        np.save(a.save_params, net.params())
+    xt = features(test_ms)
+
+    log(f"held out: {a.test} cavities from rest")
+    cold = CavityBatch(test_ms, a.storage)
+    cold.init()
+    check(lib.nx_sync())
+    steps_c, res_c, left_c, t_cold = converge(cold, a.tol, a.max_steps, a.check_every, log=log)
+    yt, wt = targets(cold, test_ms)
+    del cold
+
+    log("held out: from the predicted start")
+    warm = CavityBatch(test_ms, a.storage)
+    check(lib.nx_sync())
+    t0 = time.perf_counter()
+    pred = predict(net, xt)
+    t_inf = time.perf_counter() - t0
+    rho, u = fields_from_prediction(pred, test_ms)
+    fb = warm_start(warm, rho, u, a.check_every)
+    t_probe = time.perf_counter() - t0 - t_inf
+    steps_w, res_w, left_w, t_w = converge(warm, a.tol, a.max_steps, a.check_every, log=log)
+    t_warm = time.perf_counter() - t0
+    probe = warm_start_overhead(a.check_every)
+    out = {
+        "experiment": "learned warm start vs rest start, held-out cavities",
+        "size": a.size, "train": a.train, "test": a.test, "re": list(a.re), "boxes": list(a.boxes),
+        "edges": list(a.edges), "tol": a.tol, "check_every": a.check_every, "storage": a.storage,
+        "generate": {"seconds": round(t_gen, 2), "unconverged": int(left_g),
+                     "mean_steps": float(steps_g.mean()), "max_steps": int(steps_g.max()),
+                     "MLUPS": round(cells_g * float(steps_g.mean()) / t_gen / 1e6, 1)},
+        "network": {"levels": a.levels, "widths": list(widths), "params": net.n_params},
+        "train": {"steps": a.train_steps, "batch": a.batch, "seconds": round(t_train, 2),
+                  "loss_first": hist[0][1], "loss_last": hist[-1][1],
+                  "TFLOPs": round(flops / t_train / 1e12, 2)},
+        "held_out_loss": loss_of(pred, yt, wt),
+        "cold": {"steps_mean": float(steps_c.mean()), "steps_total": int(steps_c.sum()),
+                 "seconds": round(t_cold, 3), "unconverged": int(left_c)},
+        "warm": {"steps_mean": float(steps_w.mean() + probe), "steps_total": int(steps_w.sum() + probe * a.test),
+                 "seconds": round(t_warm, 3), "inference_s": round(t_inf, 3),
+                 "probe_s": round(t_probe, 3), "unconverged": int(left_w),
+                 "fallback": int(fb.sum())},
+    }
+    out["speedup_steps"] = round(out["cold"]["steps_total"] / max(1, out["warm"]["steps_total"]), 3)
+    out["speedup_time"] = round(t_cold / t_warm, 3)
+    print(json.dumps(out), flush=True)
+    return out
+
+
+if __name__ == "__main__":
+    main()

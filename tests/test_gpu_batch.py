@@ -74,8 +74,12 @@ def test_batch_warm_start_from_converged_fields(nx):
         u.append(uu.reshape(-1))
     drho = nx.DeviceBuffer.from_array(np.concatenate(rho).astype(np.float32))
     du = nx.DeviceBuffer.from_array(np.concatenate(u).astype(np.float32))
+    # an equilibrium start alone lacks the non-equilibrium part and relaxes
+    # almost as slowly as a start from rest; 10 constrained double steps
+    # restore a consistent state
     b.init(drho, du)
+    b.constrain(du, 10)
     steps_w, res_w, left = b.run(max_steps=20000, check_every=20, tol=1e-6)
     print("cold", steps, "warm", steps_w)
     assert left == 0
-    assert (steps_w < steps / 2).all(), (steps, steps_w)
+    assert (steps_w + 20 < steps / 2).all(), (steps, steps_w)

@@ -211,3 +211,51 @@ def test_container_leaves_differing_walls_unwritten(nx, orc):
     g.parity ^= 1
     o.parity ^= 1
     assert np.array_equal(g.populations(), o.fin)
+
+
+@pytest.mark.parametrize("st,cm,vx", [("f64", "f64", 2), ("f32", "f32", 4), ("f32", "ref", 4),
+                                      ("f32dev", "f32", 4)])
+@pytest.mark.parametrize("wg,lp", [(256, 0), (512, 1)])
+def test_container_vector_lanes_equal_plain_step(nx, st, cm, vx, wg, lp):
+    """The 16-byte-lane variant (vx cells per lane) gives the populations of
+    the single-kernel step bit for bit: obstacles make lanes with mixed
+    interior / boundary / wall cells; periodic y and a body force."""
+    dim = (40, 12, 21)
+    X, Y, Z = dim
+    rng = np.random.default_rng(12)
+    cls = np.full((Z, Y, X), 2, np.uint32)
+    cls[rng.random((Z, Y, X)) < 0.04] = 0
+    cls[:, :, 0] = 1
+    cls[:, :, X - 1] = 0
+    a = nx.DenseLattice(dim, st, cm, periodic=2)
+    b = nx.DenseLattice(dim, st, cm, periodic=2)
+    for lat in (a, b):
+        lat.set_classes(cls.reshape(-1), ulb=0.04)
+    force = (0.0, 0.0, 0.0) if cm == "ref" else (2e-5, 1e-5, 0.0)
+    try:
+        a.set_tuning(wg=wg, cpt=1, load_policy=lp, xcd_remap=1, vx=vx)
+        a.run(9, 1.6, force=force)
+    finally:
+        a.set_tuning()
+    b.run_plain(9, 1.6, force=force)
+    assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
+
+
+def test_container_vector_lanes_cavity(nx, orc):
+    N = 32
+    g, o = _pair(nx, orc, (N, N, N), "f32", "f32", 1, 1)
+    g.setup_cavity(0.05)
+    o.setup_cavity(0.05)
+    try:
+        g.set_tuning(vx=4)
+        g.run(12, 1.7)
+    finally:
+        g.set_tuning()
+    o.run(12, 1.7)
+    assert np.array_equal(g.populations(), o.fin)
+
+
+def test_vector_lanes_rejected_for_half(nx):
+    g = nx.DenseLattice((16, 16, 16), "f16dev", "f32")
+    with pytest.raises(nx.abi.NeonError):
+        g.set_tuning(vx=4)

@@ -35,8 +35,8 @@ def test_warm_start_beats_rest_on_held_out_cavities(nx):
     print("cold", steps_c.tolist(), "warm", steps_w.tolist(), "fallback", fb.tolist())
     assert left_c == 0 and left_w == 0
     assert not fb.any()
-    # the warm start's two check intervals (predicted and rest start) count against it
-    assert (steps_w + 2 * every).sum() < 0.8 * steps_c.sum()
+    # the warm start's own steps (constrained start, two probe intervals) count against it
+    assert (steps_w + S.warm_start_overhead(every)).sum() < 0.8 * steps_c.sum()
 
     # a start with the flow reversed is worse than rest: every member falls back
     fb2 = S.warm_start(warm, rho, -u, every)
