@@ -253,6 +253,9 @@ int nx_d3q19_residual_block(const nx_block_desc* d, const void* fin,
  * replays it until every member's residual
  *   r = sqrt(sum |u_t - u_{t-K}|^2 / sum |u_t|^2)   (bulk cells, FP64 sums)
  * is below tol (converged members stop stepping) or max_steps are taken.
+ * A member whose residual sums are not finite (its flow diverged) stops
+ * stepping too and reports a NaN residual; n_unconverged counts only the
+ * members still stepping. 
  * nx_batch_init sets bulk populations to the equilibrium of (rho, u)
  * (device arrays in arena cell order; NULL: rest).  nx_batch_member returns a
  * member's dense view (arena 0 holds the latest populations after an even

@@ -217,7 +217,9 @@ __global__ void __launch_bounds__(256)
         res[2 * m] = acc[0];
         res[2 * m + 1] = acc[1];
         steps_done[m] += (uint32_t)k_steps;
-        if (acc[0] < tol2 * acc[1]) active[m] = 0;
+        // converged, or diverged (a non-finite sum: the member's flow blew up)
+        const bool finite = acc[0] - acc[0] == 0.0 && acc[1] - acc[1] == 0.0;
+        if (!finite || acc[0] < tol2 * acc[1]) active[m] = 0;
     }
 }
 
@@ -664,8 +666,10 @@ extern "C" int nx_batch_run(nx_batch b, int32_t max_steps, int32_t check_every, 
     if (residual) {
         std::vector<double> r(2 * (size_t)b->n_members);
         NX_CHECK_HIP(hipMemcpy(r.data(), b->res, r.size() * 8, hipMemcpyDeviceToHost));
-        for (uint32_t k = 0; k < b->n_members; k++)
-            residual[k] = r[2 * k + 1] > 0 ? std::sqrt(r[2 * k] / r[2 * k + 1]) : 0.0;
+        for (uint32_t k = 0; k < b->n_members; k++) {
+            const bool finite = std::isfinite(r[2 * k]) && std::isfinite(r[2 * k + 1]);
+            residual[k] = !finite ? std::nan("") : (r[2 * k + 1] > 0 ? std::sqrt(r[2 * k] / r[2 * k + 1]) : 0.0);
+        }
     }
     return NX_OK;
 }

@@ -140,14 +140,17 @@ def features(members) -> np.ndarray:
 
 
 def targets(batch: CavityBatch, members):
-    """Steady moments of every member -> [n, S^3, 4] = ((rho-1)/ulb^2, u/ulb)
-    and loss weights [n, S^3] (bulk cells, each member's mean)."""
+    """Steady moments of every member -> [n, S^3, 4] = ((rho-1)/ulb, u/ulb)
+    and loss weights [n, S^3] (bulk cells, each member's mean).  The density
+    target is scaled by ulb, not ulb^2: the pressure peaks at the lid corners
+    would otherwise dominate the loss (the constrained start re-forms the
+    density field anyway)."""
     ys, ws = [], []
     for k, m in enumerate(members):
         rho, u = batch.moments(k)
         ulb = m["ulb"]
         y = np.empty((rho.size, OUT_CH), np.float32)
-        y[:, 0] = (rho.reshape(-1) - 1.0) / ulb ** 2
+        y[:, 0] = (rho.reshape(-1) - 1.0) / ulb
         y[:, 1:] = u.reshape(3, -1).T / ulb
         bulk = np.asarray(m["classes"]).reshape(-1) == abi.NX_CELL_BULK
         y[~bulk] = 0.0
@@ -156,13 +159,30 @@ def targets(batch: CavityBatch, members):
     return np.stack(ys), np.stack(ws)
 
 
+def mirror_z(x: np.ndarray, y: np.ndarray, w: np.ndarray, size: int):
+    """The z-mirrored copies of a data set: the lid-driven cube is symmetric
+    under z -> N-1-z with u_z -> -u_z (lid along x on the y = N-1 face)."""
+    n = x.shape[0]
+    f = lambda a: a.reshape(n, size, size, size, -1)[:, ::-1].reshape(n, size ** 3, -1)
+    ym = f(y).copy()
+    ym[..., 3] *= -1.0
+    return f(x).copy(), ym, f(w[..., None])[..., 0].copy()
+
+
+def velocity_error(pred: np.ndarray, y: np.ndarray, w: np.ndarray) -> float:
+    """Relative L2 error of the predicted velocity over bulk cells."""
+    bulk = w > 0
+    du = pred[..., 1:] - y[..., 1:]
+    return float(np.sqrt((du[bulk] ** 2).sum() / (y[..., 1:][bulk] ** 2).sum()))
+
+
 def fields_from_prediction(pred: np.ndarray, members):
     """Network output [n, S^3, 4] -> (rho, u) in the batch's arena order
     (rho per cell; u as 3 planes per member), float32."""
     rho, u = [], []
     for k, m in enumerate(members):
         ulb = m["ulb"]
-        rho.append(1.0 + pred[k, :, 0] * ulb ** 2)
+        rho.append(1.0 + pred[k, :, 0] * ulb)
         u.append((pred[k, :, 1:] * ulb).T.reshape(-1))
     return (np.concatenate(rho).astype(np.float32), np.concatenate(u).astype(np.float32))
 
@@ -181,6 +201,11 @@ def converge(b: CavityBatch, tol, max_steps, check_every, chunk=10000, log=None)
             log(f"  {done} steps, {left} of {b.n} unconverged")
         if left == 0 or done >= max_steps:
             return steps, res, left, time.perf_counter() - t0
+
+
+def converged(res: np.ndarray, tol: float) -> np.ndarray:
+    """Members that reached the residual (a diverged member reports NaN)."""
+    return np.isfinite(res) & (res < tol)
 
 
 def generate(members, tol=1e-6, max_steps=200000, check_every=50, storage="f32dev", log=None):
@@ -341,17 +366,26 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--save-params", default=None)
+    ap.add_argument("--mirror", action="store_true", help="add the z-mirrored copies to the training set")
     a = ap.parse_args(argv)
     log = lambda m: print(m, flush=True)
     widths = tuple(int(v) for v in a.widths.split(","))
     ms = make_cavities(a.train + a.test, a.size, a.seed, tuple(a.re), tuple(a.boxes), tuple(a.edges))
     train_ms, test_ms = ms[:a.train], ms[a.train:]
     log(f"generate {a.train} cavities {a.size}^3 to residual {a.tol:g}")
-    b, steps_g, _, left_g, t_gen = generate(train_ms, a.tol, a.max_steps, a.check_every, a.storage, log=log)
-    x = features(train_ms)
+    b, steps_g, res_g, left_g, t_gen = generate(train_ms, a.tol, a.max_steps, a.check_every, a.storage,
+                                                log=log)
+    ok = converged(res_g, a.tol)
+    x = features(train_ms)[ok]
     y, w = targets(b, train_ms)
+    y, w = y[ok], w[ok]
     cells_g = int(sum(np.prod(m["dim"]) for m in train_ms))
     del b
+    log(f"training set: {int(ok.sum())} converged, {int((~np.isfinite(res_g)).sum())} diverged, "
+        f"{int(left_g)} unconverged")
+    if a.mirror:
+        xm, ym, wm = mirror_z(x, y, w, a.size)
+        x, y, w = np.concatenate([x, xm]), np.concatenate([y, ym]), np.concatenate([w, wm])
     net = UNet(a.size, a.levels, widths, max_batch=a.batch, seed=a.seed + 1)
     data = Dataset(x, y, w)
     log(f"train {net.n_params} parameters, {a.train_steps} steps of {a.batch}")
@@ -366,11 +400,24 @@ def main(argv=None):
 
     log(f"held out: {a.test} cavities from rest")
     cold = CavityBatch(test_ms, a.storage)
+    # (members that diverge from rest are dropped from both comparisons)
     cold.init()
     check(lib.nx_sync())
     steps_c, res_c, left_c, t_cold = converge(cold, a.tol, a.max_steps, a.check_every, log=log)
     yt, wt = targets(cold, test_ms)
     del cold
+    keep = converged(res_c, a.tol)
+    if not keep.all():
+        test_ms = [m for m, k in zip(test_ms, keep) if k]
+        xt, yt, wt = xt[keep], yt[keep], wt[keep]
+        steps_c = steps_c[keep]
+        log(f"  {int((~keep).sum())} held-out cavities did not converge from rest; dropped, cold run repeated")
+        cold = CavityBatch(test_ms, a.storage)
+        cold.init()
+        check(lib.nx_sync())
+        steps_c, res_c, left_c, t_cold = converge(cold, a.tol, a.max_steps, a.check_every, log=log)
+        del cold
+    n_test = len(test_ms)
 
     log("held out: from the predicted start")
     warm = CavityBatch(test_ms, a.storage)
@@ -386,7 +433,7 @@ def main(argv=None):
     probe = warm_start_overhead(a.check_every)
     out = {
         "experiment": "learned warm start vs rest start, held-out cavities",
-        "size": a.size, "train": a.train, "test": a.test, "re": list(a.re), "boxes": list(a.boxes),
+        "size": a.size, "train": int(len(x)), "test": n_test, "re": list(a.re), "boxes": list(a.boxes),
         "edges": list(a.edges), "tol": a.tol, "check_every": a.check_every, "storage": a.storage,
         "generate": {"seconds": round(t_gen, 2), "unconverged": int(left_g),
                      "mean_steps": float(steps_g.mean()), "max_steps": int(steps_g.max()),
@@ -396,9 +443,10 @@ def main(argv=None):
                   "loss_first": hist[0][1], "loss_last": hist[-1][1],
                   "TFLOPs": round(flops / t_train / 1e12, 2)},
         "held_out_loss": loss_of(pred, yt, wt),
+        "held_out_velocity_rel_l2": velocity_error(pred, yt, wt),
         "cold": {"steps_mean": float(steps_c.mean()), "steps_total": int(steps_c.sum()),
                  "seconds": round(t_cold, 3), "unconverged": int(left_c)},
-        "warm": {"steps_mean": float(steps_w.mean() + probe), "steps_total": int(steps_w.sum() + probe * a.test),
+        "warm": {"steps_mean": float(steps_w.mean() + probe), "steps_total": int(steps_w.sum() + probe * n_test),
                  "seconds": round(t_warm, 3), "inference_s": round(t_inf, 3),
                  "probe_s": round(t_probe, 3), "unconverged": int(left_w),
                  "fallback": int(fb.sum())},
