@@ -280,6 +280,11 @@ int nx_batch_constrain(nx_batch b, const float* u, int32_t iterations, const uin
 int nx_batch_step(nx_batch b, int32_t k_steps, nx_stream s);
 int nx_batch_run(nx_batch b, int32_t max_steps, int32_t check_every, double tol,
                  int32_t* steps, double* residual, uint32_t* n_unconverged);
+/* Launch of the persistent step: tiles of wg (256 or 512) cells, wgs_per_cu
+ * workgroups per CU (0: one workgroup per tile).  A persistent grid deals
+ * the tiles to the 8 XCDs in chunks of 65536 consecutive cells, so that one
+ * L2 serves a member's neighbouring planes; nx_batch_autotune also times
+ * tile-by-tile dealing and keeps the faster. */
 int nx_batch_set_launch(nx_batch b, int32_t wg, int32_t wgs_per_cu);
 int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int32_t* best_wgs_per_cu,
                       float* best_ms);

@@ -63,6 +63,7 @@ struct nx_batch_s {
     // launch configuration
     int wg = 256;
     int wgs_per_cu = 8;
+    int xcd_chunks = 1;
     int n_cu = 256;
     // captured graph
     hipGraphExec_t graph = nullptr;
@@ -134,21 +135,42 @@ __device__ __forceinline__ DGeom geom_of(const Member& M)
     return g;
 }
 
-// Persistent step: a fixed grid walks the tile table.
+constexpr uint32_t kChunkCells = 65536;  // cells per XCD chunk of tiles
+
+// Persistent step: a fixed grid walks the tile table.  With xcd_chunks the
+// tiles are dealt to the 8 XCDs in chunks of kChunkCells consecutive cells
+// (chunk c -> XCD c % 8; workgroup w runs on XCD w % 8, the hardware's
+// round-robin dispatch) and the workgroups of one XCD walk their chunks in
+// order: the tiles one L2 serves at a time are contiguous in the arena, so
+// the z-neighbour planes of a member (at most 36 tiles away for 96^3) are
+// read through the same L2.  Dealt tile by tile instead, a tile and its z
+// neighbours sit on different XCDs and every plane is fetched ~3 times.
 template <int S, int WG>
 __global__ void __launch_bounds__(WG)
     k_batch_step(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
                  const uint32_t* __restrict__ flags, const Member* __restrict__ members,
                  const uint32_t* __restrict__ tile_member, const uint32_t* __restrict__ active,
-                 uint32_t n_tiles)
+                 uint32_t n_tiles, int xcd_chunks)
 {
-    for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    auto tile = [&](uint32_t t) {
         const uint32_t m = tile_member[t];
-        if (!active[m]) continue;
+        if (!active[m]) return;
         const Member M = members[m];
         const DGeom g = geom_of(M);
         const uint32_t i = (t - M.tile_first) * (uint32_t)WG + threadIdx.x;
         step_cell<S>(fin + M.pop_base, fout + M.pop_base, flags + M.cell_base, g, M, i);
+    };
+    if (xcd_chunks) {
+        constexpr uint32_t C = kChunkCells / (uint32_t)WG;  // tiles per chunk
+        const uint32_t xcd = blockIdx.x & 7u, G8 = gridDim.x >> 3;
+        for (uint32_t k = blockIdx.x >> 3;; k += G8) {
+            const uint32_t c = xcd + 8u * (k / C);
+            if (c * C >= n_tiles) break;  // this XCD's chunks are done
+            const uint32_t t = c * C + k % C;
+            if (t < n_tiles) tile(t);
+        }
+    } else {
+        for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) tile(t);
     }
 }
 
@@ -350,6 +372,13 @@ unsigned grid_of(const nx_batch_s& b)
     return std::max(1u, std::min<unsigned>(b.n_tiles, cap));
 }
 
+// XCD chunking needs a persistent grid that is a multiple of the 8 XCDs
+bool chunked(const nx_batch_s& b)
+{
+    const unsigned G = grid_of(b);
+    return b.xcd_chunks && G % 8 == 0 && G < b.n_tiles;
+}
+
 int launch_steps(nx_batch_s& b, int k_steps, int parity0, hipStream_t s)
 {
     for (int k = 0; k < k_steps; k++) {
@@ -362,11 +391,11 @@ int launch_steps(nx_batch_s& b, int k_steps, int parity0, hipStream_t s)
                 if (b.wg == 512)
                     hipLaunchKernelGGL((k_batch_step<S, 512>), dim3(grid_of(b)), dim3(512), 0, s,
                                        (const T*)fin, (T*)fout, b.flags, b.d_members,
-                                       b.d_tile_member, b.active, b.n_tiles);
+                                       b.d_tile_member, b.active, b.n_tiles, (int)chunked(b));
                 else
                     hipLaunchKernelGGL((k_batch_step<S, 256>), dim3(grid_of(b)), dim3(256), 0, s,
                                        (const T*)fin, (T*)fout, b.flags, b.d_members,
-                                       b.d_tile_member, b.active, b.n_tiles);
+                                       b.d_tile_member, b.active, b.n_tiles, (int)chunked(b));
             }))
             return e;
     }
@@ -700,11 +729,13 @@ extern "C" int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int3
     hipEvent_t e0, e1;
     NX_CHECK_HIP(hipEventCreate(&e0));
     NX_CHECK_HIP(hipEventCreate(&e1));
-    int bw = b->wg, bc = b->wgs_per_cu;
+    int bw = b->wg, bc = b->wgs_per_cu, bx = b->xcd_chunks;
     float bms = 1e30f;
+    for (int xc : {1, 0})
     for (int wg : {256, 512})
         for (int c : {2, 4, 8, 16, 0}) {
             if (int e = nx_batch_set_launch(b, wg, c)) return e;
+            b->xcd_chunks = xc;
             if (int e = launch_steps(*b, 2, 0, st)) return e;
             NX_CHECK_HIP(hipEventRecord(e0, st));
             if (int e = launch_steps(*b, 6, 0, st)) return e;
@@ -717,11 +748,13 @@ extern "C" int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int3
                 bms = ms;
                 bw = wg;
                 bc = c;
+                bx = xc;
             }
         }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (int e = nx_batch_set_launch(b, bw, bc)) return e;
+    b->xcd_chunks = bx;
     if (best_wg) *best_wg = bw;
     if (best_wgs_per_cu) *best_wgs_per_cu = bc;
     if (best_ms) *best_ms = bms;

@@ -240,23 +240,29 @@ __global__ void __launch_bounds__(kBV)
         for (int q = 0; q < Q; q++) f[q] = Stor<S>::bulk(val[q], q);
         const uint32_t m = fl & NX_MASK_BITS;
         if (m) {
+            // loads only for the masked links, issued as one batch (predicated
+            // loads, no wait between them); the wall's value only where the
+            // link can have an upstream voxel (not NX_FLAG_OPEN, or MIXED)
             const bool open = (fl & NX_FLAG_OPEN) != 0, mixed = (fl & NX_FLAG_MIXED) != 0;
+            const bool need_nb = !open || mixed;
             T ownv[Q], nbv[Q];
             uint32_t okm = 0;
             for_q(AllQ{}, [&](auto qc) {
                 constexpr int q = decltype(qc)::value;
                 if constexpr (q != 9) {
-                    uint32_t t, nv;
-                    upstream<q>(b, cb, x, y, z, t, nv);
-                    bool ok = !open && t != kInvalidBlock;
-                    if (mixed)
-                        ok = t != kInvalidBlock &&
-                             ((g.masks[(uint64_t)t * 16 + (nv >> 5)] >> (nv & 31u)) & 1u);
                     const bool w = (m >> q) & 1u;
+                    ownv[q] = w ? fin[own + (uint64_t)OPP[q] * kBV] : T(0);
+                    bool ok = false;
+                    uint32_t t = kInvalidBlock, nv = 0;
+                    if (w && need_nb) {
+                        upstream<q>(b, cb, x, y, z, t, nv);
+                        ok = !open && t != kInvalidBlock;
+                        if (mixed)
+                            ok = t != kInvalidBlock &&
+                                 ((g.masks[(uint64_t)t * 16 + (nv >> 5)] >> (nv & 31u)) & 1u);
+                    }
                     okm |= (ok ? 1u : 0u) << q;
-                    const uint64_t oa = own + (uint64_t)OPP[q] * kBV;
-                    ownv[q] = fin[oa];
-                    nbv[q] = fin[(w && ok) ? (uint64_t)t * kBStride + (uint32_t)OPP[q] * kBV + nv : oa];
+                    nbv[q] = ok ? fin[(uint64_t)t * kBStride + (uint32_t)OPP[q] * kBV + nv] : T(0);
                 }
             });
 #pragma unroll
