@@ -273,6 +273,8 @@ def main():
                     help="time every launch configuration first and keep the fastest")
     ap.add_argument("--vx", type=int, default=None,
                     help="cells per lane of the dense step (1, or 16 bytes of cells: 4 f32, 2 f64)")
+    ap.add_argument("--boundary", type=int, default=None,
+                    help="dense step boundary handling: 0 interior bitmap + full pull, 1 one collision with fix-ups")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 10 launches): the "
                          "reference for scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -305,9 +307,12 @@ def main():
     stream = nx.Stream()
     lat = nx.DenseLattice((N, N, N), storage, compute)
     lat.setup_cavity(ulb, stream=stream.handle)
-    if a.vx is not None:
+    if a.vx is not None or a.boundary is not None:
         t = lat.tuning()
-        t["vx"] = a.vx
+        if a.vx is not None:
+            t["vx"] = a.vx
+        if a.boundary is not None:
+            t["boundary"] = a.boundary
         lat.set_tuning(**t)
     if a.autotune:
         lat.autotune(omega, stream=stream.handle)

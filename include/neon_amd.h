@@ -110,7 +110,11 @@ typedef struct {
     int32_t cpt;         /* cells per thread: 1 or 2                      */
     int32_t load_policy; /* 0: cached loads, 1: non-temporal loads        */
     int32_t xcd_remap;   /* 1: contiguous slab of the grid per XCD        */
-    int32_t vx;          /* cells per lane along x: 1                     */
+    int32_t vx;          /* cells per lane along x: 1, or 16 bytes of     */
+                         /* cells (2 f64, 4 f32); not binary16            */
+    int32_t boundary;    /* 0: interior bitmap, other lanes run the full  */
+                         /* pull; 1: every lane reads its flags, masked   */
+                         /* links are fixed up, one collision for all     */
 } nx_tuning;
 
 /* ---- runtime (replaces libNeonSys GpuDevice / Backend, GpuDevice.h:151-190,

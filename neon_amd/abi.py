@@ -48,7 +48,7 @@ class BlockDesc(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("wg", C.c_int32), ("cpt", C.c_int32), ("load_policy", C.c_int32),
-                ("xcd_remap", C.c_int32), ("vx", C.c_int32)]
+                ("xcd_remap", C.c_int32), ("vx", C.c_int32), ("boundary", C.c_int32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

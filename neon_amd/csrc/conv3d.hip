@@ -97,12 +97,13 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int n = 0; n < NT; n++) acc[t][n] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    // K position of this lane group: kk = 32*kc + 8*grp = tap*cin + ci
+    // K position of this lane group: kk = 32*kc + 8*grp = tap*cin + ci.
+    // Fragments are double-buffered: the loads of chunk kc+1 are issued
+    // before the MFMAs of chunk kc.
     int tap = 0, ci = 8 * grp;
     while (ci >= cin) { ci -= cin; ++tap; }
-    for (int kc = 0; kc < KC; ++kc) {
+    auto fetch = [&](int kc, bf16x8* a, bf16x8* b) {
         const int kk = kc * 32 + 8 * grp;
-        bf16x8 a[2], b[NT];
         if (kk < K) {
             int dz, dy, dx;
             tap_offset<KS>(tap, dz, dy, dx);
@@ -122,12 +123,24 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
             for (int n = 0; n < NT; n++) b[n] = bf16x8{};
         }
+        ci += 32;
+        while (ci >= cin) { ci -= cin; ++tap; }
+    };
+    auto mma = [&](const bf16x8* a, const bf16x8* b) {
 #pragma unroll
         for (int t = 0; t < 2; t++)
 #pragma unroll
             for (int n = 0; n < NT; n++) acc[t][n] = mfma(a[t], b[n], acc[t][n]);
-        ci += 32;
-        while (ci >= cin) { ci -= cin; ++tap; }
+    };
+    bf16x8 a0[2], b0[NT], a1[2], b1[NT];
+    fetch(0, a0, b0);
+    for (int kc = 0; kc < KC; kc += 2) {
+        if (kc + 1 < KC) fetch(kc + 1, a1, b1);
+        mma(a0, b0);
+        if (kc + 1 < KC) {
+            if (kc + 2 < KC) fetch(kc + 2, a0, b0);
+            mma(a1, b1);
+        }
     }
 
     const bool relu = flags & kRelu, f32out = flags & kOutF32;
@@ -155,22 +168,33 @@ __global__ void __launch_bounds__(256)
 // grid.x = T * splits (tap fastest), grid.y = channel blocks.  Each wave
 // takes every 4th 32-voxel chunk of its split and writes its own partial:
 // part[(split*4 + wave)][tap][co][ci].
+// The GEMM's K index is the voxel, but channels are contiguous in memory:
+// each wave loads its chunk of dy (32 voxels x 16*NM channels) and of the
+// shifted input (32 voxels x 16*NN channels) with 16-byte loads and writes
+// it transposed into its own LDS tile [channel][voxel], from which every
+// MFMA fragment (8 consecutive voxels of one channel) is one ds_read_b128.
+// The next chunk's global loads are issued before the current chunk's MFMAs.
 template <int NM, int NN, int KS>
 __global__ void __launch_bounds__(256)
     k_bwd_w(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
             float* __restrict__ part, Geo g, int cin, int cout, uint32_t chunks_per_split)
 {
     constexpr int T = KS * KS * KS;
+    constexpr int CM = 16 * NM, CN = 16 * NN;       // channels of the two operands
+    constexpr int PITCH = 40;                        // halves per LDS row (32 voxels + pad)
+    __shared__ __attribute__((aligned(16))) uint16_t lds[4][(CM + CN) * PITCH];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, grp = lane >> 4;
     const int tap = (int)(blockIdx.x % T);
     const uint32_t split = blockIdx.x / T;
-    const int mblocks = cout / (16 * NM);
-    const int co0 = (int)(blockIdx.y % mblocks) * NM * 16;
-    const int ci0 = (int)(blockIdx.y / mblocks) * NN * 16;
+    const int mblocks = cout / CM;
+    const int co0 = (int)(blockIdx.y % mblocks) * CM;
+    const int ci0 = (int)(blockIdx.y / mblocks) * CN;
     int dz, dyo, dx;
     tap_offset<KS>(tap, dz, dyo, dx);
     const int off = (dz * g.H + dyo) * g.W + dx;
+    uint16_t* la = lds[wave];
+    uint16_t* lb = lds[wave] + CM * PITCH;
 
     f32x4 acc[NM][NN];
 #pragma unroll
@@ -178,46 +202,96 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int n = 0; n < NN; n++) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    const uint32_t n_chunks = (g.V + 31u) / 32u;
-    const uint32_t c0 = split * chunks_per_split;
-    const uint32_t c1 = min(n_chunks, c0 + chunks_per_split);
-    for (uint32_t c = c0 + (uint32_t)wave; c < c1; c += 4u) {
-        const uint32_t v0 = c * 32u + 8u * grp;
-        uint32_t vv[8], nbv[8];
-        bool vok[8], nok[8];
+    // element e = lane + 64 k of a chunk: voxel e / (C/8), channel group e % (C/8);
+    // 32 voxels x C/8 groups = NM (NN) loads of 16 bytes per lane
+    constexpr int GA = CM / 8, GB = CN / 8;
+    constexpr int NA = (32 * GA + 63) / 64, NB = (32 * GB + 63) / 64;
+    auto fetch = [&](uint32_t c, uint4* va, uint4* vb) {
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const uint32_t v = v0 + (uint32_t)e;
+        for (int k = 0; k < NA; k++) {
+            const int e = lane + 64 * k;
+            const uint32_t v = c * 32u + (uint32_t)(e / GA);
+            const bool ok = e < 32 * GA && v < g.V;
+            va[k] = ok ? *reinterpret_cast<const uint4*>(dy + (size_t)v * cout + co0 + 8 * (e % GA))
+                       : uint4{0u, 0u, 0u, 0u};
+        }
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            const int e = lane + 64 * k;
+            const uint32_t v = c * 32u + (uint32_t)(e / GB);
             const int px = (int)(v & (uint32_t)(g.W - 1));
             const int py = (int)((v >> g.lw) & (uint32_t)(g.H - 1));
             const int pz = (int)((v >> (g.lw + g.lh)) & (uint32_t)(g.D - 1));
-            vok[e] = v < g.V;
-            nok[e] = vok[e] && (unsigned)(px + dx) < (unsigned)g.W &&
-                     (unsigned)(py + dyo) < (unsigned)g.H && (unsigned)(pz + dz) < (unsigned)g.D;
-            vv[e] = v;
-            nbv[e] = (uint32_t)((int64_t)v + off);
+            const bool ok = e < 32 * GB && v < g.V && (unsigned)(px + dx) < (unsigned)g.W &&
+                            (unsigned)(py + dyo) < (unsigned)g.H && (unsigned)(pz + dz) < (unsigned)g.D;
+            vb[k] = ok ? *reinterpret_cast<const uint4*>(x + (size_t)(int64_t)((int64_t)v + off) * cin + ci0 +
+                                                         8 * (e % GB))
+                       : uint4{0u, 0u, 0u, 0u};
         }
+    };
+    auto stage = [&](const uint4* va, const uint4* vb) {
+#pragma unroll
+        for (int k = 0; k < NA; k++) {
+            const int e = lane + 64 * k;
+            if (e < 32 * GA) {
+                const int vl = e / GA, ch = 8 * (e % GA);
+                const uint32_t w4[4] = {va[k].x, va[k].y, va[k].z, va[k].w};
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    la[(ch + j) * PITCH + vl] = (uint16_t)(w4[j >> 1] >> (16 * (j & 1)));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+            const int e = lane + 64 * k;
+            if (e < 32 * GB) {
+                const int vl = e / GB, ch = 8 * (e % GB);
+                const uint32_t w4[4] = {vb[k].x, vb[k].y, vb[k].z, vb[k].w};
+#pragma unroll
+                for (int j = 0; j < 8; j++)
+                    lb[(ch + j) * PITCH + vl] = (uint16_t)(w4[j >> 1] >> (16 * (j & 1)));
+            }
+        }
+    };
+    // 16-byte fragment from the LDS tile (byte copy: the tile is written as
+    // 16-bit values, so the read must not be reordered by type-based aliasing)
+    auto frag = [&](const uint16_t* row) {
+        uint4 t;
+        __builtin_memcpy(&t, __builtin_assume_aligned(row, 16), 16);
+        return __builtin_bit_cast(bf16x8, t);
+    };
+    auto mma = [&]() {
         bf16x8 a[NM], b[NN];
 #pragma unroll
-        for (int m = 0; m < NM; m++) {
-            u16x8 t;
+        for (int m = 0; m < NM; m++) a[m] = frag(la + (m * 16 + r16) * PITCH + 8 * grp);
 #pragma unroll
-            for (int e = 0; e < 8; e++)
-                t[e] = vok[e] ? dy[(size_t)vv[e] * cout + co0 + m * 16 + r16] : (unsigned short)0;
-            a[m] = __builtin_bit_cast(bf16x8, t);
-        }
-#pragma unroll
-        for (int n = 0; n < NN; n++) {
-            u16x8 t;
-#pragma unroll
-            for (int e = 0; e < 8; e++)
-                t[e] = nok[e] ? x[(size_t)nbv[e] * cin + ci0 + n * 16 + r16] : (unsigned short)0;
-            b[n] = __builtin_bit_cast(bf16x8, t);
-        }
+        for (int n = 0; n < NN; n++) b[n] = frag(lb + (n * 16 + r16) * PITCH + 8 * grp);
 #pragma unroll
         for (int m = 0; m < NM; m++)
 #pragma unroll
             for (int n = 0; n < NN; n++) acc[m][n] = mfma(a[m], b[n], acc[m][n]);
+    };
+
+    const uint32_t n_chunks = (g.V + 31u) / 32u;
+    const uint32_t c0 = split * chunks_per_split;
+    const uint32_t c1 = min(n_chunks, c0 + chunks_per_split);
+    uint4 va[NA], vb[NB], wa[NA], wb[NB];
+    uint32_t c = c0 + (uint32_t)wave;
+    if (c < c1) fetch(c, va, vb);
+    for (; c < c1; c += 8u) {
+        const uint32_t cn = c + 4u;
+        if (cn < c1) fetch(cn, wa, wb);
+        stage(va, vb);
+        __builtin_amdgcn_wave_barrier();
+        mma();
+        __builtin_amdgcn_wave_barrier();
+        if (cn < c1) {
+            if (cn + 4u < c1) fetch(cn + 4u, va, vb);
+            stage(wa, wb);
+            __builtin_amdgcn_wave_barrier();
+            mma();
+            __builtin_amdgcn_wave_barrier();
+        }
     }
 
     float* out = part + ((size_t)(split * 4u + (uint32_t)wave) * T + tap) * (size_t)cout * cin;
@@ -269,14 +343,21 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// out[c] = sum_k part[k][c]: one workgroup per channel, fixed-order tree
 __global__ void __launch_bounds__(256)
     k_reduce_cols(const float* __restrict__ part, int P, int C, float* __restrict__ out)
 {
-    const int c = (int)(blockIdx.x * 256 + threadIdx.x);
-    if (c >= C) return;
+    __shared__ float lds[256];
+    const int c = (int)blockIdx.x;
     float s = 0.f;
-    for (int k = 0; k < P; k++) s += part[(size_t)k * C + c];
-    out[c] = s;
+    for (int k = threadIdx.x; k < P; k += 256) s += part[(size_t)k * C + c];
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) lds[threadIdx.x] += lds[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = lds[0];
 }
 
 __global__ void __launch_bounds__(256)
@@ -314,11 +395,12 @@ int by_ks(int ks, F&& f)
     return set_error(NX_ERR_INVALID, "conv3d: kernel size must be 1 or 3");
 }
 
-// largest of 8, 4, 2, 1 dividing c16 = channels/16, at most `cap`
+// largest supported tile count (8, 6, 4, 3, 2, 1) dividing c16 = channels/16,
+// at most `cap`
 int tiles_of(int c16, int cap)
 {
-    for (int t = cap; t > 1; t >>= 1)
-        if (c16 % t == 0) return t;
+    for (int t : {8, 6, 4, 3, 2})
+        if (t <= cap && c16 % t == 0) return t;
     return 1;
 }
 
@@ -332,7 +414,9 @@ int launch_fwd(const Shape& s, const uint16_t* x, const uint16_t* w, const float
     return by_ks(s.ks, [&](auto KS) {
         switch (nt) {
             case 8: hipLaunchKernelGGL((k_fwd<8, KS>), grid, dim3(256), 0, st, x, w, bias, y, g, s.cin, s.cout, flags); break;
+            case 6: hipLaunchKernelGGL((k_fwd<6, KS>), grid, dim3(256), 0, st, x, w, bias, y, g, s.cin, s.cout, flags); break;
             case 4: hipLaunchKernelGGL((k_fwd<4, KS>), grid, dim3(256), 0, st, x, w, bias, y, g, s.cin, s.cout, flags); break;
+            case 3: hipLaunchKernelGGL((k_fwd<3, KS>), grid, dim3(256), 0, st, x, w, bias, y, g, s.cin, s.cout, flags); break;
             case 2: hipLaunchKernelGGL((k_fwd<2, KS>), grid, dim3(256), 0, st, x, w, bias, y, g, s.cin, s.cout, flags); break;
             default: hipLaunchKernelGGL((k_fwd<1, KS>), grid, dim3(256), 0, st, x, w, bias, y, g, s.cin, s.cout, flags); break;
         }
@@ -363,7 +447,7 @@ WPlan plan_w(const Shape& s)
     return p;
 }
 
-constexpr uint32_t kColBlocks = 64;
+constexpr uint32_t kColBlocks = 512;
 
 }  // namespace
 
@@ -430,12 +514,14 @@ int bwd_weight(const Shape& s, const uint16_t* x, const uint16_t* dy, float* dw,
         auto by_nn = [&](auto NM) {
             switch (p.nn) {
                 case 4: go(NM, std::integral_constant<int, 4>{}); break;
+                case 3: go(NM, std::integral_constant<int, 3>{}); break;
                 case 2: go(NM, std::integral_constant<int, 2>{}); break;
                 default: go(NM, std::integral_constant<int, 1>{}); break;
             }
         };
         switch (p.nm) {
             case 4: by_nn(std::integral_constant<int, 4>{}); break;
+            case 3: by_nn(std::integral_constant<int, 3>{}); break;
             case 2: by_nn(std::integral_constant<int, 2>{}); break;
             default: by_nn(std::integral_constant<int, 1>{}); break;
         }
@@ -451,8 +537,8 @@ int bwd_weight(const Shape& s, const uint16_t* x, const uint16_t* dy, float* dw,
         const uint32_t per = (g.V + kColBlocks - 1) / kColBlocks;
         hipLaunchKernelGGL(k_colsum, dim3(kColBlocks), dim3(256), 0, st, dy, g.V, s.cout, per, part);
         NX_CHECK_LAUNCH("k_colsum");
-        hipLaunchKernelGGL(k_reduce_cols, dim3((s.cout + 255) / 256), dim3(256), 0, st, part,
-                           (int)kColBlocks, s.cout, db);
+        hipLaunchKernelGGL(k_reduce_cols, dim3(s.cout), dim3(256), 0, st, part, (int)kColBlocks,
+                           s.cout, db);
         NX_CHECK_LAUNCH("k_reduce_cols");
     }
     return NX_OK;

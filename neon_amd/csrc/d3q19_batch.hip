@@ -101,23 +101,43 @@ __device__ __forceinline__ void step_cell(const typename Stor<S>::T* __restrict_
     dense::pin(fl);
     if (i >= g.n) return;
     const double g0[3] = {0, 0, 0};
-    if (cls_of(fl) == NX_CELL_BULK) {
-        P f[Q];
-        if (fl & (NX_MASK_BITS | NX_FLAG_OPEN)) {
-            dense::pull<S, false>(fin, g, i, fl, f);
-        } else {
+    // Every lane runs one collision: in a cavity of 32..96 cells per row each
+    // wavefront holds wall and wall-adjacent lanes, so a separate boundary
+    // path would run (and wait) in every wavefront.  One batch of predicated
+    // loads fetches what the speculative loads lack:
+    //   masked link q of a bulk cell:  a[q] = own f_opp(q), w[q] = f_opp(q) of
+    //                                  the wall cell at x - c_q
+    //   wall cell:                     a[q] = own f_opp(q) for every q (copied)
+    // Only a masked link without upstream cell (NX_FLAG_OPEN) takes the
+    // general pull.
+    const bool bulk = cls_of(fl) == NX_CELL_BULK;
+    const uint32_t m = bulk ? (fl & NX_MASK_BITS) : 0u;
+    const bool open = bulk && (fl & NX_FLAG_OPEN);
+    T a[Q], w[Q];
 #pragma unroll
-            for (int q = 0; q < Q; q++) f[q] = Stor<S>::bulk(v[q], q);
-        }
-        R out[Q];
-        collide<S, (S == NX_F64 ? NX_COMPUTE_F64 : NX_COMPUTE_F32), false>(f, M.omega, g0, out);
-#pragma unroll
-        for (int q = 0; q < Q; q++) v[q] = Stor<S>::template enc<R>(out[q], q);
-    } else {
-        // wall cells: both arenas hold the same values (the batch fills both)
-#pragma unroll
-        for (int q = 0; q < Q; q++) v[q] = fin[q * g.plane + i];
+    for (int q = 0; q < Q; q++) {
+        if (q == 9) continue;
+        const bool mq = (m >> q) & 1u;
+        const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
+        const uint64_t po = (uint64_t)OPP[q] * g.plane;
+        a[q] = (mq || !bulk) ? fin[po + i] : T(0);
+        w[q] = (mq && !open) ? fin[po + min(i - (uint32_t)off, g.n - 1u)] : T(0);
     }
+    P f[Q];
+    if (open) {
+        dense::pull<S, false>(fin, g, i, fl, f);
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            f[q] = (q != 9 && ((m >> q) & 1u)) ? Stor<S>::bulk(a[q], OPP[q]) + Stor<S>::raw(w[q])
+                                               : Stor<S>::bulk(v[q], q);
+    }
+    R out[Q];
+    collide<S, (S == NX_F64 ? NX_COMPUTE_F64 : NX_COMPUTE_F32), false>(f, M.omega, g0, out);
+    // wall cells keep their values (the batch fills both arenas alike)
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        v[q] = bulk ? Stor<S>::template enc<R>(out[q], q) : (q == 9 ? v[9] : a[OPP[q]]);
 #pragma unroll
     for (int q = 0; q < Q; q++) __builtin_nontemporal_store(v[q], &fout[q * g.plane + i]);
 }

@@ -297,13 +297,86 @@ __global__ void __launch_bounds__(WG)
     }
 }
 
-// Launch configuration of k_fused per (storage, compute), process-wide.
-// Defaults: the fastest configuration measured at 256^3 on MI355X.
+// Single-collision variant: every lane loads its flag word with the 19
+// speculative loads; a bulk lane with masked links then fetches, in one
+// batch of predicated loads, its own opposite population and the wall cell's
+// opposite population of each masked link, a wall lane its own values; one
+// collision serves every lane (a separate boundary path would run, and wait,
+// in every wavefront that holds a row end).  Lanes whose masked links lack
+// an upstream cell (NX_FLAG_OPEN) or wrap a periodic face take the general
+// pull.
+template <int S, int CM, bool FORCE, bool PER, int WG, int LP>
+__global__ void __launch_bounds__(WG)
+    k_fused_fix(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
+                const uint32_t* __restrict__ flags, DGeom g, Par p, int remap, int copy_walls)
+{
+    using T = typename Stor<S>::T;
+    using P = typename Stor<S>::P;
+    using R = typename Real<S, CM>::R;
+    const uint32_t wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t i = wg * (uint32_t)WG + threadIdx.x;
+    const uint32_t ic = min(i, g.n - 1u);
+    uint32_t fl = flags[ic];
+    T v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
+        const uint32_t j = q == 9 ? ic : min(ic - (uint32_t)off, g.n - 1u);
+        v[q] = ld_plane<LP>(fin + q * g.plane, j);
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) pin(v[q]);
+    pin(fl);
+    if (i >= g.n) return;
+    const bool bulk = cls_of(fl) == NX_CELL_BULK;
+    const uint32_t m = bulk ? (fl & NX_MASK_BITS) : 0u;
+    bool general = bulk && (fl & NX_FLAG_OPEN);
+    if (PER && bulk && !general) {
+        int x, y, z;
+        coords(g, i, x, y, z);
+        general = ((g.periodic & NX_PERIODIC_X) && (x == 0 || x == g.X - 1)) ||
+                  ((g.periodic & NX_PERIODIC_Y) && (y == 0 || y == g.Y - 1)) ||
+                  ((g.periodic & NX_PERIODIC_Z) && (z == 0 || z == g.Z - 1));
+    }
+    T a[Q], w[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        if (q == 9) continue;
+        const bool mq = (m >> q) & 1u;
+        const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
+        const uint64_t po = (uint64_t)OPP[q] * g.plane;
+        a[q] = ((mq && !general) || (!bulk && copy_walls)) ? fin[po + i] : T(0);
+        w[q] = (mq && !general) ? fin[po + min(i - (uint32_t)off, g.n - 1u)] : T(0);
+    }
+    P f[Q];
+    if (general) {
+        pull<S, PER>(fin, g, i, fl, f);
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            f[q] = (q != 9 && ((m >> q) & 1u)) ? Stor<S>::bulk(a[q], OPP[q]) + Stor<S>::raw(w[q])
+                                               : Stor<S>::bulk(v[q], q);
+    }
+    R out[Q];
+    collide<S, CM, FORCE>(f, p.omega, p.g, out);
+    if (!bulk && !copy_walls) return;
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const T o = bulk ? Stor<S>::template enc<R>(out[q], q) : (q == 9 ? v[9] : a[OPP[q]]);
+        st_plane(fout + q * g.plane, i, o);
+    }
+}
+
+// Launch configuration of the container step per (storage, compute),
+// process-wide.  Defaults: the fastest configuration measured on the 256^3
+// cavity on MI355X -- the single-collision kernel for 8- and 4-byte storage
+// (FP32 35.2k vs 32.9k MLUPS, FP64 17.2k), the bitmap kernel for binary16
+// (49.1k vs 48.3k).
 nx_tuning g_tuning[4][3] = {
-    {{256, 1, 0, 0, 1}, {256, 1, 0, 0, 1}, {256, 1, 0, 0, 1}},
-    {{256, 1, 0, 0, 1}, {256, 1, 0, 0, 1}, {256, 1, 0, 0, 1}},
-    {{256, 1, 0, 0, 1}, {256, 1, 0, 0, 1}, {256, 1, 0, 0, 1}},
-    {{256, 1, 0, 0, 1}, {256, 1, 0, 0, 1}, {256, 1, 0, 0, 1}},
+    {{256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}},
+    {{256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}},
+    {{256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}},
+    {{256, 1, 0, 0, 1, 0}, {256, 1, 0, 0, 1, 0}, {256, 1, 0, 0, 1, 0}},
 };
 
 template <int S, int CM>
@@ -321,7 +394,8 @@ bool valid_tuning(const nx_tuning& t, int storage)
 {
     return (t.wg == 256 || t.wg == 512) && (t.cpt == 1 || t.cpt == 2) &&
            (t.load_policy == 0 || t.load_policy == 1) && (t.xcd_remap == 0 || t.xcd_remap == 1) &&
-           (t.vx == 1 || t.vx == vx_of(storage));
+           (t.vx == 1 || t.vx == vx_of(storage)) && (t.boundary == 0 || t.boundary == 1) &&
+           !(t.boundary == 1 && t.vx != 1);
 }
 
 template <int S, int CM, bool FORCE, bool PER>
@@ -336,6 +410,26 @@ void launch_fused(const nx_tuning& t, const nx_container_s& c, hipStream_t s)
                            (const T*)c.fin, (T*)c.fout, c.flags, c.interior, c.g, c.p,
                            t.xcd_remap, c.copy_walls);
     };
+    if (t.boundary == 1) {
+        auto go_fix = [&](auto wgc, auto lpc) {
+            constexpr int WG = decltype(wgc)::value, LP = decltype(lpc)::value;
+            const uint32_t nb = (c.g.n + WG - 1) / WG;
+            hipLaunchKernelGGL((k_fused_fix<S, CM, FORCE, PER, WG, LP>), dim3(nb), dim3(WG), 0, s,
+                               (const T*)c.fin, (T*)c.fout, c.flags, c.g, c.p, t.xcd_remap,
+                               c.copy_walls);
+        };
+        if constexpr (tunable<S, CM>() && !FORCE && !PER) {
+            auto by_lp = [&](auto wgc) {
+                if (t.load_policy) go_fix(wgc, IC<1>{});
+                else go_fix(wgc, IC<0>{});
+            };
+            if (t.wg == 512) by_lp(IC<512>{});
+            else by_lp(IC<256>{});
+        } else {
+            go_fix(IC<256>{}, IC<0>{});
+        }
+        return;
+    }
     constexpr int VX = vx_of(S);
     if constexpr (VX > 1) {
     if (t.vx == VX && c.g.X % VX == 0) {
@@ -514,7 +608,8 @@ extern "C" int nx_d3q19_set_tuning_dense(const nx_dense_desc* d, const nx_tuning
     if (!t || !valid_tuning(*t, d->storage))
         return set_error(NX_ERR_INVALID,
                          "tuning: wg in {256,512}, cpt in {1,2}, load_policy and xcd_remap in "
-                         "{0,1}, vx = 1 or 16 bytes of cells (2 f64, 4 f32, 8 f16)");
+                         "{0,1}, vx = 1 or 16 bytes of cells (2 f64, 4 f32), boundary in {0,1} "
+                         "(1 needs vx = 1)");
     g_tuning[d->storage][d->compute] = *t;
     return NX_OK;
 }
@@ -531,15 +626,16 @@ extern "C" int nx_container_autotune(nx_container c, nx_stream s, nx_tuning* bes
     nx_tuning bt = slot;
     float bms = 1e30f;
     const int reps = 10;
-    const int nvx = vx_of(c->d.storage) > 1 ? 2 : 1;
-    for (int kv = 0; kv < nvx; kv++)
+    for (int kv = 0; kv < 3; kv++)
+    if (kv != 1 || vx_of(c->d.storage) > 1)
     for (int wg : {256, 512})
         for (int cpt : {1, 2})
             for (int lp : {0, 1})
                 for (int rm : {0, 1}) {
-                    const int vx = kv ? vx_of(c->d.storage) : 1;
-                    if (vx > 1 && cpt == 2) continue;  // the vector variant has one lane of VX cells
-                    const nx_tuning t{wg, cpt, lp, rm, vx};
+                    const int vx = kv == 1 ? vx_of(c->d.storage) : 1;
+                    const int bd = kv == 2 ? 1 : 0;
+                    if ((vx > 1 || bd) && cpt == 2) continue;  // one cell (or lane of VX cells) per thread
+                    const nx_tuning t{wg, cpt, lp, rm, vx, bd};
                     for (int r = 0; r < 3; r++)
                         if (int e = run_container(*c, t, (hipStream_t)s)) return e;
                     NX_CHECK_HIP(hipEventRecord(e0, (hipStream_t)s));

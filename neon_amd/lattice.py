@@ -291,8 +291,8 @@ class DenseLattice(_Lattice):
         check(lib.nx_d3q19_get_tuning_dense(C.byref(self.desc), C.byref(t)))
         return t.as_dict()
 
-    def set_tuning(self, wg=256, cpt=1, load_policy=0, xcd_remap=0, vx=1) -> None:
-        t = abi.Tuning(int(wg), int(cpt), int(load_policy), int(xcd_remap), int(vx))
+    def set_tuning(self, wg=256, cpt=1, load_policy=0, xcd_remap=0, vx=1, boundary=0) -> None:
+        t = abi.Tuning(int(wg), int(cpt), int(load_policy), int(xcd_remap), int(vx), int(boundary))
         check(lib.nx_d3q19_set_tuning_dense(C.byref(self.desc), C.byref(t)))
 
     # -- containers (LbmIterationD3Q19: one per ping-pong direction) ---------

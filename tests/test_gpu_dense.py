@@ -159,8 +159,9 @@ def test_fp16_dev_velocity_error_vs_f64(nx):
     assert errs["f32"] < 1e-4 and errs["f32dev"] < 1e-4 and errs["f16dev"] < 1e-3
 
 
+@pytest.mark.parametrize("boundary", [0, 1])
 @pytest.mark.parametrize("st,cm", [("f64", "f64"), ("f32", "f32"), ("f32", "ref"), ("f16dev", "f32")])
-def test_container_equals_plain_step(nx, orc, st, cm):
+def test_container_equals_plain_step(nx, orc, st, cm, boundary):
     """The container (interior bitmap + boundary list, two kernels) and the
     single-kernel nx_d3q19_step_dense give identical populations, on a grid
     with periodic y, open x/z faces, walls and a body force."""
@@ -175,7 +176,12 @@ def test_container_equals_plain_step(nx, orc, st, cm):
     for lat in (a, b):
         lat.set_classes(cls.reshape(-1), ulb=0.04)
     force = (0.0, 0.0, 0.0) if cm == "ref" else (2e-5, 1e-5, 0.0)
-    a.run(7, 1.5, force=force)
+    t0 = a.tuning()
+    try:
+        a.set_tuning(**{**t0, "boundary": boundary})
+        a.run(7, 1.5, force=force)
+    finally:
+        a.set_tuning(**t0)
     b.run_plain(7, 1.5, force=force)
     assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
     n_int, n_bd, walls = a.container_info(1.5, force)
@@ -232,11 +238,12 @@ def test_container_vector_lanes_equal_plain_step(nx, st, cm, vx, wg, lp):
     for lat in (a, b):
         lat.set_classes(cls.reshape(-1), ulb=0.04)
     force = (0.0, 0.0, 0.0) if cm == "ref" else (2e-5, 1e-5, 0.0)
+    t0 = a.tuning()
     try:
-        a.set_tuning(wg=wg, cpt=1, load_policy=lp, xcd_remap=1, vx=vx)
+        a.set_tuning(wg=wg, cpt=1, load_policy=lp, xcd_remap=1, vx=vx, boundary=0)
         a.run(9, 1.6, force=force)
     finally:
-        a.set_tuning()
+        a.set_tuning(**t0)
     b.run_plain(9, 1.6, force=force)
     assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
 
@@ -246,11 +253,12 @@ def test_container_vector_lanes_cavity(nx, orc):
     g, o = _pair(nx, orc, (N, N, N), "f32", "f32", 1, 1)
     g.setup_cavity(0.05)
     o.setup_cavity(0.05)
+    t0 = g.tuning()
     try:
-        g.set_tuning(vx=4)
+        g.set_tuning(vx=4, boundary=0)
         g.run(12, 1.7)
     finally:
-        g.set_tuning()
+        g.set_tuning(**t0)
     o.run(12, 1.7)
     assert np.array_equal(g.populations(), o.fin)
 
@@ -259,3 +267,46 @@ def test_vector_lanes_rejected_for_half(nx):
     g = nx.DenseLattice((16, 16, 16), "f16dev", "f32")
     with pytest.raises(nx.abi.NeonError):
         g.set_tuning(vx=4)
+
+
+@pytest.mark.parametrize("st,cm", [("f64", "f64"), ("f32", "f32"), ("f32", "ref"), ("f32dev", "f32"),
+                                   ("f16dev", "f32")])
+@pytest.mark.parametrize("wg", [256, 512])
+def test_container_single_collision_equals_plain_step(nx, st, cm, wg):
+    """boundary = 1 (flags read by every lane, masked links fixed up, one
+    collision): bit for bit the single-kernel step, with open faces (masked
+    links without upstream cell), periodic y, obstacles and a body force."""
+    dim = (37, 12, 21)
+    X, Y, Z = dim
+    rng = np.random.default_rng(13)
+    cls = np.full((Z, Y, X), 2, np.uint32)
+    cls[rng.random((Z, Y, X)) < 0.05] = 0
+    cls[:, :, 0] = 1
+    a = nx.DenseLattice(dim, st, cm, periodic=2)
+    b = nx.DenseLattice(dim, st, cm, periodic=2)
+    for lat in (a, b):
+        lat.set_classes(cls.reshape(-1), ulb=0.04)
+    force = (0.0, 0.0, 0.0) if cm == "ref" else (2e-5, 1e-5, 0.0)
+    t0 = a.tuning()
+    try:
+        a.set_tuning(wg=wg, boundary=1)
+        a.run(7, 1.5, force=force)
+    finally:
+        a.set_tuning(**t0)
+    b.run_plain(7, 1.5, force=force)
+    assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
+
+
+def test_container_single_collision_cavity(nx, orc):
+    N = 24
+    g, o = _pair(nx, orc, (N, N, N), "f32", "f32", 1, 1)
+    g.setup_cavity(0.05)
+    o.setup_cavity(0.05)
+    t0 = g.tuning()
+    try:
+        g.set_tuning(boundary=1)
+        g.run(10, 1.7)
+    finally:
+        g.set_tuning(**t0)
+    o.run(10, 1.7)
+    assert np.array_equal(g.populations(), o.fin)
