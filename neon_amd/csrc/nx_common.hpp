@@ -114,7 +114,12 @@ struct Stor<NX_F16DEV> {
     template <typename R>
     static __device__ __forceinline__ T enc(R v, int)
     {
-        return (_Float16)(float)v;
+        // round to binary32 first, then to binary16: the opaque value keeps the
+        // compiler from fusing the producing FMA and this conversion into one
+        // v_fma_mixlo_f16 (a single rounding, different results)
+        float f = (float)v;
+        asm volatile("" : "+v"(f));
+        return (_Float16)f;
     }
     static __device__ __forceinline__ T enc_full(double f, int q)
     {
@@ -221,8 +226,17 @@ __device__ __forceinline__ void bgk_forced(const R f[Q], R omega, R g0, R g1, R 
 }
 
 // Deviation form (project-defined): d = f - w_q.  rho = 1 + sum d,
-// u = sum c d / rho, and feq - w_q = w_q (rho - 1) + w_q rho (3 c.u
-// + 4.5 (c.u)^2 - 1.5 u^2): the same BGK as bgk<R>, every term small.
+// u = sum c d * (1 / rho), and feq - w_q = w_q (rho - 1) + w_q rho (3 c.u
+// + 4.5 (c.u)^2 - 1.5 u^2): the same BGK as bgk<R>, every term small.  The
+// expression is written with one reciprocal and explicit fused multiply-adds
+// (the oracle restates the same operations with fmaf):
+//   poly = fma(c, fma(4.5, c, -3), -usqr)  (for c = c_q.u with the reference's
+//          sign convention of the first nine directions), deq = fma(rho w,
+//          poly, w drho), deq_opp = fma(6 rho w, c, deq),
+//   d' = fma(omega, deq - d, d).
+__device__ __forceinline__ float fmaR(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaR(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename R>
 __device__ __forceinline__ void macroscopic_dev(const R d[Q], R& drho, R& rho, R& u0, R& u1, R& u2)
 {
@@ -235,9 +249,10 @@ __device__ __forceinline__ void macroscopic_dev(const R d[Q], R& drho, R& rho, R
     const R ZP1 = d[6] + d[8] + d[12] + d[15] + d[17];
     drho = XM1 + XP1 + X0;
     rho = R(1.) + drho;
-    u0 = (XP1 - XM1) / rho;
-    u1 = (YP1 - YM1) / rho;
-    u2 = (ZP1 - ZM1) / rho;
+    const R inv = R(1.) / rho;
+    u0 = (XP1 - XM1) * inv;
+    u1 = (YP1 - YM1) * inv;
+    u2 = (ZP1 - ZM1) * inv;
 }
 
 template <typename R>
@@ -247,17 +262,21 @@ __device__ __forceinline__ void bgk_dev(const R d[Q], R omega, R out[Q])
     macroscopic_dev<R>(d, drho, rho, u0, u1, u2);
     const R usqr = R(1.5) * (u0 * u0 + u1 * u1 + u2 * u2);
     const R ck[9] = {u0, u1, u2, u0 + u1, u0 - u1, u0 + u2, u0 - u2, u1 + u2, u1 - u2};
+    const R w18 = R(1. / 18.), w36 = R(1. / 36.), w3 = R(1. / 3.);
+    const R dr18 = w18 * drho, dr36 = w36 * drho;
+    const R rw18 = rho * w18, rw36 = rho * w36;
+    const R rw18x6 = rw18 * R(6.), rw36x6 = rw36 * R(6.);
 #pragma unroll
     for (int i = 0; i < 9; i++) {
-        const R wi = i < 3 ? R(1. / 18.) : R(1. / 36.);
-        const R rw = rho * wi;
-        const R deq = wi * drho + rw * (R(4.5) * ck[i] * ck[i] - R(3.) * ck[i] - usqr);
-        const R deqopp = deq + rw * R(6.) * ck[i];
-        out[i] = (R(1.) - omega) * d[i] + omega * deq;
-        out[i + 10] = (R(1.) - omega) * d[i + 10] + omega * deqopp;
+        const R c = ck[i];
+        const R poly = fmaR(c, fmaR(R(4.5), c, R(-3.)), -usqr);
+        const R deq = fmaR(i < 3 ? rw18 : rw36, poly, i < 3 ? dr18 : dr36);
+        const R deqopp = fmaR(i < 3 ? rw18x6 : rw36x6, c, deq);
+        out[i] = fmaR(omega, deq - d[i], d[i]);
+        out[i + 10] = fmaR(omega, deqopp - d[i + 10], d[i + 10]);
     }
-    const R deq9 = R(1. / 3.) * drho - rho * R(1. / 3.) * usqr;
-    out[9] = (R(1.) - omega) * d[9] + omega * deq9;
+    const R deq9 = fmaR(-(rho * w3), usqr, w3 * drho);
+    out[9] = fmaR(omega, deq9 - d[9], d[9]);
 }
 
 template <typename R>
@@ -276,9 +295,10 @@ __device__ __forceinline__ void bgk_forced_dev(const R d[Q], R omega, R g0, R g1
         const R w = R(wq(q));
         const R cu = R(CX[q]) * u0 + R(CY[q]) * u1 + R(CZ[q]) * u2;
         const R cg = R(CX[q]) * g0 + R(CY[q]) * g1 + R(CZ[q]) * g2;
-        const R deq = w * drho + rho * w * (R(3) * cu + R(4.5) * cu * cu - usqr);
+        const R poly = fmaR(cu, fmaR(R(4.5), cu, R(3.)), -usqr);
+        const R deq = fmaR(rho * w, poly, w * drho);
         const R src = fpre * w * rho * (R(3) * (cg - ug) + R(9) * cu * cg);
-        out[q] = (R(1) - omega) * d[q] + omega * deq + src;
+        out[q] = fmaR(omega, deq - d[q], d[q]) + src;
     }
 }
 

@@ -342,6 +342,66 @@ def forward_flops(net: UNet) -> float:
     return f
 
 
+def conv_layers(levels: int, widths, out_ch_padded: int = 16):
+    """(cin, cout, ksize, level) of every convolution of the U-Net, in the
+    parameter order of nx_unet_*."""
+    L, wd = int(levels), [int(v) for v in widths]
+    convs = []
+    for l in range(L):
+        convs.append((16 if l == 0 else wd[l - 1], wd[l], 3, l))
+        convs.append((wd[l], wd[l], 3, l))
+    for l in range(L - 1):
+        convs.append((wd[l + 1] + wd[l], wd[l], 3, l))
+        convs.append((wd[l], wd[l], 3, l))
+    convs.append((wd[0], out_ch_padded, 1, 0))
+    return convs
+
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0   # MI355X dense BF16 matrix-core peak
+
+
+def conv_bench(size: int = 32, batch: int = 16, levels: int = 3, widths=(48, 96, 96), reps: int = 10):
+    """Times nx_conv3d_{fwd,bwd_data,bwd_weight}_bf16 on every layer shape of
+    the network (HIP events on one stream); FLOPs = 2 k^3 cin cout voxels per
+    pass.  Returns a list of per-layer records."""
+    from .lattice import Event, Stream
+    init()
+    st = Stream()
+    rng = np.random.default_rng(0)
+    out = []
+    for ci, co, ks, lvl in conv_layers(levels, widths):
+        side = size >> lvl
+        shape = (batch, side, side, side)
+        V = batch * side ** 3
+        bits = lambda n: (rng.integers(0x3C00, 0x3F80, n, dtype=np.uint16))
+        x = DeviceBuffer.from_array(bits(V * ci))
+        dyb = DeviceBuffer.from_array(bits(V * co))
+        w = DeviceBuffer.from_array(bits(co * ks ** 3 * ci))
+        y = DeviceBuffer(V * co * 2)
+        dx = DeviceBuffer(V * ci * 2)
+        dw = DeviceBuffer(co * ks ** 3 * ci * 4)
+        db = DeviceBuffer(co * 4)
+        flops = 2.0 * ks ** 3 * ci * co * V
+        rec = {"cin": ci, "cout": co, "k": ks, "shape": list(shape), "gflop": round(flops / 1e9, 3)}
+        for name, fn in (("fwd", lambda: conv3d_fwd(x, w, None, y, shape, ci, co, ks, relu=True, stream=st.handle)),
+                         ("bwd_data", lambda: conv3d_bwd_data(dyb, w, dx, shape, ci, co, ks, stream=st.handle)),
+                         ("bwd_weight", lambda: conv3d_bwd_weight(x, dyb, dw, db, shape, ci, co, ks, stream=st.handle))):
+            for _ in range(2):
+                fn()
+            e0, e1 = Event(), Event()
+            e0.record(st)
+            for _ in range(reps):
+                fn()
+            e1.record(st)
+            e1.sync()
+            ms = e0.elapsed_ms(e1) / reps
+            tf = flops / (ms * 1e-3) / 1e12
+            rec[name] = {"us": round(ms * 1e3, 1), "TFLOPs": round(tf, 1),
+                         "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4)}
+        out.append(rec)
+    return out
+
+
 def main(argv=None):
     """Warm-start experiment: generate steady cavities with the batched
     solver, train the U-Net on the device, then iterate held-out cavities to
@@ -367,7 +427,22 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--save-params", default=None)
     ap.add_argument("--mirror", action="store_true", help="add the z-mirrored copies to the training set")
+    ap.add_argument("--conv-bench", action="store_true",
+                    help="only time the BF16 convolutions on every layer shape of the network")
     a = ap.parse_args(argv)
+    if a.conv_bench:
+        widths = tuple(int(v) for v in a.widths.split(","))
+        recs = conv_bench(a.size, a.batch, a.levels, widths)
+        tot = {k: sum(r[k]["us"] for r in recs) for k in ("fwd", "bwd_data", "bwd_weight")}
+        fl = sum(r["gflop"] for r in recs) * 1e9
+        for r in recs:
+            print(json.dumps(r), flush=True)
+        summary = {"conv_bench": True, "size": a.size, "batch": a.batch, "widths": list(widths),
+                   "total_us": tot,
+                   "TFLOPs": {k: round(fl / (v * 1e-6) / 1e12, 1) for k, v in tot.items()},
+                   "peak_TFLOPs": MFMA_BF16_PEAK_TFLOPS}
+        print(json.dumps(summary), flush=True)
+        return summary
     log = lambda m: print(m, flush=True)
     widths = tuple(int(v) for v in a.widths.split(","))
     ms = make_cavities(a.train + a.test, a.size, a.seed, tuple(a.re), tuple(a.boxes), tuple(a.edges))

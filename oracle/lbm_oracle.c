@@ -286,8 +286,9 @@ DEF_FORCED(forced_d, double)
 DEF_FORCED(forced_f, float)
 
 /* Deviation form (project-defined), binary32: d = f - w_q, rho = 1 + sum d,
- * u = sum c d / rho, feq - w_q = w_q (rho-1) + w_q rho (3 c.u + 4.5 (c.u)^2
- * - 1.5 u^2); the expression trees of the device's bgk_dev / bgk_forced_dev. */
+ * u = sum c d * (1 / rho), feq - w_q = w_q (rho-1) + w_q rho (3 c.u +
+ * 4.5 (c.u)^2 - 1.5 u^2), written with one reciprocal and fused
+ * multiply-adds: the operations of the device's bgk_dev / bgk_forced_dev. */
 static inline void macro_dev(const float d[19], float* drho, float* rho, float u[3])
 {
     const float XM1 = d[0] + d[3] + d[4] + d[5] + d[6];
@@ -299,9 +300,10 @@ static inline void macro_dev(const float d[19], float* drho, float* rho, float u
     const float ZP1 = d[6] + d[8] + d[12] + d[15] + d[17];
     *drho = XM1 + XP1 + X0;
     *rho = 1.f + *drho;
-    u[0] = (XP1 - XM1) / *rho;
-    u[1] = (YP1 - YM1) / *rho;
-    u[2] = (ZP1 - ZM1) / *rho;
+    const float inv = 1.f / *rho;
+    u[0] = (XP1 - XM1) * inv;
+    u[1] = (YP1 - YM1) * inv;
+    u[2] = (ZP1 - ZM1) * inv;
 }
 
 static void collide_dev(const float d[19], float omega, float out[19])
@@ -312,16 +314,19 @@ static void collide_dev(const float d[19], float omega, float out[19])
     const float ck[9] = {u[0],        u[1],        u[2],        u[0] + u[1], u[0] - u[1],
                          u[0] + u[2], u[0] - u[2], u[1] + u[2], u[1] - u[2]};
     const float w18 = (float)(1. / 18.), w36 = (float)(1. / 36.), w3 = (float)(1. / 3.);
+    const float dr18 = w18 * drho, dr36 = w36 * drho;
+    const float rw18 = rho * w18, rw36 = rho * w36;
+    const float rw18x6 = rw18 * 6.f, rw36x6 = rw36 * 6.f;
     for (int i = 0; i < 9; i++) {
-        const float wi = i < 3 ? w18 : w36;
-        const float rw = rho * wi;
-        const float deq = wi * drho + rw * (4.5f * ck[i] * ck[i] - 3.f * ck[i] - usqr);
-        const float deqopp = deq + rw * 6.f * ck[i];
-        out[i] = (1.f - omega) * d[i] + omega * deq;
-        out[i + 10] = (1.f - omega) * d[i + 10] + omega * deqopp;
+        const float c = ck[i];
+        const float poly = fmaf(c, fmaf(4.5f, c, -3.f), -usqr);
+        const float deq = fmaf(i < 3 ? rw18 : rw36, poly, i < 3 ? dr18 : dr36);
+        const float deqopp = fmaf(i < 3 ? rw18x6 : rw36x6, c, deq);
+        out[i] = fmaf(omega, deq - d[i], d[i]);
+        out[i + 10] = fmaf(omega, deqopp - d[i + 10], d[i + 10]);
     }
-    const float deq9 = w3 * drho - rho * w3 * usqr;
-    out[9] = (1.f - omega) * d[9] + omega * deq9;
+    const float deq9 = fmaf(-(rho * w3), usqr, w3 * drho);
+    out[9] = fmaf(omega, deq9 - d[9], d[9]);
 }
 
 static void forced_dev(const float d[19], float omega, const float g[3], float out[19])
@@ -338,9 +343,10 @@ static void forced_dev(const float d[19], float omega, const float g[3], float o
         const float wq = (float)W[q];
         const float cu = (float)C[q][0] * u[0] + (float)C[q][1] * u[1] + (float)C[q][2] * u[2];
         const float cg = (float)C[q][0] * g[0] + (float)C[q][1] * g[1] + (float)C[q][2] * g[2];
-        const float deq = wq * drho + rho * wq * (3.f * cu + 4.5f * cu * cu - usqr);
+        const float poly = fmaf(cu, fmaf(4.5f, cu, 3.f), -usqr);
+        const float deq = fmaf(rho * wq, poly, wq * drho);
         const float src = fpre * wq * rho * (3.f * (cg - ug) + 9.f * cu * cg);
-        out[q] = (1.f - omega) * d[q] + omega * deq + src;
+        out[q] = fmaf(omega, deq - d[q], d[q]) + src;
     }
 }
 
