@@ -62,7 +62,7 @@ struct nx_batch_s {
     float* u_prev = nullptr;         // 3 per cell
     // launch configuration
     int wg = 256;
-    int wgs_per_cu = 8;
+    int wgs_per_cu = 2;   // the autotuner's choice for 512 cavities of 32..96 cells on MI355X
     int xcd_chunks = 1;
     int n_cu = 256;
     // captured graph
