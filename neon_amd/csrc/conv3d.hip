@@ -202,56 +202,64 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int n = 0; n < NN; n++) acc[m][n] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    // element e = lane + 64 k of a chunk: voxel e / (C/8), channel group e % (C/8);
-    // 32 voxels x C/8 groups = NM (NN) loads of 16 bytes per lane
+    // element e = lane + 64 k of a chunk: voxel quad e / (C/8) (4 consecutive
+    // voxels), channel group e % (C/8) (8 channels): four 16-byte loads, then
+    // eight 8-byte LDS writes (4 voxels of one channel each)
     constexpr int GA = CM / 8, GB = CN / 8;
-    constexpr int NA = (32 * GA + 63) / 64, NB = (32 * GB + 63) / 64;
-    auto fetch = [&](uint32_t c, uint4* va, uint4* vb) {
+    constexpr int EA = 8 * GA, EB = 8 * GB;
+    constexpr int NA = (EA + 63) / 64, NB = (EB + 63) / 64;
+    auto fetch = [&](uint32_t c, uint4 (*va)[4], uint4 (*vb)[4]) {
 #pragma unroll
         for (int k = 0; k < NA; k++) {
             const int e = lane + 64 * k;
-            const uint32_t v = c * 32u + (uint32_t)(e / GA);
-            const bool ok = e < 32 * GA && v < g.V;
-            va[k] = ok ? *reinterpret_cast<const uint4*>(dy + (size_t)v * cout + co0 + 8 * (e % GA))
-                       : uint4{0u, 0u, 0u, 0u};
+            const uint32_t v0 = c * 32u + 4u * (uint32_t)(e / GA);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const uint32_t v = v0 + (uint32_t)t;
+                const bool ok = e < EA && v < g.V;
+                va[k][t] = ok ? *reinterpret_cast<const uint4*>(dy + (size_t)v * cout + co0 + 8 * (e % GA))
+                              : uint4{0u, 0u, 0u, 0u};
+            }
         }
 #pragma unroll
         for (int k = 0; k < NB; k++) {
             const int e = lane + 64 * k;
-            const uint32_t v = c * 32u + (uint32_t)(e / GB);
-            const int px = (int)(v & (uint32_t)(g.W - 1));
-            const int py = (int)((v >> g.lw) & (uint32_t)(g.H - 1));
-            const int pz = (int)((v >> (g.lw + g.lh)) & (uint32_t)(g.D - 1));
-            const bool ok = e < 32 * GB && v < g.V && (unsigned)(px + dx) < (unsigned)g.W &&
-                            (unsigned)(py + dyo) < (unsigned)g.H && (unsigned)(pz + dz) < (unsigned)g.D;
-            vb[k] = ok ? *reinterpret_cast<const uint4*>(x + (size_t)(int64_t)((int64_t)v + off) * cin + ci0 +
-                                                         8 * (e % GB))
-                       : uint4{0u, 0u, 0u, 0u};
+            const uint32_t v0 = c * 32u + 4u * (uint32_t)(e / GB);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const uint32_t v = v0 + (uint32_t)t;
+                const int px = (int)(v & (uint32_t)(g.W - 1));
+                const int py = (int)((v >> g.lw) & (uint32_t)(g.H - 1));
+                const int pz = (int)((v >> (g.lw + g.lh)) & (uint32_t)(g.D - 1));
+                const bool ok = e < EB && v < g.V && (unsigned)(px + dx) < (unsigned)g.W &&
+                                (unsigned)(py + dyo) < (unsigned)g.H && (unsigned)(pz + dz) < (unsigned)g.D;
+                vb[k][t] = ok ? *reinterpret_cast<const uint4*>(x + (size_t)(int64_t)((int64_t)v + off) * cin +
+                                                                ci0 + 8 * (e % GB))
+                              : uint4{0u, 0u, 0u, 0u};
+            }
         }
     };
-    auto stage = [&](const uint4* va, const uint4* vb) {
+    // channel j of two voxels' 8-channel words -> one 32-bit word (voxel a low)
+    auto pair = [](const uint4& a, const uint4& b, int j) {
+        const uint32_t wa = (j >> 1) == 0 ? a.x : (j >> 1) == 1 ? a.y : (j >> 1) == 2 ? a.z : a.w;
+        const uint32_t wb = (j >> 1) == 0 ? b.x : (j >> 1) == 1 ? b.y : (j >> 1) == 2 ? b.z : b.w;
+        return (j & 1) ? ((wa >> 16) | (wb & 0xffff0000u)) : ((wa & 0xffffu) | (wb << 16));
+    };
+    auto put = [&](uint16_t* tile, int e, int G, const uint4* q4) {
+        const int quad = e / G, ch = 8 * (e % G);
 #pragma unroll
-        for (int k = 0; k < NA; k++) {
-            const int e = lane + 64 * k;
-            if (e < 32 * GA) {
-                const int vl = e / GA, ch = 8 * (e % GA);
-                const uint32_t w4[4] = {va[k].x, va[k].y, va[k].z, va[k].w};
-#pragma unroll
-                for (int j = 0; j < 8; j++)
-                    la[(ch + j) * PITCH + vl] = (uint16_t)(w4[j >> 1] >> (16 * (j & 1)));
-            }
+        for (int j = 0; j < 8; j++) {
+            const uint32_t w2[2] = {pair(q4[0], q4[1], j), pair(q4[2], q4[3], j)};
+            __builtin_memcpy(__builtin_assume_aligned(tile + (ch + j) * PITCH + 4 * quad, 8), w2, 8);
         }
+    };
+    auto stage = [&](uint4 (*va)[4], uint4 (*vb)[4]) {
 #pragma unroll
-        for (int k = 0; k < NB; k++) {
-            const int e = lane + 64 * k;
-            if (e < 32 * GB) {
-                const int vl = e / GB, ch = 8 * (e % GB);
-                const uint32_t w4[4] = {vb[k].x, vb[k].y, vb[k].z, vb[k].w};
+        for (int k = 0; k < NA; k++)
+            if (lane + 64 * k < EA) put(la, lane + 64 * k, GA, va[k]);
 #pragma unroll
-                for (int j = 0; j < 8; j++)
-                    lb[(ch + j) * PITCH + vl] = (uint16_t)(w4[j >> 1] >> (16 * (j & 1)));
-            }
-        }
+        for (int k = 0; k < NB; k++)
+            if (lane + 64 * k < EB) put(lb, lane + 64 * k, GB, vb[k]);
     };
     // 16-byte fragment from the LDS tile (byte copy: the tile is written as
     // 16-bit values, so the read must not be reordered by type-based aliasing)
@@ -275,7 +283,7 @@ __global__ void __launch_bounds__(256)
     const uint32_t n_chunks = (g.V + 31u) / 32u;
     const uint32_t c0 = split * chunks_per_split;
     const uint32_t c1 = min(n_chunks, c0 + chunks_per_split);
-    uint4 va[NA], vb[NB], wa[NA], wb[NB];
+    uint4 va[NA][4], vb[NB][4], wa[NA][4], wb[NB][4];
     uint32_t c = c0 + (uint32_t)wave;
     if (c < c1) fetch(c, va, vb);
     for (; c < c1; c += 8u) {

@@ -508,7 +508,7 @@ def main(argv=None):
     probe = warm_start_overhead(a.check_every)
     out = {
         "experiment": "learned warm start vs rest start, held-out cavities",
-        "size": a.size, "train": int(len(x)), "test": n_test, "re": list(a.re), "boxes": list(a.boxes),
+        "size": a.size, "train_fields": int(len(x)), "test": n_test, "re": list(a.re), "boxes": list(a.boxes),
         "edges": list(a.edges), "tol": a.tol, "check_every": a.check_every, "storage": a.storage,
         "generate": {"seconds": round(t_gen, 2), "unconverged": int(left_g),
                      "mean_steps": float(steps_g.mean()), "max_steps": int(steps_g.max()),
