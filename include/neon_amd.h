@@ -114,7 +114,8 @@ typedef struct {
                          /* cells (2 f64, 4 f32); not binary16            */
     int32_t boundary;    /* 0: interior bitmap, other lanes run the full  */
                          /* pull; 1: every lane reads its flags, masked   */
-                         /* links are fixed up, one collision for all     */
+                         /* links are fixed up, one collision for all;    */
+                         /* 2: as 1, interior lanes read the bitmap only  */
 } nx_tuning;
 
 /* ---- runtime (replaces libNeonSys GpuDevice / Backend, GpuDevice.h:151-190,

@@ -305,10 +305,14 @@ __global__ void __launch_bounds__(WG)
 // in every wavefront that holds a row end).  Lanes whose masked links lack
 // an upstream cell (NX_FLAG_OPEN) or wrap a periodic face take the general
 // pull.
-template <int S, int CM, bool FORCE, bool PER, int WG, int LP>
+// With BITMAP the lane reads the container's interior bit instead of its
+// flag word; only non-interior lanes then load their flags (a dependent
+// load, 4 bytes per cell less traffic).
+template <int S, int CM, bool FORCE, bool PER, int WG, int LP, bool BITMAP>
 __global__ void __launch_bounds__(WG)
     k_fused_fix(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
-                const uint32_t* __restrict__ flags, DGeom g, Par p, int remap, int copy_walls)
+                const uint32_t* __restrict__ flags, const uint32_t* __restrict__ bits, DGeom g, Par p,
+                int remap, int copy_walls)
 {
     using T = typename Stor<S>::T;
     using P = typename Stor<S>::P;
@@ -316,7 +320,7 @@ __global__ void __launch_bounds__(WG)
     const uint32_t wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t i = wg * (uint32_t)WG + threadIdx.x;
     const uint32_t ic = min(i, g.n - 1u);
-    uint32_t fl = flags[ic];
+    uint32_t fl = BITMAP ? bits[ic >> 5] : flags[ic];
     T v[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
@@ -328,6 +332,8 @@ __global__ void __launch_bounds__(WG)
     for (int q = 0; q < Q; q++) pin(v[q]);
     pin(fl);
     if (i >= g.n) return;
+    if constexpr (BITMAP)
+        fl = ((fl >> (i & 31u)) & 1u) ? ((uint32_t)NX_CELL_BULK << NX_CLASS_SHIFT) : flags[i];
     const bool bulk = cls_of(fl) == NX_CELL_BULK;
     const uint32_t m = bulk ? (fl & NX_MASK_BITS) : 0u;
     bool general = bulk && (fl & NX_FLAG_OPEN);
@@ -394,8 +400,8 @@ bool valid_tuning(const nx_tuning& t, int storage)
 {
     return (t.wg == 256 || t.wg == 512) && (t.cpt == 1 || t.cpt == 2) &&
            (t.load_policy == 0 || t.load_policy == 1) && (t.xcd_remap == 0 || t.xcd_remap == 1) &&
-           (t.vx == 1 || t.vx == vx_of(storage)) && (t.boundary == 0 || t.boundary == 1) &&
-           !(t.boundary == 1 && t.vx != 1);
+           (t.vx == 1 || t.vx == vx_of(storage)) && t.boundary >= 0 && t.boundary <= 2 &&
+           !(t.boundary != 0 && t.vx != 1);
 }
 
 template <int S, int CM, bool FORCE, bool PER>
@@ -410,13 +416,18 @@ void launch_fused(const nx_tuning& t, const nx_container_s& c, hipStream_t s)
                            (const T*)c.fin, (T*)c.fout, c.flags, c.interior, c.g, c.p,
                            t.xcd_remap, c.copy_walls);
     };
-    if (t.boundary == 1) {
+    if (t.boundary == 1 || t.boundary == 2) {
         auto go_fix = [&](auto wgc, auto lpc) {
             constexpr int WG = decltype(wgc)::value, LP = decltype(lpc)::value;
             const uint32_t nb = (c.g.n + WG - 1) / WG;
-            hipLaunchKernelGGL((k_fused_fix<S, CM, FORCE, PER, WG, LP>), dim3(nb), dim3(WG), 0, s,
-                               (const T*)c.fin, (T*)c.fout, c.flags, c.g, c.p, t.xcd_remap,
-                               c.copy_walls);
+            if (t.boundary == 2)
+                hipLaunchKernelGGL((k_fused_fix<S, CM, FORCE, PER, WG, LP, true>), dim3(nb), dim3(WG), 0, s,
+                                   (const T*)c.fin, (T*)c.fout, c.flags, c.interior, c.g, c.p,
+                                   t.xcd_remap, c.copy_walls);
+            else
+                hipLaunchKernelGGL((k_fused_fix<S, CM, FORCE, PER, WG, LP, false>), dim3(nb), dim3(WG), 0, s,
+                                   (const T*)c.fin, (T*)c.fout, c.flags, c.interior, c.g, c.p,
+                                   t.xcd_remap, c.copy_walls);
         };
         if constexpr (tunable<S, CM>() && !FORCE && !PER) {
             auto by_lp = [&](auto wgc) {
@@ -608,8 +619,8 @@ extern "C" int nx_d3q19_set_tuning_dense(const nx_dense_desc* d, const nx_tuning
     if (!t || !valid_tuning(*t, d->storage))
         return set_error(NX_ERR_INVALID,
                          "tuning: wg in {256,512}, cpt in {1,2}, load_policy and xcd_remap in "
-                         "{0,1}, vx = 1 or 16 bytes of cells (2 f64, 4 f32), boundary in {0,1} "
-                         "(1 needs vx = 1)");
+                         "{0,1}, vx = 1 or 16 bytes of cells (2 f64, 4 f32), boundary in {0,1,2} "
+                         "(1 and 2 need vx = 1)");
     g_tuning[d->storage][d->compute] = *t;
     return NX_OK;
 }
@@ -626,14 +637,14 @@ extern "C" int nx_container_autotune(nx_container c, nx_stream s, nx_tuning* bes
     nx_tuning bt = slot;
     float bms = 1e30f;
     const int reps = 10;
-    for (int kv = 0; kv < 3; kv++)
+    for (int kv = 0; kv < 4; kv++)
     if (kv != 1 || vx_of(c->d.storage) > 1)
     for (int wg : {256, 512})
         for (int cpt : {1, 2})
             for (int lp : {0, 1})
                 for (int rm : {0, 1}) {
                     const int vx = kv == 1 ? vx_of(c->d.storage) : 1;
-                    const int bd = kv == 2 ? 1 : 0;
+                    const int bd = kv >= 2 ? kv - 1 : 0;
                     if ((vx > 1 || bd) && cpt == 2) continue;  // one cell (or lane of VX cells) per thread
                     const nx_tuning t{wg, cpt, lp, rm, vx, bd};
                     for (int r = 0; r < 3; r++)

@@ -116,8 +116,10 @@ int nx_device_name(char* buf, int len)
     NX_CHECK_HIP(hipGetDevice(&dev));
     hipDeviceProp_t p;
     NX_CHECK_HIP(hipGetDeviceProperties(&p, dev));
-    std::string s = std::string(p.name) + " (" + p.gcnArchName + ", " +
-                    std::to_string(p.multiProcessorCount) + " CUs)";
+    // the marketing name can be empty (no access to the product table)
+    const std::string name = p.name[0] ? std::string(p.name) : std::string("AMD GPU");
+    std::string s = name + " (" + p.gcnArchName + ", " + std::to_string(p.multiProcessorCount) +
+                    " CUs, " + std::to_string(p.totalGlobalMem >> 30) + " GiB)";
     snprintf(buf, (size_t)len, "%s", s.c_str());
     return NX_OK;
 }

@@ -310,3 +310,26 @@ def test_container_single_collision_cavity(nx, orc):
         g.set_tuning(**t0)
     o.run(10, 1.7)
     assert np.array_equal(g.populations(), o.fin)
+
+
+@pytest.mark.parametrize("st,cm", [("f64", "f64"), ("f32", "f32"), ("f16dev", "f32")])
+def test_container_single_collision_bitmap_equals_plain_step(nx, st, cm):
+    dim = (37, 12, 21)
+    X, Y, Z = dim
+    rng = np.random.default_rng(14)
+    cls = np.full((Z, Y, X), 2, np.uint32)
+    cls[rng.random((Z, Y, X)) < 0.05] = 0
+    cls[:, :, 0] = 1
+    a = nx.DenseLattice(dim, st, cm, periodic=2)
+    b = nx.DenseLattice(dim, st, cm, periodic=2)
+    for lat in (a, b):
+        lat.set_classes(cls.reshape(-1), ulb=0.04)
+    force = (2e-5, 1e-5, 0.0)
+    t0 = a.tuning()
+    try:
+        a.set_tuning(wg=512, boundary=2)
+        a.run(7, 1.5, force=force)
+    finally:
+        a.set_tuning(**t0)
+    b.run_plain(7, 1.5, force=force)
+    assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
