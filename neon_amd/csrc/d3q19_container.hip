@@ -376,11 +376,12 @@ __global__ void __launch_bounds__(WG)
 // Launch configuration of the container step per (storage, compute),
 // process-wide.  Defaults: the fastest configuration measured on the 256^3
 // cavity on MI355X -- the single-collision kernel for 8- and 4-byte storage
-// (FP32 35.2k vs 32.9k MLUPS, FP64 17.2k), the bitmap kernel for binary16
-// (49.1k vs 48.3k).
+// (FP32 35.4k MLUPS with the interior bitmap, 35.3k without, 32.9k for the
+// bitmap + full-pull kernel; FP64 17.2k), the bitmap + full-pull kernel for
+// binary16 (50.4k vs 48.3k).
 nx_tuning g_tuning[4][3] = {
     {{256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}},
-    {{256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}},
+    {{256, 1, 0, 0, 1, 2}, {256, 1, 0, 0, 1, 2}, {256, 1, 0, 0, 1, 2}},
     {{256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}},
     {{256, 1, 0, 0, 1, 0}, {256, 1, 0, 0, 1, 0}, {256, 1, 0, 0, 1, 0}},
 };
