@@ -3,9 +3,9 @@
 Populations are compared bit for bit: the kernels evaluate the oracle's
 expression tree with no FMA contraction and the same IEEE conversions, so
 every storage / compute combination is expected to agree exactly.  The
-north-star bound for binary64 (1e-12 relative after 1000 steps on the 64^3
-cavity) is asserted as well, and size-independent properties are checked at
-the benchmark size (256^3).
+binary64 bound (1e-12 relative after 1000 steps on the 64^3 cavity) is
+asserted as well, and size-independent properties are checked at the
+benchmark size (256^3).
 """
 import numpy as np
 import pytest
