@@ -22,7 +22,9 @@
 
 #include <cstdint>
 #include <cstring>
+#include <fstream>
 #include <functional>
+#include <ostream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -54,6 +56,87 @@ enum class Runtime { stream };
 enum class Occ { none, standard };
 enum class TransferMode { get, put };
 enum class StencilSemantic { standard, streaming };
+enum class IoFileType { ASCII, BINARY };
+
+namespace vtk {
+
+// Legacy VTK writer for one field over the full grid box, in the format of
+// the reference's ioToVTK (libNeonCore/include/Neon/core/tools/io/ioToVTK.h:
+// header :430-442, grid info :352-354, SCALARS / LOOKUP_TABLE / METADATA
+// blocks :239-272, CELL_DATA before POINT_DATA :320-325; text values
+// separated by one space with a newline per x row :194-215; binary values
+// big-endian followed by one newline :166-183).  `values` holds
+// box.x * box.y * box.z * card values, x fastest, then the component.
+// isNodeSpace false (the reference's default for Field::ioToVtk) writes the
+// values as cell data of a (box + 1)-point lattice.
+template <typename V>
+inline const char* typeName()
+{
+    if constexpr (std::is_same_v<V, double>) return "double";
+    else if constexpr (std::is_same_v<V, float>) return "float";
+    else if constexpr (std::is_same_v<V, int32_t> || std::is_same_v<V, uint32_t>) return "int";
+    else static_assert(sizeof(V) == 0, "ioToVtk exports double, float or 32-bit integer values");
+}
+
+template <typename V>
+inline void writeScalars(std::ostream& out, const std::string& name, const std::vector<V>& values,
+                         const index_3d& box, int card, IoFileType type)
+{
+    out << "SCALARS " << name << " " << typeName<V>() << " ";
+    if (card != 1) out << " " << card << "\n";
+    else out << "\n";
+    out << "LOOKUP_TABLE default\n";
+    const size_t row = size_t(box.x) * size_t(card);
+    const size_t rows = size_t(box.y) * size_t(box.z);
+    if (type == IoFileType::ASCII) {
+        for (size_t r = 0; r < rows; r++) {
+            for (size_t i = 0; i < row; i++) out << values[r * row + i] << " ";
+            out << "\n";
+        }
+    } else {
+        std::vector<unsigned char> buf(row * sizeof(V));
+        for (size_t r = 0; r < rows; r++) {
+            for (size_t i = 0; i < row; i++) {
+                unsigned char b[sizeof(V)];
+                std::memcpy(b, &values[r * row + i], sizeof(V));
+                for (size_t k = 0; k < sizeof(V); k++) buf[i * sizeof(V) + k] = b[sizeof(V) - 1 - k];
+            }
+            out.write(reinterpret_cast<const char*>(buf.data()), std::streamsize(buf.size()));
+        }
+        out << "\n";
+    }
+    out << "METADATA\n";
+    out << "INFORMATION 0\n\n";
+}
+
+// The whole file: fileName + ".vtk" holding one field of `card` components
+// over the box `dim` (values x fastest, then the component), plus the
+// all-zero "Domain" cell field when includeDomain (ioToVTK.h:413-445,
+// IOGridVTK_imp.h:6-18, FieldBase_imp.h:297-318).
+template <typename V>
+inline void writeFile(const std::string& fileName, const std::string& fieldName, const std::vector<V>& values,
+                      const index_3d& dim, int card, bool includeDomain, IoFileType type, bool isNodeSpace)
+{
+    const index_3d node = isNodeSpace ? dim : index_3d(dim.x + 1, dim.y + 1, dim.z + 1);
+    const index_3d vox(node.x - 1, node.y - 1, node.z - 1);
+    std::ofstream out(fileName + ".vtk", std::ios::out | std::ios::binary);
+    if (!out.is_open()) throw NeonException("ioToVtk: file " + fileName + ".vtk could not be opened");
+    out << "# vtk DataFile Version 3.0" << std::endl;
+    out << "Title Neon" << std::endl;
+    out << (type == IoFileType::ASCII ? "ASCII" : "BINARY") << std::endl;
+    out << "DATASET STRUCTURED_POINTS" << std::endl;
+    out << "DIMENSIONS " << node.x << " " << node.y << " " << node.z << "\n";
+    out << "ORIGIN " << 0.0 << " " << 0.0 << " " << 0.0 << "\n";
+    out << "SPACING " << 1.0 << " " << 1.0 << " " << 1.0 << "\n";
+    out << "CELL_DATA " << vox.rMulTyped() << std::endl;
+    if (!isNodeSpace) writeScalars(out, fieldName, values, dim, card, type);
+    if (includeDomain) writeScalars(out, "Domain", std::vector<V>(vox.rMulTyped(), V(0)), vox, 1, type);
+    out << "POINT_DATA " << node.rMulTyped() << std::endl;
+    if (isNodeSpace) writeScalars(out, fieldName, values, dim, card, type);
+    if (!out.good()) throw NeonException("ioToVtk: writing " + fileName + ".vtk failed");
+}
+
+}  // namespace vtk
 
 // ---- Backend ---------------------------------------------------------------
 
@@ -380,6 +463,27 @@ class Field
         });
     }
     T& hostRef(size_t cell, int card) { return s_->host[s_->grid.slot(cell, card, s_->card)]; }
+
+    // Writes fileName + ".vtk" from the host mirror (call updateHostData and
+    // sync first, as the reference's callers do), FieldBase::ioToVtk
+    // (libNeonDomain/include/Neon/domain/interface/FieldBase_imp.h:297-318,
+    // IOGridVTK_imp.h:6-64): every cell of the grid box, inactive cells
+    // holding the field's inactive value; includeDomain adds the cell-data
+    // field "Domain" with the owning partition (0: one partition).
+    template <typename VtiExportType = T>
+    void ioToVtk(const std::string& fileName, const std::string& fieldName, bool includeDomain = false,
+                 IoFileType ioFileType = IoFileType::ASCII, bool isNodeSpace = false) const
+    {
+        const index_3d dim = s_->grid.getDimension();
+        const int      card = s_->card;
+        std::vector<VtiExportType> dense(dim.rMulTyped() * size_t(card), VtiExportType(s_->inactive));
+        s_->grid.forEachActive([&](const index_3d& idx, size_t cell) {
+            const size_t at = (size_t(idx.z) * size_t(dim.y) + size_t(idx.y)) * size_t(dim.x) + size_t(idx.x);
+            for (int k = 0; k < card; k++)
+                dense[at * size_t(card) + size_t(k)] = VtiExportType(s_->host[s_->grid.slot(cell, k, card)]);
+        });
+        vtk::writeFile(fileName, fieldName, dense, dim, card, includeDomain, ioFileType, isNodeSpace);
+    }
 
     void updateDeviceData(int streamIdx)
     {

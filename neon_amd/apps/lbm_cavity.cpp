@@ -14,9 +14,11 @@
 // As in the reference: nu = ulb (N-2) / Re, omega = 1 / (3 nu + 1/2)
 // (Config.cpp:105-111); MLUPS = N^3 iterations / microseconds over the timed
 // loop (Metrics.h:31-50); visual mode writes the centre lines every 100
-// iterations (RunCavityTwoPop.cu:86-151).  float-fast, float-dev, half-dev,
-// --Re and --ulb are additions; eGrid, --deviceType cpu and --vti are not
-// part of this build and are rejected.
+// iterations (RunCavityTwoPop.cu:86-151), and with --vti also the legacy VTK
+// files u_<iter>.vtk and rho_<iter>.vtk that the reference writes there
+// unconditionally (:105-106; its --vti flag, Config.cpp:90, is only
+// recorded).  float-fast, float-dev, half-dev, --Re and --ulb are additions;
+// eGrid and --deviceType cpu are not part of this build and are rejected.
 #include "neon_amd.hpp"
 
 #include <algorithm>
@@ -145,7 +147,6 @@ Config parse(int argc, char** argv)
     if (c.deviceType != "gpu") usage("only --deviceType gpu is part of this build");
     if (c.devices.size() != 1) usage("one process drives one GPU: pass exactly one device id");
     if (c.gridType != "dGrid" && c.gridType != "bGrid") usage("--grid must be dGrid or bGrid (eGrid is not part of this build)");
-    if (c.vti) usage("--vti export is not part of this build");
     if (c.N < 3 || c.benchMaxIter < 0 || c.benchIniIter < 0 || c.repetitions < 1) usage("bad sizes");
     c.setLbmParameters();
     return c;
@@ -290,6 +291,10 @@ void run(const Config& config, Report& report)
             std::ofstream f(name);
             for (auto& v : pv) f << v.first << " " << v.second << "\n";
         };
+        if (config.vti) {
+            u.ioToVtk("u_" + id, "u", false);
+            rho.ioToVtk("rho_" + id, "rho", false);
+        }
         write(yPosVal, "NeonUniformLBM_" + id + "_Y.dat");
         write(xPosVal, "NeonUniformLBM_" + id + "_X.dat");
     };

@@ -20,7 +20,7 @@ def run_app(tmp, *args):
 
 
 def test_cli_rejects_what_is_not_built(tmp_path):
-    for extra in (["--grid", "eGrid"], ["--vti"], ["--storageFP", "quad"]):
+    for extra in (["--grid", "eGrid"], ["--deviceType", "cpu"], ["--storageFP", "quad"]):
         r = subprocess.run([str(EXE), "--deviceType", "gpu", "--deviceIds", "0", *extra],
                            cwd=tmp_path, capture_output=True, text=True, timeout=60)
         assert r.returncode == 2 and "usage" in r.stderr
@@ -66,3 +66,33 @@ def test_app_benchmark_mode_reports(tmp_path, grid, storage, compute):
     rep = json.loads((tmp_path / "rep.json").read_text())
     assert rep["gridType"] == grid and rep["N"] == 64 and rep["MLUPS"][0] > 0
     assert abs(rep["omega"] - 1.0 / (3.0 * 0.04 * 62 / 100.0 + 0.5)) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("grid", ["dGrid", "bGrid"])
+def test_app_vtk_export_matches_oracle(orc, tmp_path, grid):
+    from neon_amd import export
+    N, Re, ulb = 12, 100.0, 0.04
+    run_app(tmp_path, "--grid", grid, "--domain-size", N, "--storageFP", "double", "--computeFP", "double",
+            "--visual", "--vti", "--max-iter", 101, "--Re", Re, "--ulb", ulb)
+    for it in ("00000", "00100"):
+        assert (tmp_path / f"u_{it}.vtk").exists() and (tmp_path / f"rho_{it}.vtk").exists()
+    u_file = export.read_vtk(str(tmp_path / "u_00100.vtk"))
+    rho_file = export.read_vtk(str(tmp_path / "rho_00100.vtk"))
+    assert u_file["dimensions"] == (N + 1, N + 1, N + 1) and not u_file["binary"]
+    u = u_file["cell"]["u"]
+    rho = rho_file["cell"]["rho"][..., 0]
+    assert u.shape == (N, N, N, 3) and u.dtype == np.float64
+    nu = ulb * (N - 2) / Re
+    o = orc.DenseOracle((N, N, N), orc.F64, orc.COMPUTE_F64)
+    o.setup_cavity(ulb)
+    o.run(100, 1.0 / (3.0 * nu + 0.5))
+    r_ref, u_ref = o.moments()
+    b = (slice(1, N - 1),) * 3                      # bulk cells
+    # six significant digits in the text file
+    np.testing.assert_allclose(rho[b], r_ref[b], rtol=6e-6, atol=0)
+    for k in range(3):
+        np.testing.assert_allclose(u[b + (k,)], u_ref[(k,) + b], rtol=2e-5, atol=1e-10)
+    # the centre line written beside it is the same field
+    y = np.loadtxt(tmp_path / "NeonUniformLBM_00100_Y.dat")
+    np.testing.assert_allclose(y[:, 1], u[N // 2, :, N // 2, 0], rtol=2e-5, atol=1e-10)
