@@ -162,6 +162,10 @@ __global__ void __launch_bounds__(WG)
     for (int c = 0; c < CPT; c++)
 #pragma unroll
         for (int q = 0; q < Q; q++) pin(v[c][q]);
+    // the interior word travels with the population loads (left to the
+    // compiler it is fetched after they complete: one more memory latency)
+#pragma unroll
+    for (int c = 0; c < CPT; c++) pin(word[c]);
 #pragma unroll
     for (int c = 0; c < CPT; c++) {
         const uint32_t i = base + (uint32_t)(c * WG);
