@@ -214,9 +214,11 @@ def batched(nx, a):
                         "ulb": ulb, "ulid": (1.0, 0.0, 0.0)})
     b = nx.CavityBatch(members, storage)
     t_setup = time.perf_counter() - t0
-    launch = None
+    if a.edge is not None:
+        b.set_variant(xcd_chunks=b.launch()["xcd_chunks"], edge=a.edge)
     if a.autotune:
-        launch, _ = b.autotune()
+        b.autotune()
+    launch = b.launch()
     b.init()
     stream = nx.Stream()
     b.step(warmup, stream=stream.handle)
@@ -274,7 +276,10 @@ def main():
     ap.add_argument("--vx", type=int, default=None,
                     help="cells per lane of the dense step (1, or 16 bytes of cells: 4 f32, 2 f64)")
     ap.add_argument("--boundary", type=int, default=None,
-                    help="dense step boundary handling: 0 interior bitmap + full pull, 1 one collision with fix-ups")
+                    help="dense step boundary handling: 0 interior bitmap + full pull, 1 one collision with fix-ups, "
+                         "2 as 1 with the interior bitmap, 3 as 2 with edge speculation")
+    ap.add_argument("--edge", type=int, default=None, choices=(0, 1),
+                    help="batched cavities: 1 edge-speculating step, 0 wall links fetched after the flags")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 10 launches): the "
                          "reference for scaling FETCH_SIZE / WRITE_SIZE counters")

@@ -116,6 +116,9 @@ typedef struct {
                          /* pull; 1: every lane reads its flags, masked   */
                          /* links are fixed up, one collision for all;    */
                          /* 2: as 1, interior lanes read the bitmap only  */
+                         /* 3: as 2, and lanes in the two outer layers of */
+                         /* the box fetch flags and wall-link values with */
+                         /* the streaming loads (no dependent round trip) */
 } nx_tuning;
 
 /* ---- runtime (replaces libNeonSys GpuDevice / Backend, GpuDevice.h:151-190,
@@ -291,6 +294,14 @@ int nx_batch_run(nx_batch b, int32_t max_steps, int32_t check_every, double tol,
  * L2 serves a member's neighbouring planes; nx_batch_autotune also times
  * tile-by-tile dealing and keeps the faster. */
 int nx_batch_set_launch(nx_batch b, int32_t wg, int32_t wgs_per_cu);
+/* Step variant: xcd_chunks (1: deal tiles to XCDs in 65536-cell chunks, 0:
+ * tile by tile) and edge (1: cells in the two outer layers of a cavity fetch
+ * their wall-link values with the streaming loads; 0: after their flag word).
+ * Both give identical results.  nx_batch_get_launch reports the current
+ * launch (after nx_batch_autotune: the fastest one measured). */
+int nx_batch_set_variant(nx_batch b, int32_t xcd_chunks, int32_t edge);
+int nx_batch_get_launch(nx_batch b, int32_t* wg, int32_t* wgs_per_cu, int32_t* xcd_chunks,
+                        int32_t* edge);
 int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int32_t* best_wgs_per_cu,
                       float* best_ms);
 int nx_batch_destroy(nx_batch b);

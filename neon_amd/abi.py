@@ -134,6 +134,8 @@ SIGNATURES = {
     "nx_batch_step": (_I, [_P, _I32, _P]),
     "nx_batch_run": (_I, [_P, _I32, _I32, _D, _P, _P, C.POINTER(_U32)]),
     "nx_batch_set_launch": (_I, [_P, _I32, _I32]),
+    "nx_batch_set_variant": (_I, [_P, _I32, _I32]),
+    "nx_batch_get_launch": (_I, [_P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)]),
     "nx_batch_autotune": (_I, [_P, _P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(C.c_float)]),
     "nx_batch_destroy": (_I, [_P]),
     "nx_batch_constrain": (_I, [_P, _P, _I32, _P, _P]),

@@ -64,6 +64,7 @@ struct nx_batch_s {
     int wg = 256;
     int wgs_per_cu = 2;   // the autotuner's choice for 512 cavities of 32..96 cells on MI355X
     int xcd_chunks = 1;
+    int edge = 0;         // edge-speculating step (set per storage type, autotuned)
     int n_cu = 256;
     // captured graph
     hipGraphExec_t graph = nullptr;
@@ -78,7 +79,12 @@ namespace {
 
 constexpr int kTile = 256;
 
-template <int S>
+// EDGE: a lane in the two outer layers of its cavity (the wall layer and the
+// cells beside it) fetches its own 18 moving populations and the 18 upstream
+// values of the opposite populations with the streaming loads, so the cavity
+// walls cost no second memory round trip; cells beside the boxes inside the
+// cavity still fetch what they lack after their flag word arrives.
+template <int S, bool EDGE>
 __device__ __forceinline__ void step_cell(const typename Stor<S>::T* __restrict__ fin,
                                           typename Stor<S>::T* __restrict__ fout,
                                           const uint32_t* __restrict__ flags, const DGeom& g,
@@ -89,12 +95,35 @@ __device__ __forceinline__ void step_cell(const typename Stor<S>::T* __restrict_
     using R = typename Real<S, NX_COMPUTE_F32>::R;
     const uint32_t ic = min(i, g.n - 1u);
     uint32_t fl = flags[ic];
-    T v[Q];
+    T v[Q], a[Q], w[Q];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
         const uint32_t j = q == 9 ? ic : min(ic - (uint32_t)off, g.n - 1u);
         v[q] = fin[q * g.plane + j];
+        a[q] = T(0);
+        w[q] = T(0);
+    }
+    bool edge = false;
+    if constexpr (EDGE) {
+        int x, y, z;
+        dense::coords(g, ic, x, y, z);
+        edge = x <= 1 || x >= g.X - 2 || y <= 1 || y >= g.Y - 2 || z <= 1 || z >= g.Z - 2;
+        if (edge) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                if (q == 9) continue;
+                const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
+                const uint64_t po = (uint64_t)OPP[q] * g.plane;
+                a[q] = fin[po + ic];
+                w[q] = fin[po + min(ic - (uint32_t)off, g.n - 1u)];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            dense::pin(a[q]);
+            dense::pin(w[q]);
+        }
     }
 #pragma unroll
     for (int q = 0; q < Q; q++) dense::pin(v[q]);
@@ -113,15 +142,16 @@ __device__ __forceinline__ void step_cell(const typename Stor<S>::T* __restrict_
     const bool bulk = cls_of(fl) == NX_CELL_BULK;
     const uint32_t m = bulk ? (fl & NX_MASK_BITS) : 0u;
     const bool open = bulk && (fl & NX_FLAG_OPEN);
-    T a[Q], w[Q];
+    if (!edge) {
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-        if (q == 9) continue;
-        const bool mq = (m >> q) & 1u;
-        const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
-        const uint64_t po = (uint64_t)OPP[q] * g.plane;
-        a[q] = (mq || !bulk) ? fin[po + i] : T(0);
-        w[q] = (mq && !open) ? fin[po + min(i - (uint32_t)off, g.n - 1u)] : T(0);
+        for (int q = 0; q < Q; q++) {
+            if (q == 9) continue;
+            const bool mq = (m >> q) & 1u;
+            const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
+            const uint64_t po = (uint64_t)OPP[q] * g.plane;
+            a[q] = (mq || !bulk) ? fin[po + i] : T(0);
+            w[q] = (mq && !open) ? fin[po + min(i - (uint32_t)off, g.n - 1u)] : T(0);
+        }
     }
     P f[Q];
     if (open) {
@@ -165,7 +195,7 @@ constexpr uint32_t kChunkCells = 65536;  // cells per XCD chunk of tiles
 // the z-neighbour planes of a member (at most 36 tiles away for 96^3) are
 // read through the same L2.  Dealt tile by tile instead, a tile and its z
 // neighbours sit on different XCDs and every plane is fetched ~3 times.
-template <int S, int WG>
+template <int S, int WG, bool EDGE>
 __global__ void __launch_bounds__(WG)
     k_batch_step(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
                  const uint32_t* __restrict__ flags, const Member* __restrict__ members,
@@ -178,7 +208,7 @@ __global__ void __launch_bounds__(WG)
         const Member M = members[m];
         const DGeom g = geom_of(M);
         const uint32_t i = (t - M.tile_first) * (uint32_t)WG + threadIdx.x;
-        step_cell<S>(fin + M.pop_base, fout + M.pop_base, flags + M.cell_base, g, M, i);
+        step_cell<S, EDGE>(fin + M.pop_base, fout + M.pop_base, flags + M.cell_base, g, M, i);
     };
     if (xcd_chunks) {
         constexpr uint32_t C = kChunkCells / (uint32_t)WG;  // tiles per chunk
@@ -408,14 +438,20 @@ int launch_steps(nx_batch_s& b, int k_steps, int parity0, hipStream_t s)
         if (int e = by_storage(b.storage, [&](auto sc) {
                 constexpr int S = decltype(sc)::value;
                 using T = typename Stor<S>::T;
-                if (b.wg == 512)
-                    hipLaunchKernelGGL((k_batch_step<S, 512>), dim3(grid_of(b)), dim3(512), 0, s,
+                auto go = [&](auto wgc, auto edgec) {
+                    constexpr int WG = decltype(wgc)::value;
+                    constexpr bool E = decltype(edgec)::value;
+                    hipLaunchKernelGGL((k_batch_step<S, WG, E>), dim3(grid_of(b)), dim3(WG), 0, s,
                                        (const T*)fin, (T*)fout, b.flags, b.d_members,
                                        b.d_tile_member, b.active, b.n_tiles, (int)chunked(b));
-                else
-                    hipLaunchKernelGGL((k_batch_step<S, 256>), dim3(grid_of(b)), dim3(256), 0, s,
-                                       (const T*)fin, (T*)fout, b.flags, b.d_members,
-                                       b.d_tile_member, b.active, b.n_tiles, (int)chunked(b));
+                };
+                if (b.wg == 512) {
+                    if (b.edge) go(std::integral_constant<int, 512>{}, std::true_type{});
+                    else go(std::integral_constant<int, 512>{}, std::false_type{});
+                } else {
+                    if (b.edge) go(std::integral_constant<int, 256>{}, std::true_type{});
+                    else go(std::integral_constant<int, 256>{}, std::false_type{});
+                }
             }))
             return e;
     }
@@ -506,6 +542,17 @@ extern "C" int nx_batch_create(uint32_t n, const int32_t* dims, const uint8_t* c
         return set_error(NX_ERR_INVALID, "unknown storage type");
     auto* b = new nx_batch_s;
     b->storage = storage;
+    // measured on 512 cavities of 32..96 cells (the autotuner's choices):
+    // binary16 takes the edge-speculating step with 16 workgroups per CU
+    // dealt tile by tile (45.7k MLUPS; 32.6k with 2 per CU in XCD chunks,
+    // 19.0k without the speculation), FP32 the flag-dependent fetch with 2
+    // per CU (28.5k; 25.9k with the speculation, whose 36 extra registers
+    // cost more than the round trip they save)
+    if (storage == NX_F16DEV) {
+        b->edge = 1;
+        b->wgs_per_cu = 16;
+        b->xcd_chunks = 0;
+    }
     b->n_members = n;
     uint64_t off = 0;
     for (uint32_t k = 0; k < n; k++) {
@@ -741,6 +788,27 @@ extern "C" int nx_batch_set_launch(nx_batch b, int32_t wg, int32_t wgs_per_cu)
 
 // Times the step for every launch configuration (the populations advance by
 // an even number of steps per configuration) and keeps the fastest.
+extern "C" int nx_batch_set_variant(nx_batch b, int32_t xcd_chunks, int32_t edge)
+{
+    if (!b) return set_error(NX_ERR_INVALID, "null batch");
+    if ((xcd_chunks != 0 && xcd_chunks != 1) || (edge != 0 && edge != 1))
+        return set_error(NX_ERR_INVALID, "batch variant: xcd_chunks and edge are 0 or 1");
+    b->xcd_chunks = xcd_chunks;
+    b->edge = edge;
+    return NX_OK;
+}
+
+extern "C" int nx_batch_get_launch(nx_batch b, int32_t* wg, int32_t* wgs_per_cu, int32_t* xcd_chunks,
+                                   int32_t* edge)
+{
+    if (!b) return set_error(NX_ERR_INVALID, "null batch");
+    if (wg) *wg = b->wg;
+    if (wgs_per_cu) *wgs_per_cu = b->wgs_per_cu;
+    if (xcd_chunks) *xcd_chunks = b->xcd_chunks;
+    if (edge) *edge = b->edge;
+    return NX_OK;
+}
+
 extern "C" int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int32_t* best_wgs_per_cu,
                                  float* best_ms)
 {
@@ -749,13 +817,15 @@ extern "C" int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int3
     hipEvent_t e0, e1;
     NX_CHECK_HIP(hipEventCreate(&e0));
     NX_CHECK_HIP(hipEventCreate(&e1));
-    int bw = b->wg, bc = b->wgs_per_cu, bx = b->xcd_chunks;
+    int bw = b->wg, bc = b->wgs_per_cu, bx = b->xcd_chunks, be = b->edge;
     float bms = 1e30f;
+    for (int ed : {1, 0})
     for (int xc : {1, 0})
     for (int wg : {256, 512})
         for (int c : {2, 4, 8, 16, 0}) {
             if (int e = nx_batch_set_launch(b, wg, c)) return e;
             b->xcd_chunks = xc;
+            b->edge = ed;
             if (int e = launch_steps(*b, 2, 0, st)) return e;
             NX_CHECK_HIP(hipEventRecord(e0, st));
             if (int e = launch_steps(*b, 6, 0, st)) return e;
@@ -769,12 +839,14 @@ extern "C" int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int3
                 bw = wg;
                 bc = c;
                 bx = xc;
+                be = ed;
             }
         }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (int e = nx_batch_set_launch(b, bw, bc)) return e;
     b->xcd_chunks = bx;
+    b->edge = be;
     if (best_wg) *best_wg = bw;
     if (best_wgs_per_cu) *best_wgs_per_cu = bc;
     if (best_ms) *best_ms = bms;

@@ -377,17 +377,111 @@ __global__ void __launch_bounds__(WG)
     }
 }
 
+// Edge-speculating variant.  A lane whose cell lies in the two outermost
+// layers of the box (any coordinate in {0, 1, N-2, N-1}: the wall layer and
+// the cells beside it) fetches, in the same batch as the 19 streaming loads,
+// its flag word, its own 18 moving populations and the 18 upstream values of
+// the opposite populations, so a box wall -- bounce-back links of the cell
+// beside it, or the copy of a wall cell -- costs no second memory round trip.
+// Every lane stores, so each wavefront writes whole cache lines (a row whose
+// wall lanes skip their stores costs partial-line writes).  Other lanes read
+// the interior bitmap; a non-interior lane off the box edge (an obstacle
+// inside the box) fetches its flags and takes the general pull.  The
+// arithmetic is that of k_fused_fix, so results agree bit for bit.
+template <int S, int CM, bool FORCE, bool PER, int WG, int LP>
+__global__ void __launch_bounds__(WG)
+    k_fused_edge(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
+                 const uint32_t* __restrict__ flags, const uint32_t* __restrict__ bits, DGeom g, Par p,
+                 int remap, int copy_walls)
+{
+    using T = typename Stor<S>::T;
+    using P = typename Stor<S>::P;
+    using R = typename Real<S, CM>::R;
+    const uint32_t wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t i = wg * (uint32_t)WG + threadIdx.x;
+    const uint32_t ic = min(i, g.n - 1u);
+    int x, y, z;
+    coords(g, ic, x, y, z);
+    const bool edge = x <= 1 || x >= g.X - 2 || y <= 1 || y >= g.Y - 2 || z <= 1 || z >= g.Z - 2;
+    uint32_t word = bits[ic >> 5];
+    uint32_t fl = 0u;
+    T v[Q], a[Q], w[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
+        const uint32_t j = q == 9 ? ic : min(ic - (uint32_t)off, g.n - 1u);
+        v[q] = ld_plane<LP>(fin + q * g.plane, j);
+        a[q] = T(0);
+        w[q] = T(0);
+    }
+    if (edge) {
+        fl = flags[ic];
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            if (q == 9) continue;
+            const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
+            const T* po = fin + (uint64_t)OPP[q] * g.plane;
+            a[q] = ld_plane<LP>(po, ic);
+            w[q] = ld_plane<LP>(po, min(ic - (uint32_t)off, g.n - 1u));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        pin(v[q]);
+        pin(a[q]);
+        pin(w[q]);
+    }
+    pin(word);
+    pin(fl);
+    if (i >= g.n) return;
+    const bool interior = (word >> (i & 31u)) & 1u;
+    if (interior) fl = (uint32_t)NX_CELL_BULK << NX_CLASS_SHIFT;
+    else if (!edge) fl = flags[i];
+    const bool bulk = cls_of(fl) == NX_CELL_BULK;
+    const uint32_t m = (bulk && !interior) ? (fl & NX_MASK_BITS) : 0u;
+    bool general = bulk && !interior && (!edge || (fl & NX_FLAG_OPEN));
+    if (PER && bulk && !interior && !general)
+        general = ((g.periodic & NX_PERIODIC_X) && (x == 0 || x == g.X - 1)) ||
+                  ((g.periodic & NX_PERIODIC_Y) && (y == 0 || y == g.Y - 1)) ||
+                  ((g.periodic & NX_PERIODIC_Z) && (z == 0 || z == g.Z - 1));
+    if (!bulk && !copy_walls) return;
+    if (!bulk && !edge) {
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            if (q != 9) a[q] = fin[(uint64_t)OPP[q] * g.plane + i];
+    }
+    P f[Q];
+    if (general) {
+        pull<S, PER>(fin, g, i, fl, f);
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            f[q] = (q != 9 && ((m >> q) & 1u)) ? Stor<S>::bulk(a[q], OPP[q]) + Stor<S>::raw(w[q])
+                                               : Stor<S>::bulk(v[q], q);
+    }
+    R out[Q];
+    collide<S, CM, FORCE>(f, p.omega, p.g, out);
+    // one store per population for every lane (wall lanes store their copy)
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const T o = bulk ? Stor<S>::template enc<R>(out[q], q) : (q == 9 ? v[9] : a[OPP[q]]);
+        st_plane(fout + q * g.plane, i, o);
+    }
+}
+
 // Launch configuration of the container step per (storage, compute),
 // process-wide.  Defaults: the fastest configuration measured on the 256^3
 // cavity on MI355X -- the single-collision kernel for 8- and 4-byte storage
 // (FP32 35.4k MLUPS with the interior bitmap, 35.3k without, 32.9k for the
-// bitmap + full-pull kernel; FP64 17.2k), the bitmap + full-pull kernel for
-// binary16 (50.4k vs 48.3k).
+// bitmap + full-pull kernel; FP64 17.2k), the edge-speculating kernel for
+// binary16 (61.7k vs 51.9k for the bitmap + full-pull kernel; for 4- and
+// 8-byte storage its 36 extra registers cost more than the round trip it
+// saves: FP32 34.5k, FP64 16.6k).
 nx_tuning g_tuning[4][3] = {
     {{256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}},
     {{256, 1, 0, 0, 1, 2}, {256, 1, 0, 0, 1, 2}, {256, 1, 0, 0, 1, 2}},
     {{256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}},
-    {{256, 1, 0, 0, 1, 0}, {256, 1, 0, 0, 1, 0}, {256, 1, 0, 0, 1, 0}},
+    {{256, 1, 0, 0, 1, 3}, {256, 1, 0, 0, 1, 3}, {256, 1, 0, 0, 1, 3}},
 };
 
 template <int S, int CM>
@@ -405,7 +499,7 @@ bool valid_tuning(const nx_tuning& t, int storage)
 {
     return (t.wg == 256 || t.wg == 512) && (t.cpt == 1 || t.cpt == 2) &&
            (t.load_policy == 0 || t.load_policy == 1) && (t.xcd_remap == 0 || t.xcd_remap == 1) &&
-           (t.vx == 1 || t.vx == vx_of(storage)) && t.boundary >= 0 && t.boundary <= 2 &&
+           (t.vx == 1 || t.vx == vx_of(storage)) && t.boundary >= 0 && t.boundary <= 3 &&
            !(t.boundary != 0 && t.vx != 1);
 }
 
@@ -421,11 +515,15 @@ void launch_fused(const nx_tuning& t, const nx_container_s& c, hipStream_t s)
                            (const T*)c.fin, (T*)c.fout, c.flags, c.interior, c.g, c.p,
                            t.xcd_remap, c.copy_walls);
     };
-    if (t.boundary == 1 || t.boundary == 2) {
+    if (t.boundary >= 1 && t.boundary <= 3) {
         auto go_fix = [&](auto wgc, auto lpc) {
             constexpr int WG = decltype(wgc)::value, LP = decltype(lpc)::value;
             const uint32_t nb = (c.g.n + WG - 1) / WG;
-            if (t.boundary == 2)
+            if (t.boundary == 3)
+                hipLaunchKernelGGL((k_fused_edge<S, CM, FORCE, PER, WG, LP>), dim3(nb), dim3(WG), 0, s,
+                                   (const T*)c.fin, (T*)c.fout, c.flags, c.interior, c.g, c.p,
+                                   t.xcd_remap, c.copy_walls);
+            else if (t.boundary == 2)
                 hipLaunchKernelGGL((k_fused_fix<S, CM, FORCE, PER, WG, LP, true>), dim3(nb), dim3(WG), 0, s,
                                    (const T*)c.fin, (T*)c.fout, c.flags, c.interior, c.g, c.p,
                                    t.xcd_remap, c.copy_walls);
@@ -624,7 +722,7 @@ extern "C" int nx_d3q19_set_tuning_dense(const nx_dense_desc* d, const nx_tuning
     if (!t || !valid_tuning(*t, d->storage))
         return set_error(NX_ERR_INVALID,
                          "tuning: wg in {256,512}, cpt in {1,2}, load_policy and xcd_remap in "
-                         "{0,1}, vx = 1 or 16 bytes of cells (2 f64, 4 f32), boundary in {0,1,2} "
+                         "{0,1}, vx = 1 or 16 bytes of cells (2 f64, 4 f32), boundary in {0,1,2,3} "
                          "(1 and 2 need vx = 1)");
     g_tuning[d->storage][d->compute] = *t;
     return NX_OK;
@@ -642,7 +740,7 @@ extern "C" int nx_container_autotune(nx_container c, nx_stream s, nx_tuning* bes
     nx_tuning bt = slot;
     float bms = 1e30f;
     const int reps = 10;
-    for (int kv = 0; kv < 4; kv++)
+    for (int kv = 0; kv < 5; kv++)
     if (kv != 1 || vx_of(c->d.storage) > 1)
     for (int wg : {256, 512})
         for (int cpt : {1, 2})

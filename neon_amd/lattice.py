@@ -506,10 +506,18 @@ class CavityBatch:
     def set_launch(self, wg: int, wgs_per_cu: int):
         check(lib.nx_batch_set_launch(self.h, int(wg), int(wgs_per_cu)))
 
+    def set_variant(self, xcd_chunks: int = 1, edge: int = 1):
+        check(lib.nx_batch_set_variant(self.h, int(xcd_chunks), int(edge)))
+
+    def launch(self) -> dict:
+        v = [C.c_int32() for _ in range(4)]
+        check(lib.nx_batch_get_launch(self.h, *(C.byref(x) for x in v)))
+        return dict(zip(("wg", "wgs_per_cu", "xcd_chunks", "edge"), (int(x.value) for x in v)))
+
     def autotune(self, stream=None):
         w, c, ms = C.c_int32(), C.c_int32(), C.c_float()
         check(lib.nx_batch_autotune(self.h, stream, C.byref(w), C.byref(c), C.byref(ms)))
-        return {"wg": int(w.value), "wgs_per_cu": int(c.value)}, float(ms.value)
+        return self.launch(), float(ms.value)
 
     def member(self, k: int, which: int = 0):
         """(desc, populations pointer, flags pointer) of member k."""

@@ -27,12 +27,16 @@ def standalone(nx, m, st, steps):
     return g
 
 
-@pytest.mark.parametrize("st", ["f32", "f64", "f16dev"])
-@pytest.mark.parametrize("launch", [(256, 4), (512, 0)])
+@pytest.mark.parametrize("st", ["f32", "f64", "f16dev", "f32dev"])
+@pytest.mark.parametrize("launch", [(256, 4, 1, 0), (512, 0, 0, 1), (256, 16, 0, 1), (256, 2, 1, 1)])
 def test_batch_members_equal_standalone(nx, st, launch):
+    """Every launch (wg, workgroups per CU, XCD chunks, edge speculation):
+    the members' populations equal the standalone single-kernel step's."""
     ms = make_members(nx)
     b = nx.CavityBatch(ms, st)
-    b.set_launch(*launch)
+    b.set_launch(*launch[:2])
+    b.set_variant(xcd_chunks=launch[2], edge=launch[3])
+    assert b.launch()["edge"] == launch[3] and b.launch()["xcd_chunks"] == launch[2]
     b.init()
     b.step(14)
     for k, m in enumerate(ms):

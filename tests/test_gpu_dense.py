@@ -198,7 +198,8 @@ def test_container_cavity_counts(nx):
     assert n_int == (N - 4) ** 3 and n_bd == (N - 2) ** 3 - (N - 4) ** 3 and walls
 
 
-def test_container_leaves_differing_walls_unwritten(nx, orc):
+@pytest.mark.parametrize("boundary", [0, 3])
+def test_container_leaves_differing_walls_unwritten(nx, orc, boundary):
     """If the two fields hold different wall values the container must not
     copy them (the reference never writes wall cells)."""
     N = 12
@@ -211,7 +212,12 @@ def test_container_leaves_differing_walls_unwritten(nx, orc):
     o.pop[1][:] = p1.reshape(-1)
     g.set_populations(p1.reshape(-1), which=1)
     assert g.container_info(1.7)[2] is False
-    g.run(5, 1.7)
+    t0 = g.tuning()
+    try:
+        g.set_tuning(**{**t0, "boundary": boundary})
+        g.run(5, 1.7)
+    finally:
+        g.set_tuning(**t0)
     o.run(5, 1.7)
     assert np.array_equal(g.populations(), o.fin)
     g.parity ^= 1
@@ -297,23 +303,34 @@ def test_container_single_collision_equals_plain_step(nx, st, cm, wg):
     assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
 
 
-def test_container_single_collision_cavity(nx, orc):
-    N = 24
-    g, o = _pair(nx, orc, (N, N, N), "f32", "f32", 1, 1)
+@pytest.mark.parametrize("boundary", [1, 3])
+@pytest.mark.parametrize("st,cm,sto,cmo", COMBOS)
+def test_container_single_collision_cavity(nx, orc, st, cm, sto, cmo, boundary):
+    """boundary = 1 and 3 (the edge-speculating kernel: the cells of the two
+    outer layers fetch their wall-link values with the streaming loads) on the
+    cavity, bit for bit against the oracle; N = 26 puts row ends inside
+    wavefronts."""
+    N = 26
+    g, o = _pair(nx, orc, (N, N, N), st, cm, sto, cmo)
     g.setup_cavity(0.05)
     o.setup_cavity(0.05)
     t0 = g.tuning()
     try:
-        g.set_tuning(boundary=1)
+        g.set_tuning(boundary=boundary)
         g.run(10, 1.7)
     finally:
         g.set_tuning(**t0)
     o.run(10, 1.7)
-    assert np.array_equal(g.populations(), o.fin)
+    assert np.array_equal(_graw(g), _raw(o))
 
 
-@pytest.mark.parametrize("st,cm", [("f64", "f64"), ("f32", "f32"), ("f16dev", "f32")])
-def test_container_single_collision_bitmap_equals_plain_step(nx, st, cm):
+@pytest.mark.parametrize("boundary,wg", [(2, 512), (3, 256), (3, 512)])
+@pytest.mark.parametrize("st,cm", [("f64", "f64"), ("f32", "f32"), ("f32", "ref"), ("f32dev", "f32"),
+                                   ("f16dev", "f32")])
+def test_container_single_collision_bitmap_equals_plain_step(nx, st, cm, boundary, wg):
+    """boundary = 2 (interior bitmap) and 3 (bitmap + edge speculation) against
+    the single-kernel step: obstacles inside the box take the general path,
+    open x/z faces, periodic y and a body force."""
     dim = (37, 12, 21)
     X, Y, Z = dim
     rng = np.random.default_rng(14)
@@ -324,10 +341,10 @@ def test_container_single_collision_bitmap_equals_plain_step(nx, st, cm):
     b = nx.DenseLattice(dim, st, cm, periodic=2)
     for lat in (a, b):
         lat.set_classes(cls.reshape(-1), ulb=0.04)
-    force = (2e-5, 1e-5, 0.0)
+    force = (0.0, 0.0, 0.0) if cm == "ref" else (2e-5, 1e-5, 0.0)
     t0 = a.tuning()
     try:
-        a.set_tuning(wg=512, boundary=2)
+        a.set_tuning(wg=wg, boundary=boundary)
         a.run(7, 1.5, force=force)
     finally:
         a.set_tuning(**t0)
