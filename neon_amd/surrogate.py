@@ -208,13 +208,13 @@ def converged(res: np.ndarray, tol: float) -> np.ndarray:
     return np.isfinite(res) & (res < tol)
 
 
-def generate(members, tol=1e-6, max_steps=200000, check_every=50, storage="f32dev", log=None):
+def generate(members, tol=1e-6, max_steps=200000, check_every=50, storage="f32dev", log=None, chunk=10000):
     """Step every member to the residual with one batch (FP32 deviation
     storage by default: raw FP32 populations stall near a residual of 1e-5
     on small cavities); returns (batch, steps, residual, unconverged, seconds)."""
     b = CavityBatch(members, storage)
     b.init()
-    steps, res, left, secs = converge(b, tol, max_steps, check_every, log=log)
+    steps, res, left, secs = converge(b, tol, max_steps, check_every, chunk=chunk, log=log)
     return b, steps, res, left, secs
 
 
@@ -418,6 +418,7 @@ def main(argv=None):
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--check-every", type=int, default=50)
     ap.add_argument("--max-steps", type=int, default=200000)
+    ap.add_argument("--chunk", type=int, default=10000, help="steps between progress lines of the solver runs")
     ap.add_argument("--storage", default="f32dev")
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--widths", default="16,32,64,64")
@@ -449,7 +450,7 @@ def main(argv=None):
     train_ms, test_ms = ms[:a.train], ms[a.train:]
     log(f"generate {a.train} cavities {a.size}^3 to residual {a.tol:g}")
     b, steps_g, res_g, left_g, t_gen = generate(train_ms, a.tol, a.max_steps, a.check_every, a.storage,
-                                                log=log)
+                                                log=log, chunk=a.chunk)
     ok = converged(res_g, a.tol)
     x = features(train_ms)[ok]
     y, w = targets(b, train_ms)
@@ -478,7 +479,7 @@ def main(argv=None):
     # (members that diverge from rest are dropped from both comparisons)
     cold.init()
     check(lib.nx_sync())
-    steps_c, res_c, left_c, t_cold = converge(cold, a.tol, a.max_steps, a.check_every, log=log)
+    steps_c, res_c, left_c, t_cold = converge(cold, a.tol, a.max_steps, a.check_every, chunk=a.chunk, log=log)
     yt, wt = targets(cold, test_ms)
     del cold
     keep = converged(res_c, a.tol)
@@ -490,7 +491,7 @@ def main(argv=None):
         cold = CavityBatch(test_ms, a.storage)
         cold.init()
         check(lib.nx_sync())
-        steps_c, res_c, left_c, t_cold = converge(cold, a.tol, a.max_steps, a.check_every, log=log)
+        steps_c, res_c, left_c, t_cold = converge(cold, a.tol, a.max_steps, a.check_every, chunk=a.chunk, log=log)
         del cold
     n_test = len(test_ms)
 
@@ -503,7 +504,7 @@ def main(argv=None):
     rho, u = fields_from_prediction(pred, test_ms)
     fb = warm_start(warm, rho, u, a.check_every)
     t_probe = time.perf_counter() - t0 - t_inf
-    steps_w, res_w, left_w, t_w = converge(warm, a.tol, a.max_steps, a.check_every, log=log)
+    steps_w, res_w, left_w, t_w = converge(warm, a.tol, a.max_steps, a.check_every, chunk=a.chunk, log=log)
     t_warm = time.perf_counter() - t0
     probe = warm_start_overhead(a.check_every)
     out = {
