@@ -61,6 +61,17 @@ BATCH = {
 }
 
 
+# Duct with Zou-He ends: X x Y x Y cells along x, bounce-back walls on the
+# four side faces, density faces rho_in / rho_out on the x planes (the
+# flow-over-sphere benchmark's inlet/outlet cells), nu = 0.02; stepped by
+# nx_d3q19_step_dense_zouhe (one thread per cell, flag word + pull).
+ZOUHE = {
+    # name: (X, Y, storage, default steps, default warmup)
+    "duct512_zouhe_f32": (512, 128, "f32", 200, 20),
+    "duct512_zouhe_f64": (512, 128, "f64", 100, 10),
+}
+
+
 def traffic_from_profiles(kernel_tag: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, scaled by a copy kernel of known size), if committed."""
@@ -192,6 +203,63 @@ def porous(nx, a):
     print(json.dumps(out), flush=True)
 
 
+def zouhe_duct(nx, a):
+    X, Y, storage, dsteps, dwarm = ZOUHE[a.workload]
+    steps = dsteps if a.steps is None else a.steps
+    warmup = dwarm if a.warmup is None else a.warmup
+    from neon_amd import abi
+    nu = 0.02
+    omega = 1.0 / (3.0 * nu + 0.5)
+    nx.init(0)
+    dim = (X, Y, Y)
+    cls = np.full((Y, Y, X), abi.NX_CELL_BULK, np.uint32)
+    cls[:, 0, :] = cls[:, -1, :] = cls[0, :, :] = cls[-1, :, :] = abi.NX_CELL_BOUNCEBACK
+    zh = np.zeros((Y, Y, X), np.uint32)
+    zh[:, :, 0] = abi.NX_ZH_XMIN
+    zh[:, :, -1] = abi.NX_ZH_XMAX
+    zh[cls != abi.NX_CELL_BULK] = 0
+    values = [1.0005, 0.9995, 0.0, 0.0, 0.0, 0.0]
+    lat = nx.DenseLattice(dim, storage, storage)
+    lat.set_classes(cls.reshape(-1), zouhe=zh.reshape(-1))
+    stream = nx.Stream()
+    lat.run_zouhe(warmup, omega, values, stream=stream.handle)
+    stream.sync()
+    e0, e1 = nx.Event(), nx.Event()
+    nx.sync()
+    tw = time.perf_counter()
+    e0.record(stream)
+    lat.run_zouhe(steps, omega, values, stream=stream.handle)
+    e1.record(stream)
+    e1.sync()
+    wall = time.perf_counter() - tw
+    ms = e0.elapsed_ms(e1)
+    ms_step = ms / steps
+    fluid = int((cls == abi.NX_CELL_BULK).sum())
+    mlups = fluid * steps / (ms * 1e-3) / 1e6
+    b_alg = 2 * 19 * STORAGE_BYTES[storage]
+    achieved = fluid * b_alg / (ms_step * 1e-3) / 1e9
+    _, u = lat.moments(stream=stream.handle)
+    ux = u[0].reshape(Y, Y, X).astype(np.float64)
+    out = {
+        "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": storage,
+        "data": "synthetic (duct at rest, f = w_q)",
+        "config": {"workload": f"pressure-driven duct {X}x{Y}x{Y}, Zou-He density faces 1.0005 / 0.9995 "
+                               f"on the x planes, bounce-back side walls, nu = 0.02, {storage}, "
+                               f"{steps} timed steps after {warmup}",
+                   "fluid_cells": fluid, "zouhe_cells": int((zh > 0).sum()), "device": nx.device_name(),
+                   "centre_ux": float(ux[Y // 2, Y // 2, X // 2])},
+        "wall_s": round(wall, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(f"d3q19_zouhe_{storage}_{X}"),
+                     "kernel": f"k_step_zh<{storage}> ({b_alg} B/fluid cell)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def batched(nx, a):
     n, storage, dsteps, dwarm = BATCH[a.workload]
     steps = dsteps if a.steps is None else a.steps
@@ -268,7 +336,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="cavity256_f32", choices=sorted(WORKLOADS) + sorted(POROUS) + sorted(BATCH))
+    ap.add_argument("--workload", default="cavity256_f32", choices=sorted(WORKLOADS) + sorted(POROUS) + sorted(BATCH) + sorted(ZOUHE))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--autotune", action="store_true",
@@ -300,6 +368,8 @@ def main():
         return porous(nx, a)
     if a.workload in BATCH:
         return batched(nx, a)
+    if a.workload in ZOUHE:
+        return zouhe_duct(nx, a)
 
     N, storage, compute, re, dsteps, dwarm = WORKLOADS[a.workload]
     steps = dsteps if a.steps is None else a.steps

@@ -17,6 +17,10 @@
  *                grid or at an inactive voxel (its bounce-back term is 0)
  *   bit  29      NX_FLAG_MIXED: masked links of both kinds (some upstream
  *                cells exist, some do not)
+ *   bits 19..21  Zou-He face of a bulk boundary cell (NX_ZH_*; 0: none),
+ *   bit  22      NX_ZH_VELOCITY: prescribed normal velocity (else density);
+ *                kept by the wall-mask kernels, read by
+ *                nx_d3q19_step_dense_zouhe only
  *
  * Population layouts:
  *   dense (dGrid SoA, dField_imp.h:67-87):  f[q][z][y][x], index
@@ -66,6 +70,21 @@ extern "C" {
 #define NX_MASK_BITS 0x7FDFFu
 #define NX_FLAG_OPEN (1u << 28)
 #define NX_FLAG_MIXED (1u << 29)
+/* Zou-He boundary cells (the pressure / velocity cells of
+ * benchmarks/lbm-flow-over-sphere/src/CellType.h:7-15): the face names the
+ * side the unknown populations come from (NX_ZH_XMIN: the cell's -x
+ * neighbour is outside the fluid, the unknowns are the 5 populations with
+ * c_x = +1), value[face - 1] of nx_d3q19_step_dense_zouhe the prescribed
+ * density, or with NX_ZH_VELOCITY the velocity along the inward normal. */
+#define NX_ZH_SHIFT 19
+#define NX_ZH_XMIN 1
+#define NX_ZH_XMAX 2
+#define NX_ZH_YMIN 3
+#define NX_ZH_YMAX 4
+#define NX_ZH_ZMIN 5
+#define NX_ZH_ZMAX 6
+#define NX_ZH_VELOCITY (1u << 22)
+#define NX_ZH_BITS (0xFu << 19)
 
 #define NX_PERIODIC_X 1
 #define NX_PERIODIC_Y 2
@@ -175,6 +194,23 @@ int nx_d3q19_wall_mask_dense(const nx_dense_desc* d, const uint32_t* flags_in,
                              uint32_t* flags_out, nx_stream s);
 int nx_d3q19_step_dense(const nx_dense_desc* d, const void* fin, void* fout,
                         const uint32_t* flags, const nx_lbm_params* p, nx_stream s);
+/* As nx_d3q19_step_dense, and bulk cells with a Zou-He face bit run the
+ * Zou-He completion of streamAndZouhe
+ * (benchmarks/lbm-flow-over-sphere/src/LbmContainers.h:145-287; tables of
+ * unknown and middle directions :633-646): pull,
+ * known sum K over the 5 outgoing populations, middle sum M over the 9 with
+ * no normal component; density face: rho = value, normal velocity
+ * u.n = 1 - (M + 2K) / rho; velocity face: u.n = value,
+ * rho = (M + 2K) / (1 - u.n) (the reference's velocity branch reads its
+ * velocity as a density, :128-133 and :230; this is the textbook form);
+ * tangential velocity 0; each unknown f_i = f_opp(i) + feq_i - feq_opp(i);
+ * BGK with that rho and u.  value[6] is indexed by face - 1.  Storage
+ * NX_F64 (compute F64) or NX_F32 (compute F32); force must be zero.
+ * Containers reject flag fields holding Zou-He cells, and
+ * nx_d3q19_step_dense treats them as plain bulk cells. */
+int nx_d3q19_step_dense_zouhe(const nx_dense_desc* d, const void* fin, void* fout,
+                              const uint32_t* flags, const nx_lbm_params* p,
+                              const double value[6], nx_stream s);
 int nx_d3q19_moments_dense(const nx_dense_desc* d, const void* fin,
                            const uint32_t* flags, void* rho, void* u, nx_stream s);
 

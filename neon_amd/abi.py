@@ -19,6 +19,9 @@ NX_CLASS_SHIFT = 24
 NX_MASK_BITS = 0x7FDFF
 NX_FLAG_OPEN = 1 << 28
 NX_FLAG_MIXED = 1 << 29
+NX_ZH_SHIFT = 19
+NX_ZH_XMIN, NX_ZH_XMAX, NX_ZH_YMIN, NX_ZH_YMAX, NX_ZH_ZMIN, NX_ZH_ZMAX = range(1, 7)
+NX_ZH_VELOCITY_CODE = 8   # added to a face code: bit 22 of the flag word
 NX_PERIODIC_X, NX_PERIODIC_Y, NX_PERIODIC_Z = 1, 2, 4
 NX_CONV_RELU, NX_CONV_OUT_F32 = 1, 2
 
@@ -112,6 +115,7 @@ SIGNATURES = {
     "nx_d3q19_fill_dense": (_I, [_DP, _P, _P, _D, C.POINTER(_D), _P]),
     "nx_d3q19_wall_mask_dense": (_I, [_DP, _P, _P, _P]),
     "nx_d3q19_step_dense": (_I, [_DP, _P, _P, _P, _PP, _P]),
+    "nx_d3q19_step_dense_zouhe": (_I, [_DP, _P, _P, _P, _PP, C.POINTER(C.c_double), _P]),
     "nx_d3q19_moments_dense": (_I, [_DP, _P, _P, _P, _P, _P]),
     "nx_d3q19_mass_dense": (_I, [_DP, _P, _P, _P, _P]),
     "nx_d3q19_residual_dense": (_I, [_DP, _P, _P, _P, _P, _P]),

@@ -54,7 +54,7 @@ constexpr int kCWG = 256;  // classification workgroup (4 wavefronts)
 
 __device__ __forceinline__ bool is_interior(const DGeom& g, uint32_t i, uint32_t fl)
 {
-    if (cls_of(fl) != NX_CELL_BULK || (fl & (NX_MASK_BITS | NX_FLAG_OPEN))) return false;
+    if (cls_of(fl) != NX_CELL_BULK || (fl & (NX_MASK_BITS | NX_FLAG_OPEN | NX_ZH_BITS))) return false;
     if (!g.periodic) return true;
     int x, y, z;
     coords(g, i, x, y, z);
@@ -78,22 +78,27 @@ __global__ void __launch_bounds__(kCWG)
     const bool bd = i < g.n && cls_of(fl) == NX_CELL_BULK && !in;
     const uint64_t bi = __ballot(in);
     const uint64_t bb = __ballot(bd);
+    const uint64_t bz = __ballot(bd && (fl & NX_ZH_BITS));
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // wavefront w of workgroup b covers cells 64*(4b+w) .. +63: words 2*(4b+w), +1
     const uint32_t word = 2u * (blockIdx.x * (kCWG / 64) + wave);
     if (lane == 0 && word < (g.n + 31) / 32) bits[word] = (uint32_t)bi;
     if (lane == 1 && word + 1 < (g.n + 31) / 32) bits[word + 1] = (uint32_t)(bi >> 32);
     if (lane == 0) {
-        cnt[wave] = (uint32_t)__popcll(bb);
+        // bit 31: the wavefront holds a Zou-He cell (counts are <= 256)
+        cnt[wave] = (uint32_t)__popcll(bb) | (bz ? 0x80000000u : 0u);
         icnt[wave] = (uint32_t)__popcll(bi);
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t c = 0, ic = 0;
+        uint32_t zh = 0;
         for (int w = 0; w < kCWG / 64; w++) {
-            c += cnt[w];
+            c += cnt[w] & 0x7FFFFFFFu;
+            zh |= cnt[w] & 0x80000000u;
             ic += icnt[w];
         }
+        c |= zh;
         wg_count[blockIdx.x] = c;
         wg_interior[blockIdx.x] = ic;
     }
@@ -658,10 +663,16 @@ extern "C" int nx_d3q19_iteration_dense(const nx_dense_desc* d, const void* fin,
     if ((e = hipMemcpy(icnt.data(), d_icnt, (size_t)nwg_c * 8, hipMemcpyDeviceToHost)) != hipSuccess)
         return fail(hip_error(e, "hipMemcpy"));
     uint64_t total = 0, itotal = 0;
+    bool zouhe = false;
     for (uint32_t k = 0; k < nwg_c; k++) {
-        total += cnt[k];
+        total += cnt[k] & 0x7FFFFFFFu;
+        zouhe = zouhe || (cnt[k] & 0x80000000u);
         itotal += icnt[k];
     }
+    if (zouhe)
+        return fail(set_error(NX_ERR_UNSUPPORTED,
+                              "container: the flags hold Zou-He cells; step them with "
+                              "nx_d3q19_step_dense_zouhe"));
     c->n_boundary = total;
     c->n_interior = itotal;
     // do both fields hold the same wall values?

@@ -9,6 +9,8 @@
 // remapped so each XCD walks a contiguous slab (L2 keeps the row-boundary
 // lines).  Each population is read once and written once per step:
 // 2*19*sizeof(storage) bytes per cell update.
+#include <cmath>
+
 #include "dense_common.hpp"
 
 namespace nx {
@@ -30,6 +32,41 @@ __global__ void __launch_bounds__(kWG)
     pull<S, PERIODIC>(fin, g, i, fl, f);
     R out[Q];
     collide<S, CM, FORCE>(f, p.omega, p.g, out);
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        __builtin_nontemporal_store(Stor<S>::template enc<R>(out[q], q), &fout[q * g.plane + i]);
+}
+
+// Step with Zou-He boundary cells (no body force): bulk cells whose flag
+// carries a face in bits 19..21 complete their 5 unknown populations and
+// collide with the prescribed density or velocity (zouhe_bgk); every other
+// bulk cell is the plain step.
+struct ZhValues {
+    double v[6];
+};
+
+template <int S, int CM, bool PERIODIC>
+__global__ void __launch_bounds__(kWG)
+    k_step_zh(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
+              const uint32_t* __restrict__ flags, DGeom g, Par p, ZhValues zh)
+{
+    using R = typename Real<S, CM>::R;
+    const uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * kWG + threadIdx.x;
+    if (i >= g.n) return;
+    const uint32_t fl = __builtin_nontemporal_load(&flags[i]);
+    if (cls_of(fl) != NX_CELL_BULK) return;
+    typename Stor<S>::P f[Q];
+    pull<S, PERIODIC>(fin, g, i, fl, f);
+    R out[Q];
+    const int face = (int)((fl >> NX_ZH_SHIFT) & 7u);
+    if (face >= 1 && face <= 6) {
+        R fr[Q];
+#pragma unroll
+        for (int q = 0; q < Q; q++) fr[q] = (R)f[q];
+        zouhe_bgk<R>(fr, face, (fl & NX_ZH_VELOCITY) != 0, (R)zh.v[face - 1], (R)p.omega, out);
+    } else {
+        collide<S, CM, false>(f, p.omega, p.g, out);
+    }
 #pragma unroll
     for (int q = 0; q < Q; q++)
         __builtin_nontemporal_store(Stor<S>::template enc<R>(out[q], q), &fout[q * g.plane + i]);
@@ -104,7 +141,7 @@ __global__ void __launch_bounds__(kWG)
         }
     }
     fout[i] = ((uint32_t)NX_CELL_BULK << NX_CLASS_SHIFT) | m | open |
-              (open && closed ? NX_FLAG_MIXED : 0u);
+              (open && closed ? NX_FLAG_MIXED : 0u) | (fl & NX_ZH_BITS);
 }
 
 template <typename M>
@@ -299,6 +336,46 @@ extern "C" int nx_d3q19_step_dense(const nx_dense_desc* d, const void* fin, void
         }))
         return e;
     NX_CHECK_LAUNCH("k_step");
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_step_dense_zouhe(const nx_dense_desc* d, const void* fin, void* fout,
+                                         const uint32_t* flags, const nx_lbm_params* p,
+                                         const double value[6], nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!fin || !fout || !flags || !p || !value) return set_error(NX_ERR_INVALID, "null pointer");
+    if (fin == fout)
+        return set_error(NX_ERR_INVALID, "fin and fout must differ (two-population scheme)");
+    if (p->force[0] != 0. || p->force[1] != 0. || p->force[2] != 0.)
+        return set_error(NX_ERR_UNSUPPORTED, "Zou-He step: body force must be zero");
+    const bool f64 = d->storage == NX_F64 && d->compute == NX_COMPUTE_F64;
+    const bool f32 = d->storage == NX_F32 && d->compute == NX_COMPUTE_F32;
+    if (!f64 && !f32)
+        return set_error(NX_ERR_UNSUPPORTED,
+                         "Zou-He step: storage/compute NX_F64/NX_COMPUTE_F64 or NX_F32/NX_COMPUTE_F32");
+    for (int k = 0; k < 6; k++)
+        if (!std::isfinite(value[k])) return set_error(NX_ERR_INVALID, "Zou-He step: non-finite value");
+    const DGeom g = make_geom(d);
+    const Par par{p->omega, {0., 0., 0.}};
+    ZhValues zh;
+    for (int k = 0; k < 6; k++) zh.v[k] = value[k];
+    const hipStream_t st = (hipStream_t)s;
+    auto go = [&](auto sc, auto cc, auto pc) {
+        constexpr int S = decltype(sc)::value, CM = decltype(cc)::value;
+        constexpr bool PER = decltype(pc)::value;
+        using T = typename Stor<S>::T;
+        hipLaunchKernelGGL((k_step_zh<S, CM, PER>), dim3(nwg(g)), dim3(kWG), 0, st, (const T*)fin,
+                           (T*)fout, flags, g, par, zh);
+    };
+    if (f64) {
+        if (g.periodic) go(std::integral_constant<int, NX_F64>{}, std::integral_constant<int, NX_COMPUTE_F64>{}, std::true_type{});
+        else go(std::integral_constant<int, NX_F64>{}, std::integral_constant<int, NX_COMPUTE_F64>{}, std::false_type{});
+    } else {
+        if (g.periodic) go(std::integral_constant<int, NX_F32>{}, std::integral_constant<int, NX_COMPUTE_F32>{}, std::true_type{});
+        else go(std::integral_constant<int, NX_F32>{}, std::integral_constant<int, NX_COMPUTE_F32>{}, std::false_type{});
+    }
+    NX_CHECK_LAUNCH("k_step_zh");
     return NX_OK;
 }
 

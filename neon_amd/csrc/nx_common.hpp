@@ -199,6 +199,79 @@ __device__ __forceinline__ void bgk_ref(const float f[Q], float omega, float out
     out[9] = (float)((1. - (double)omega) * (double)f[9] + (double)(omega * eq9));
 }
 
+// Zou-He boundary cell, streamAndZouhe + collideBgkUnrolled
+// (benchmarks/lbm-flow-over-sphere/src/LbmContainers.h:145-287, :351-443).
+// face 1..6 = unknowns from -x, +x, -y, +y, -z, +z; the unknown and middle
+// direction lists (and so the summation orders) are the reference's tables
+// (:633-646).  f holds the pulled populations; the 5 unknown entries are
+// replaced.  Density face: rho = value, u.n = 1 - (M + 2K) / rho, written as
+// the reference does, u_axis = ((M + 2K) / rho - 1) * (+1 if the unknowns
+// point to -axis, else -1).  Velocity face (textbook form): u.n = value,
+// rho = (M + 2K) / (1 - u.n).
+constexpr int kZhUnk[6][5] = {{10, 13, 14, 15, 16}, {0, 3, 4, 5, 6},   {11, 4, 13, 17, 18},
+                              {1, 3, 7, 8, 14},     {12, 6, 8, 15, 17}, {2, 5, 7, 16, 18}};
+constexpr int kZhMid[3][4] = {{1, 2, 7, 8}, {0, 2, 5, 6}, {0, 1, 3, 4}};
+
+// One face, every direction index a compile-time constant (f stays in
+// registers).
+template <typename R, int FACE>
+__device__ __forceinline__ void zouhe_face(R f[Q], bool velocity, R value, R omega, R out[Q])
+{
+    constexpr int AXIS = (FACE - 1) >> 1;
+    R known = R(0), middle = R(0);
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const int iu = kZhUnk[FACE - 1][k];
+        known += f[iu < 9 ? iu + 10 : iu - 10];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        middle += f[kZhMid[AXIS][k]];
+        middle += f[kZhMid[AXIS][k] + 10];
+    }
+    middle += f[9];
+    const R sgn = kZhUnk[FACE - 1][0] < 9 ? R(1) : R(-1);
+    R rho, ua;
+    if (velocity) {
+        ua = -sgn * value;
+        rho = (middle + known * R(2)) / (R(1) - value);
+    } else {
+        rho = value;
+        ua = (((middle + known * R(2)) / rho) - R(1)) * sgn;
+    }
+    const R u0 = AXIS == 0 ? ua : R(0), u1 = AXIS == 1 ? ua : R(0), u2 = AXIS == 2 ? ua : R(0);
+    const R usqr = R(1.5) * (u0 * u0 + u1 * u1 + u2 * u2);
+    const R ck[9] = {u0, u1, u2, u0 + u1, u0 - u1, u0 + u2, u0 - u2, u1 + u2, u1 - u2};
+    R eq[Q];
+#pragma unroll
+    for (int i = 0; i < 9; i++) {
+        const R wi = i < 3 ? R(1. / 18.) : R(1. / 36.);
+        eq[i] = rho * wi * (R(1.) - R(3.) * ck[i] + R(4.5) * ck[i] * ck[i] - usqr);
+        eq[i + 10] = eq[i] + rho * wi * R(6.) * ck[i];
+    }
+    eq[9] = rho * R(1. / 3.) * (R(1.) - usqr);
+#pragma unroll
+    for (int k = 0; k < 5; k++) {
+        const int iu = kZhUnk[FACE - 1][k], ik = iu < 9 ? iu + 10 : iu - 10;
+        f[iu] = f[ik] + eq[iu] - eq[ik];
+    }
+#pragma unroll
+    for (int q = 0; q < Q; q++) out[q] = (R(1.) - omega) * f[q] + omega * eq[q];
+}
+
+template <typename R>
+__device__ __forceinline__ void zouhe_bgk(R f[Q], int face, bool velocity, R value, R omega, R out[Q])
+{
+    switch (face) {
+        case 1: zouhe_face<R, 1>(f, velocity, value, omega, out); break;
+        case 2: zouhe_face<R, 2>(f, velocity, value, omega, out); break;
+        case 3: zouhe_face<R, 3>(f, velocity, value, omega, out); break;
+        case 4: zouhe_face<R, 4>(f, velocity, value, omega, out); break;
+        case 5: zouhe_face<R, 5>(f, velocity, value, omega, out); break;
+        default: zouhe_face<R, 6>(f, velocity, value, omega, out); break;
+    }
+}
+
 // BGK with Guo et al. (2002) forcing (no reference counterpart), body
 // acceleration g, force density rho*g:
 //   u = sum c f / rho + g/2 ;  out = (1-omega) f + omega feq(rho,u)

@@ -253,12 +253,30 @@ class DenseLattice(_Lattice):
         self._fill_and_mask(ulb, ulid, stream)
 
     def set_classes(self, cls: np.ndarray, ulb: float = 0.0, ulid=(1.0, 0.0, 0.0),
-                    stream=None) -> None:
-        """Arbitrary classification (x fastest), then fill + wall mask."""
+                    stream=None, zouhe: np.ndarray | None = None) -> None:
+        """Arbitrary classification (x fastest), then fill + wall mask.
+        zouhe: optional per-cell Zou-He code for bulk cells (face
+        NX_ZH_XMIN..NX_ZH_ZMAX, plus 8 for a prescribed velocity), stepped by
+        run_zouhe."""
         cls = np.ascontiguousarray(cls, dtype=np.uint32).reshape(-1)
         assert cls.size == self.n_slots
-        self.flags.upload(cls << abi.NX_CLASS_SHIFT)
+        fl = cls << abi.NX_CLASS_SHIFT
+        if zouhe is not None:
+            zh = np.ascontiguousarray(zouhe, dtype=np.uint32).reshape(-1)
+            assert zh.size == self.n_slots and (zh & 7).max() <= 6 and zh.max() < 16
+            fl = fl | (zh << abi.NX_ZH_SHIFT)
+        self.flags.upload(fl)
         self._fill_and_mask(ulb, ulid, stream)
+
+    def run_zouhe(self, nsteps: int, omega: float, values, stream=None) -> None:
+        """Steps through nx_d3q19_step_dense_zouhe: values[face - 1] is the
+        prescribed density (or inward normal velocity) of that face."""
+        v = (C.c_double * 6)(*[float(x) for x in values])
+        p = params(omega)
+        for _ in range(int(nsteps)):
+            check(lib.nx_d3q19_step_dense_zouhe(C.byref(self.desc), self.fin.ptr, self.fout.ptr,
+                                                self.flags.ptr, C.byref(p), v, stream))
+            self.parity ^= 1
 
     def _fill_and_mask(self, ulb, ulid, stream):
         self._drop_containers()

@@ -20,6 +20,11 @@
  * Project-defined parts with no reference counterpart (marked below):
  * the value of an invalid neighbour on a wall link (0), periodic wrap,
  * Guo body forcing, deviation storage, binary16 rounding.
+ * Zou-He boundary cells follow
+ *   benchmarks/lbm-flow-over-sphere/src/LbmContainers.h:145-287 (streamAndZouhe),
+ *   :351-443 (collideBgkUnrolled with the given rho, u), tables :633-646;
+ * the velocity variant is the textbook one (the reference reads its
+ * prescribed velocity as a density there, :128-133 and :230).
  */
 #include "lbm_oracle.h"
 
@@ -481,7 +486,7 @@ void or_wall_mask_dense(const or_dense* d, const uint32_t* in, uint32_t* out)
                     }
                 }
                 tmp[i] = ((uint32_t)OR_BULK << OR_CLASS_SHIFT) | m | open |
-                         (open && closed ? OR_FLAG_MIXED : 0u);
+                         (open && closed ? OR_FLAG_MIXED : 0u) | (fl & OR_ZH_BITS);
             }
     memcpy(out, tmp, g.plane * sizeof(uint32_t));
     free(tmp);
@@ -551,6 +556,131 @@ void or_step_dense(const or_dense* d, const void* fin, void* fout, const uint32_
                 }
                 pull_dense(&g, d->storage, fin, fl, x, y, z, i, f);
                 collide_any(d->storage, d->compute, f, omega, gg, forced, o);
+                for (int q = 0; q < 19; q++) st(fout, d->storage, q * g.plane + i, o[q]);
+            }
+}
+
+/* ---- Zou-He boundary cells ------------------------------------------------
+ * Unknown directions per face (face 1..6: unknowns coming from -x, +x, -y,
+ * +y, -z, +z) and the 4 middle pairs per axis: LbmContainers.h:633-646. */
+static const int ZH_UNK[6][5] = {{10, 13, 14, 15, 16}, {0, 3, 4, 5, 6},   {11, 4, 13, 17, 18},
+                                 {1, 3, 7, 8, 14},     {12, 6, 8, 15, 17}, {2, 5, 7, 16, 18}};
+static const int ZH_MID[3][4] = {{1, 2, 7, 8}, {0, 2, 5, 6}, {0, 1, 3, 4}};
+
+static int zh_opp(int i) { return i < 9 ? i + 10 : i - 10; }
+
+/* binary64: known sum K (LbmContainers.h:203-215), middle sum M (:217-227),
+ * rho / normal velocity (:229-239), usqr and eq (:242-272, with eqopp =
+ * eq + rho w 6 c.u), unknowns f_i = f_opp + eq_i - eq_opp (:274-286), then
+ * BGK with that rho and u (:351-443). */
+static void zouhe_d(double f[19], int face, int velocity, double value, double omega, double out[19])
+{
+    const int* unk = ZH_UNK[face - 1];
+    const int axis = (face - 1) / 2;
+    double known = 0., middle = 0.;
+    for (int k = 0; k < 5; k++) known += f[zh_opp(unk[k])];
+    for (int k = 0; k < 4; k++) {
+        middle += f[ZH_MID[axis][k]];
+        middle += f[ZH_MID[axis][k] + 10];
+    }
+    middle += f[9];
+    const double sgn = unk[0] < 9 ? 1. : -1.;
+    double rho, ua;
+    if (velocity) {
+        ua = -sgn * value;
+        rho = (middle + known * 2.) / (1. - value);
+    } else {
+        rho = value;
+        ua = (((middle + known * 2.) / rho) - 1.) * sgn;
+    }
+    double u[3] = {0., 0., 0.};
+    u[axis] = ua;
+    const double usqr = 1.5 * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const double ck[9] = {u[0], u[1], u[2], u[0] + u[1], u[0] - u[1], u[0] + u[2], u[0] - u[2],
+                          u[1] + u[2], u[1] - u[2]};
+    double eq[19];
+    for (int i = 0; i < 9; i++) {
+        const double wi = i < 3 ? 1. / 18. : 1. / 36.;
+        eq[i] = rho * wi * (1. - 3. * ck[i] + 4.5 * ck[i] * ck[i] - usqr);
+        eq[i + 10] = eq[i] + rho * wi * 6. * ck[i];
+    }
+    eq[9] = rho * (1. / 3.) * (1. - usqr);
+    for (int k = 0; k < 5; k++) {
+        const int iu = unk[k], ik = zh_opp(iu);
+        f[iu] = f[ik] + eq[iu] - eq[ik];
+    }
+    for (int q = 0; q < 19; q++) out[q] = (1. - omega) * f[q] + omega * eq[q];
+}
+
+/* the same in binary32 (the NX_COMPUTE_F32 collision of binary32 storage) */
+static void zouhe_f(float f[19], int face, int velocity, float value, float omega, float out[19])
+{
+    const int* unk = ZH_UNK[face - 1];
+    const int axis = (face - 1) / 2;
+    float known = 0.f, middle = 0.f;
+    for (int k = 0; k < 5; k++) known += f[zh_opp(unk[k])];
+    for (int k = 0; k < 4; k++) {
+        middle += f[ZH_MID[axis][k]];
+        middle += f[ZH_MID[axis][k] + 10];
+    }
+    middle += f[9];
+    const float sgn = unk[0] < 9 ? 1.f : -1.f;
+    float rho, ua;
+    if (velocity) {
+        ua = -sgn * value;
+        rho = (middle + known * 2.f) / (1.f - value);
+    } else {
+        rho = value;
+        ua = (((middle + known * 2.f) / rho) - 1.f) * sgn;
+    }
+    float u[3] = {0.f, 0.f, 0.f};
+    u[axis] = ua;
+    const float usqr = 1.5f * (u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    const float ck[9] = {u[0], u[1], u[2], u[0] + u[1], u[0] - u[1], u[0] + u[2], u[0] - u[2],
+                         u[1] + u[2], u[1] - u[2]};
+    float eq[19];
+    for (int i = 0; i < 9; i++) {
+        const float wi = i < 3 ? (float)(1. / 18.) : (float)(1. / 36.);
+        eq[i] = rho * wi * (1.f - 3.f * ck[i] + 4.5f * ck[i] * ck[i] - usqr);
+        eq[i + 10] = eq[i] + rho * wi * 6.f * ck[i];
+    }
+    eq[9] = rho * (float)(1. / 3.) * (1.f - usqr);
+    for (int k = 0; k < 5; k++) {
+        const int iu = unk[k], ik = zh_opp(iu);
+        f[iu] = f[ik] + eq[iu] - eq[ik];
+    }
+    for (int q = 0; q < 19; q++) out[q] = (1.f - omega) * f[q] + omega * eq[q];
+}
+
+void or_step_dense_zouhe(const or_dense* d, const void* fin, void* fout, const uint32_t* flags,
+                         double omega, const double value[6])
+{
+    const geo_t g = geo_of(d);
+    const double zero[3] = {0, 0, 0};
+    const int f64 = d->storage == OR_F64;
+#pragma omp parallel for schedule(static) num_threads(or_get_threads())
+    for (int z = 0; z < g.Z; z++)
+        for (int y = 0; y < g.Y; y++)
+            for (int x = 0; x < g.X; x++) {
+                const size_t i = (size_t)x + (size_t)g.X * ((size_t)y + (size_t)g.Y * z);
+                const uint32_t fl = flags[i];
+                if (cls_of(fl) != OR_BULK) continue;
+                double f[19], o[19];
+                pull_dense(&g, d->storage, fin, fl, x, y, z, i, f);
+                const int face = (int)((fl >> OR_ZH_SHIFT) & 7u);
+                const int vel = (fl & OR_ZH_VELOCITY) != 0;
+                if (face >= 1 && face <= 6) {
+                    if (f64) {
+                        zouhe_d(f, face, vel, value[face - 1], omega, o);
+                    } else {
+                        float ff[19], of[19];
+                        for (int q = 0; q < 19; q++) ff[q] = (float)f[q];
+                        zouhe_f(ff, face, vel, (float)value[face - 1], (float)omega, of);
+                        for (int q = 0; q < 19; q++) o[q] = (double)of[q];
+                    }
+                } else {
+                    collide_any(d->storage, d->compute, f, omega, zero, 0, o);
+                }
                 for (int q = 0; q < 19; q++) st(fout, d->storage, q * g.plane + i, o[q]);
             }
 }

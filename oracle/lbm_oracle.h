@@ -29,6 +29,9 @@
 #define OR_MASK_BITS 0x7FDFFu
 #define OR_FLAG_OPEN (1u << 28)
 #define OR_FLAG_MIXED (1u << 29)
+#define OR_ZH_SHIFT 19
+#define OR_ZH_VELOCITY (1u << 22)
+#define OR_ZH_BITS (0xFu << 19)
 #define OR_INVALID_BLOCK 0xFFFFFFFFu
 
 typedef struct {
@@ -66,6 +69,10 @@ void or_fill_dense(const or_dense* d, void* pop, const uint32_t* flags, double u
 void or_wall_mask_dense(const or_dense* d, const uint32_t* in, uint32_t* out);
 void or_step_dense(const or_dense* d, const void* fin, void* fout, const uint32_t* flags,
                    double omega, const double g[3]);
+/* Step with Zou-He boundary cells (bits 19..22 of a bulk cell's flag word;
+ * no body force; storage F64 + compute F64, or F32 + compute F32). */
+void or_step_dense_zouhe(const or_dense* d, const void* fin, void* fout, const uint32_t* flags,
+                         double omega, const double value[6]);
 void or_moments_dense(const or_dense* d, const void* fin, const uint32_t* flags, void* rho,
                       void* u);
 double or_mass_dense(const or_dense* d, const void* fin, const uint32_t* flags);

@@ -22,6 +22,8 @@ CLASS_SHIFT = 24
 MASK_BITS = 0x7FDFF
 FLAG_OPEN = 1 << 28
 FLAG_MIXED = 1 << 29
+ZH_SHIFT = 19
+ZH_VELOCITY = 8          # added to a face code (bit 22 of the flag word)
 NP_STORAGE = {F64: np.float64, F32: np.float32, F32DEV: np.float32, F16DEV: np.uint16}
 
 
@@ -53,6 +55,7 @@ _SIG = {
     "or_fill_dense": (None, [_P, _P, _P, _D, _P]),
     "or_wall_mask_dense": (None, [_P, _P, _P]),
     "or_step_dense": (None, [_P, _P, _P, _P, _D, _P]),
+    "or_step_dense_zouhe": (None, [_P, _P, _P, _P, _D, _P]),
     "or_moments_dense": (None, [_P, _P, _P, _P, _P]),
     "or_mass_dense": (_D, [_P, _P, _P]),
     "or_residual_dense": (None, [_P, _P, _P, _P, _P]),
@@ -126,8 +129,12 @@ class DenseOracle:
         lib.or_cavity_flags_dense(C.byref(self.d), _ptr(self.flags))
         self._fill_and_mask(ulb, ulid)
 
-    def set_classes(self, cls, ulb=0.0, ulid=(1.0, 0.0, 0.0)):
+    def set_classes(self, cls, ulb=0.0, ulid=(1.0, 0.0, 0.0), zouhe=None):
+        """zouhe: optional per-cell Zou-He code (face 1..6, + 8 for a
+        velocity face), stored in bits 19..22 of bulk cells' flag words."""
         self.flags[:] = np.asarray(cls, np.uint32).reshape(-1) << CLASS_SHIFT
+        if zouhe is not None:
+            self.flags[:] |= (np.asarray(zouhe, np.uint32).reshape(-1) & 15) << ZH_SHIFT
         self._fill_and_mask(ulb, ulid)
 
     def _fill_and_mask(self, ulb, ulid):
@@ -145,6 +152,15 @@ class DenseOracle:
             lib.or_step_dense(C.byref(self.d), _ptr(self.pop[self.parity]),
                               _ptr(self.pop[self.parity ^ 1]), _ptr(self.flags), float(omega),
                               _d3(force))
+            self.parity ^= 1
+
+    def run_zouhe(self, nsteps, omega, values):
+        """Steps with Zou-He cells; values[face - 1] = prescribed density or
+        inward normal velocity (LbmContainers.h:145-287 of lbm-flow-over-sphere)."""
+        v = (C.c_double * 6)(*[float(x) for x in values])
+        for _ in range(int(nsteps)):
+            lib.or_step_dense_zouhe(C.byref(self.d), _ptr(self.pop[self.parity]),
+                                    _ptr(self.pop[self.parity ^ 1]), _ptr(self.flags), float(omega), v)
             self.parity ^= 1
 
     def moments(self):
