@@ -249,7 +249,7 @@ def zouhe_duct(nx, a):
                                f"on the x planes, bounce-back side walls, nu = 0.02, {storage}, "
                                f"{steps} timed steps after {warmup}",
                    "fluid_cells": fluid, "zouhe_cells": int((zh > 0).sum()), "device": nx.device_name(),
-                   "centre_ux": float(ux[Y // 2, Y // 2, X // 2])},
+                   "ux_axis_x32": float(ux[Y // 2, Y // 2, 32])},
         "wall_s": round(wall, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
