@@ -72,6 +72,13 @@ ZOUHE = {
 }
 
 
+def dense_kernel_name(t: dict) -> str:
+    """Container step kernel that a dense launch configuration runs."""
+    if t.get("vx", 1) > 1:
+        return "k_fused_vx"
+    return {0: "k_fused", 1: "k_fused_fix", 2: "k_fused_fix[bitmap]", 3: "k_fused_edge"}[t.get("boundary", 0)]
+
+
 def traffic_from_profiles(kernel_tag: str):
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, scaled by a copy kernel of known size), if committed."""
@@ -455,7 +462,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": f"k_fused<{storage},{compute}> ({b_alg} B/fluid cell)"},
+                     "kernel": f"{dense_kernel_name(lat.tuning())}<{storage},{compute}> ({b_alg} B/fluid cell)"},
         "velocity_error_vs_f64": err,
         "cpu_baseline": cpu,
     }
