@@ -7,8 +7,10 @@
 //   forward      M = voxels, N = cout, K = taps*cin      (A: 8 channels of one
 //                tap per lane = one 16-byte load; B: 8 weights, 16 bytes)
 //   bwd-data     the forward on the flipped, transposed weights
-//   bwd-weight   per tap: M = cout, N = cin, K = voxels (split over
-//                workgroups, per-wave FP32 partials, fixed-order reduction)
+//   bwd-weight   M = cout, N = cin, K = voxels (split over workgroups,
+//                FP32 partials, fixed-order reduction): all 27 taps of a
+//                chunk from one LDS staging when rows hold >= 32 voxels
+//                (k_bwd_w27), else one tap per workgroup (k_bwd_w)
 // MFMA 16x16x32 bf16 operand layout: lane l holds row (A) / column (B) l%16
 // and the 8 K values of group l/16; D register r of lane l is row 4*(l/16)+r,
 // column l%16.
@@ -312,20 +314,174 @@ __global__ void __launch_bounds__(256)
                 out[(size_t)(co0 + m * 16 + 4 * grp + r) * cin + ci0 + n * 16 + r16] = acc[m][n][r];
 }
 
-// dw[co][tap][ci] = sum_p part[p][tap][co][ci], p in order.
+// Weight gradient of all 27 taps of a 3^3 convolution for a (16*NM) x 16
+// block of (co, ci), rows of at least 32 voxels (W >= 32).  grid.x = splits,
+// grid.y = channel blocks.  The workgroup walks 32-voxel chunks (one row
+// segment each): it stages the chunk's dy (32 voxels x 16*NM channels) and,
+// for every tap, the input the tap reads (the 9 neighbour row segments, each
+// in the 3 x shifts: 27 tiles of 16 channels x 32 voxels) transposed into
+// LDS [channel][voxel], so every MFMA fragment is one 16-byte LDS read and
+// each input element is fetched from memory once per chunk instead of once
+// per tap.  Wave w accumulates taps w, w+4, ..., w+24; one partial per split:
+// part[split][tap][co][ci].
+template <int NM>
 __global__ void __launch_bounds__(256)
-    k_reduce_w(const float* __restrict__ part, int P, int T, int cin, int cout, float* __restrict__ dw)
+    k_bwd_w27(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
+              float* __restrict__ part, Geo g, int cin, int cout, uint32_t chunks_per_split)
 {
+    constexpr int CM = 16 * NM, CN = 16, T = 27, TW = 7;
+    constexpr int PITCH = 40;                            // halves per LDS row (32 voxels + pad)
+    constexpr int XITEMS = T * 8 * 2;                    // (tile, voxel quad, 8-channel group)
+    constexpr int DITEMS = 8 * (CM / 8);                 // (voxel quad, 8-channel group)
+    __shared__ __attribute__((aligned(16))) uint16_t lx[T * CN * PITCH];
+    __shared__ __attribute__((aligned(16))) uint16_t ld[CM * PITCH];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, grp = lane >> 4;
+    const int mblocks = cout / CM;
+    const int co0 = (int)(blockIdx.y % mblocks) * CM;
+    const int ci0 = (int)(blockIdx.y / mblocks) * CN;
+
+    f32x4 acc[TW][NM];
+#pragma unroll
+    for (int t = 0; t < TW; t++)
+#pragma unroll
+        for (int m = 0; m < NM; m++) acc[t][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // this thread's staging items: x items tid, tid + 256 (< XITEMS), dy item tid (< DITEMS)
+    uint4 qx[2][4], qd[4];
+    auto fetch = [&](uint32_t c) {
+        const uint32_t v0 = c * 32u;
+        const int x0 = (int)(v0 & (uint32_t)(g.W - 1));
+        const int y0 = (int)((v0 >> g.lw) & (uint32_t)(g.H - 1));
+        const int z0 = (int)((v0 >> (g.lw + g.lh)) & (uint32_t)(g.D - 1));
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int it = tid + 256 * k;
+            const int tile = it >> 4, quad = (it >> 1) & 7, cg = it & 1;
+            int dz, dyo, dx;
+            tap_offset<3>(tile, dz, dyo, dx);
+            const bool row = it < XITEMS && (unsigned)(z0 + dz) < (unsigned)g.D &&
+                             (unsigned)(y0 + dyo) < (unsigned)g.H;
+            const int64_t base = (int64_t)v0 + (int64_t)(dz * g.H + dyo) * g.W + dx;
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int pos = 4 * quad + t;
+                const bool ok = row && (unsigned)(x0 + pos + dx) < (unsigned)g.W;
+                qx[k][t] = ok ? *reinterpret_cast<const uint4*>(x + (size_t)(base + pos) * cin + ci0 + 8 * cg)
+                              : uint4{0u, 0u, 0u, 0u};
+            }
+        }
+        {
+            const int quad = tid / (CM / 8), cg = tid % (CM / 8);
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                qd[t] = tid < DITEMS ? *reinterpret_cast<const uint4*>(dy + (size_t)(v0 + 4u * quad + t) * cout +
+                                                                      co0 + 8 * cg)
+                                     : uint4{0u, 0u, 0u, 0u};
+        }
+    };
+    // channel j of two voxels' 8-channel words -> one 32-bit word (voxel a low)
+    auto pair = [](const uint4& a, const uint4& b, int j) {
+        const uint32_t wa = (j >> 1) == 0 ? a.x : (j >> 1) == 1 ? a.y : (j >> 1) == 2 ? a.z : a.w;
+        const uint32_t wb = (j >> 1) == 0 ? b.x : (j >> 1) == 1 ? b.y : (j >> 1) == 2 ? b.z : b.w;
+        return (j & 1) ? ((wa >> 16) | (wb & 0xffff0000u)) : ((wa & 0xffffu) | (wb << 16));
+    };
+    // 4 voxels x 8 channels -> eight 8-byte LDS writes (4 voxels of one channel each)
+    auto put = [&](uint16_t* tile, int ch, int quad, const uint4* q4) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint32_t w2[2] = {pair(q4[0], q4[1], j), pair(q4[2], q4[3], j)};
+            __builtin_memcpy(__builtin_assume_aligned(tile + (ch + j) * PITCH + 4 * quad, 8), w2, 8);
+        }
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int it = tid + 256 * k;
+            if (it < XITEMS) put(lx + (it >> 4) * CN * PITCH, 8 * (it & 1), (it >> 1) & 7, qx[k]);
+        }
+        if (tid < DITEMS) put(ld, 8 * (tid % (CM / 8)), tid / (CM / 8), qd);
+    };
+    auto frag = [&](const uint16_t* row) {
+        uint4 t;
+        __builtin_memcpy(&t, __builtin_assume_aligned(row, 16), 16);
+        return __builtin_bit_cast(bf16x8, t);
+    };
+
+    const uint32_t n_chunks = g.V / 32u;
+    const uint32_t c0 = blockIdx.x * chunks_per_split;
+    const uint32_t c1 = min(n_chunks, c0 + chunks_per_split);
+    if (c0 < c1) fetch(c0);
+    for (uint32_t c = c0; c < c1; c++) {
+        stage();
+        __syncthreads();
+        if (c + 1 < c1) fetch(c + 1);
+        bf16x8 a[NM];
+#pragma unroll
+        for (int m = 0; m < NM; m++) a[m] = frag(ld + (m * 16 + r16) * PITCH + 8 * grp);
+#pragma unroll
+        for (int t = 0; t < TW; t++) {
+            const int tap = wave + 4 * t;
+            if (tap < T) {
+                const bf16x8 b = frag(lx + (tap * CN + r16) * PITCH + 8 * grp);
+#pragma unroll
+                for (int m = 0; m < NM; m++) acc[t][m] = mfma(a[m], b, acc[t][m]);
+            }
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int t = 0; t < TW; t++) {
+        const int tap = wave + 4 * t;
+        if (tap >= T) continue;
+        float* out = part + ((size_t)blockIdx.x * T + tap) * (size_t)cout * cin;
+#pragma unroll
+        for (int m = 0; m < NM; m++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                out[(size_t)(co0 + m * 16 + 4 * grp + r) * cin + ci0 + r16] = acc[t][m][r];
+    }
+}
+
+// dw[co][tap][ci] = sum_p part[p][tap][co][ci] in two fixed-order levels
+// (deterministic, coalesced over the outputs, 64 outputs per workgroup with
+// 4 threads per output adding every 4th partial, the four sums combined in
+// a fixed order):
+//   level 1 (grid.y = G slices of S partials): the sum of slice y replaces
+//            its first partial, part[y*S] (only this workgroup touches it)
+//   level 2: dw = sum over y of part[y*S]
+__global__ void __launch_bounds__(256)
+    k_reduce_w(float* __restrict__ part, int P, int S, int T, int cin, int cout, float* __restrict__ dw,
+               int level)
+{
+    __shared__ float lds[4][64];
     const size_t n = (size_t)cout * T * cin;
-    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+    const int e = threadIdx.x & 63, k0 = threadIdx.x >> 6;
+    const size_t i = (size_t)blockIdx.x * 64 + e;
+    const size_t stride = (size_t)T * cout * cin;
+    size_t at = 0;
+    float s = 0.f;
+    if (i < n) {
         const int ci = (int)(i % cin);
         const int tap = (int)((i / cin) % T);
         const int co = (int)(i / ((size_t)cin * T));
-        const size_t stride = (size_t)T * cout * cin;
-        const float* p = part + ((size_t)tap * cout + co) * cin + ci;
-        float s = 0.f;
-        for (int k = 0; k < P; k++) s += p[(size_t)k * stride];
-        dw[i] = s;
+        at = ((size_t)tap * cout + co) * cin + ci;
+        if (level == 1) {
+            const int k1 = min(P, (int)(blockIdx.y + 1) * S);
+            for (int k = (int)blockIdx.y * S + k0; k < k1; k += 4) s += part[(size_t)k * stride + at];
+        } else {
+            for (int y = k0; y * S < P; y += 4) s += part[(size_t)y * S * stride + at];
+        }
+    }
+    lds[k0][e] = s;
+    __syncthreads();
+    if (k0 == 0 && i < n) {
+        const float t = ((lds[0][e] + lds[1][e]) + lds[2][e]) + lds[3][e];
+        if (level == 1)
+            part[(size_t)blockIdx.y * S * stride + at] = t;
+        else
+            dw[i] = t;
     }
 }
 
@@ -433,25 +589,47 @@ int launch_fwd(const Shape& s, const uint16_t* x, const uint16_t* w, const float
 
 struct WPlan {
     int nm, nn, T;
+    bool all_taps;  // k_bwd_w27: 3^3 kernel, rows of >= 32 voxels
     uint32_t splits, chunks_per_split;
     size_t part_floats;
+    int partials;   // partials per output element (reduced by k_reduce_w)
 };
 
 WPlan plan_w(const Shape& s)
 {
     WPlan p;
     p.T = s.ks * s.ks * s.ks;
+    p.all_taps = s.ks == 3 && s.w >= 32;
+    if (p.all_taps) {
+        p.nm = tiles_of(s.cout / 16, 4);
+        p.nn = 1;
+        const uint64_t V = (uint64_t)s.n * s.d * s.h * s.w;
+        const uint32_t n_chunks = (uint32_t)(V / 32);
+        const uint32_t blocks = (uint32_t)((s.cout / (16 * p.nm)) * (s.cin / 16));
+        // about three workgroups per CU (the kernel's register use allows
+        // three), each walking a contiguous run of chunks
+        uint32_t splits = (768u + blocks - 1) / blocks;
+        // at most 16 M partial floats (64 MB of workspace)
+        const uint32_t cap = (uint32_t)std::max<uint64_t>(1, (16ull << 20) / ((uint64_t)27 * s.cout * s.cin));
+        splits = std::max(1u, std::min({splits, cap, n_chunks}));
+        p.chunks_per_split = (n_chunks + splits - 1) / splits;
+        p.splits = (n_chunks + p.chunks_per_split - 1) / p.chunks_per_split;
+        p.part_floats = (size_t)p.splits * 27 * s.cout * s.cin;
+        p.partials = (int)p.splits;
+        return p;
+    }
     p.nm = tiles_of(s.cout / 16, 4);
     p.nn = tiles_of(s.cin / 16, 4);
     const uint64_t V = (uint64_t)s.n * s.d * s.h * s.w;
     const uint32_t n_chunks = (uint32_t)((V + 31) / 32);
     const uint32_t blocks = (uint32_t)(p.T * (s.cout / (16 * p.nm)) * (s.cin / (16 * p.nn)));
     uint32_t splits = (2048u + blocks - 1) / blocks;
-    splits = std::max(1u, std::min({splits, 64u, (n_chunks + 3u) / 4u}));
+    splits = std::max(1u, std::min({splits, 512u, (n_chunks + 3u) / 4u}));
     p.chunks_per_split = (n_chunks + splits - 1) / splits;
     p.splits = (n_chunks + p.chunks_per_split - 1) / p.chunks_per_split;
     if (p.splits == 0) p.splits = 1;
     p.part_floats = (size_t)p.splits * 4 * p.T * s.cout * s.cin;
+    p.partials = (int)(p.splits * 4);
     return p;
 }
 
@@ -513,8 +691,22 @@ int bwd_weight(const Shape& s, const uint16_t* x, const uint16_t* dy, float* dw,
     const Geo g = geo_of(s);
     const WPlan p = plan_w(s);
     float* part = static_cast<float*>(ws);
-    const dim3 grid(p.T * p.splits, (unsigned)((s.cout / (16 * p.nm)) * (s.cin / (16 * p.nn))));
-    int err = by_ks(s.ks, [&](auto KS) {
+    const dim3 grid(p.all_taps ? p.splits : p.T * p.splits,
+                    (unsigned)((s.cout / (16 * p.nm)) * (s.cin / (16 * p.nn))));
+    if (p.all_taps) {
+        auto go27 = [&](auto NM) {
+            hipLaunchKernelGGL((k_bwd_w27<decltype(NM)::value>), grid, dim3(256), 0, st, x, dy, part, g,
+                               s.cin, s.cout, p.chunks_per_split);
+        };
+        switch (p.nm) {
+            case 4: go27(std::integral_constant<int, 4>{}); break;
+            case 3: go27(std::integral_constant<int, 3>{}); break;
+            case 2: go27(std::integral_constant<int, 2>{}); break;
+            default: go27(std::integral_constant<int, 1>{}); break;
+        }
+        NX_CHECK_LAUNCH("k_bwd_w27");
+    }
+    int err = p.all_taps ? 0 : by_ks(s.ks, [&](auto KS) {
         auto go = [&](auto NM, auto NN) {
             hipLaunchKernelGGL((k_bwd_w<decltype(NM)::value, decltype(NN)::value, KS>), grid,
                                dim3(256), 0, st, x, dy, part, g, s.cin, s.cout, p.chunks_per_split);
@@ -537,8 +729,14 @@ int bwd_weight(const Shape& s, const uint16_t* x, const uint16_t* dy, float* dw,
     if (err) return err;
     NX_CHECK_LAUNCH("k_bwd_w");
     const size_t nw = (size_t)s.cout * p.T * s.cin;
-    hipLaunchKernelGGL(k_reduce_w, dim3((unsigned)std::min<size_t>((nw + 255) / 256, 4096)), dim3(256),
-                       0, st, part, (int)(p.splits * 4), p.T, s.cin, s.cout, dw);
+    // level 1 in slices of S partials, then the G slice sums
+    const int G = std::max(1, std::min(64, p.partials / 32));
+    const int S = (p.partials + G - 1) / G;
+    const unsigned gx = (unsigned)((nw + 63) / 64);
+    hipLaunchKernelGGL(k_reduce_w, dim3(gx, (unsigned)((p.partials + S - 1) / S)), dim3(256), 0, st, part,
+                       p.partials, S, p.T, s.cin, s.cout, dw, 1);
+    hipLaunchKernelGGL(k_reduce_w, dim3(gx), dim3(256), 0, st, part, p.partials, S, p.T, s.cin, s.cout,
+                       dw, 2);
     NX_CHECK_LAUNCH("k_reduce_w");
     if (db) {
         // the column partials reuse the workspace after the weight reduction

@@ -80,7 +80,13 @@ def test_conv_bwd_data_vs_oracle(nx, shape, cin, cout, ks):
     assert np.all(np.abs(got - ref) <= 1e-5 * bound + 1e-30)
 
 
-@pytest.mark.parametrize("shape,cin,cout,ks", CASES)
+# rows of >= 32 voxels: the all-taps weight-gradient kernel (k_bwd_w27),
+# including 1, 3 and 4 output-channel tiles and several input-channel blocks
+WIDE = [((1, 4, 4, 32), 16, 16, 3), ((2, 2, 4, 64), 48, 16, 3), ((1, 4, 2, 32), 32, 64, 3),
+        ((1, 2, 2, 32), 16, 48, 3), ((2, 2, 2, 64), 96, 32, 3)]
+
+
+@pytest.mark.parametrize("shape,cin,cout,ks", CASES + WIDE)
 def test_conv_bwd_weight_vs_oracle(nx, shape, cin, cout, ks):
     from neon_amd import surrogate as S
     rng = np.random.default_rng(5 * cin + cout)
@@ -122,6 +128,7 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("levels,widths,size,batch", [(4, (16, 32, 64, 64), 8, 2),
                                                       (3, (16, 32, 48), 16, 1),
+                                                      (2, (16, 32), 32, 1),
                                                       (2, (32, 16), 4, 3)])
 def test_unet_forward_backward_vs_oracle(nx, levels, widths, size, batch):
     from neon_amd import surrogate as S
