@@ -228,6 +228,13 @@ int nx_d3q19_moments_dense(const nx_dense_desc* d, const void* fin,
 int nx_d3q19_iteration_dense(const nx_dense_desc* d, const void* fin, void* fout,
                              const uint32_t* flags, const nx_lbm_params* p, nx_container* c);
 int nx_container_run(nx_container c, nx_stream s);
+/* n_steps of the ping-pong pair a, b, a, b, ... (b steps b's input, which is
+ * a's output, back into a's input; LbmIterationD3Q19::run n times,
+ * LbmIteration.h:56-69).  On a created stream, every 64 steps replay one
+ * hipGraph captured at the first call (rebuilt when the launch
+ * configuration changes); on the default stream each step is launched.
+ * Not for concurrent use of the same containers from several threads. */
+int nx_container_run_pair(nx_container a, nx_container b, int32_t n_steps, nx_stream s);
 int nx_container_info(nx_container c, uint64_t* n_interior, uint64_t* n_boundary,
                       int* copies_walls);
 int nx_container_destroy(nx_container c);

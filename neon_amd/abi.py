@@ -105,6 +105,7 @@ SIGNATURES = {
     "nx_d3q19_set_tuning_dense": (_I, [_DP, C.POINTER(Tuning)]),
     "nx_d3q19_iteration_dense": (_I, [_DP, _P, _P, _P, _PP, C.POINTER(_P)]),
     "nx_container_run": (_I, [_P, _P]),
+    "nx_container_run_pair": (_I, [_P, _P, _I32, _P]),
     "nx_container_info": (_I, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                C.POINTER(C.c_int)]),
     "nx_container_destroy": (_I, [_P]),

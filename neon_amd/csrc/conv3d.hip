@@ -324,10 +324,13 @@ __global__ void __launch_bounds__(256)
 // each input element is fetched from memory once per chunk instead of once
 // per tap.  Wave w accumulates taps w, w+4, ..., w+24; one partial per split:
 // part[split][tap][co][ci].
+// With part_b, the workgroups of the first input-channel block also sum the
+// staged dy over their voxels (bias gradient): part_b[split][co].
 template <int NM>
 __global__ void __launch_bounds__(256)
     k_bwd_w27(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
-              float* __restrict__ part, Geo g, int cin, int cout, uint32_t chunks_per_split)
+              float* __restrict__ part, float* __restrict__ part_b, Geo g, int cin, int cout,
+              uint32_t chunks_per_split)
 {
     constexpr int CM = 16 * NM, CN = 16, T = 27, TW = 7;
     constexpr int PITCH = 40;                            // halves per LDS row (32 voxels + pad)
@@ -411,11 +414,15 @@ __global__ void __launch_bounds__(256)
     const uint32_t n_chunks = g.V / 32u;
     const uint32_t c0 = blockIdx.x * chunks_per_split;
     const uint32_t c1 = min(n_chunks, c0 + chunks_per_split);
+    const bool bias = part_b != nullptr && ci0 == 0 && tid < CM;
+    float sb = 0.f;
     if (c0 < c1) fetch(c0);
     for (uint32_t c = c0; c < c1; c++) {
         stage();
         __syncthreads();
         if (c + 1 < c1) fetch(c + 1);
+        if (bias)
+            for (int v = 0; v < 32; v++) sb += __builtin_bit_cast(float, (uint32_t)ld[tid * PITCH + v] << 16);
         bf16x8 a[NM];
 #pragma unroll
         for (int m = 0; m < NM; m++) a[m] = frag(ld + (m * 16 + r16) * PITCH + 8 * grp);
@@ -431,6 +438,7 @@ __global__ void __launch_bounds__(256)
         __syncthreads();
     }
 
+    if (bias) part_b[(size_t)blockIdx.x * cout + co0 + tid] = sb;
 #pragma unroll
     for (int t = 0; t < TW; t++) {
         const int tap = wave + 4 * t;
@@ -682,6 +690,8 @@ size_t bwd_weight_ws(const Shape& s)
 {
     const WPlan p = plan_w(s);
     const size_t cols = (size_t)kColBlocks * s.cout;
+    // all-taps plan: weight partials, then the bias partials (splits x cout)
+    if (p.all_taps) return (p.part_floats + (size_t)p.splits * s.cout) * sizeof(float);
     return (std::max(p.part_floats, cols)) * sizeof(float);
 }
 
@@ -695,8 +705,8 @@ int bwd_weight(const Shape& s, const uint16_t* x, const uint16_t* dy, float* dw,
                     (unsigned)((s.cout / (16 * p.nm)) * (s.cin / (16 * p.nn))));
     if (p.all_taps) {
         auto go27 = [&](auto NM) {
-            hipLaunchKernelGGL((k_bwd_w27<decltype(NM)::value>), grid, dim3(256), 0, st, x, dy, part, g,
-                               s.cin, s.cout, p.chunks_per_split);
+            hipLaunchKernelGGL((k_bwd_w27<decltype(NM)::value>), grid, dim3(256), 0, st, x, dy, part,
+                               db ? part + p.part_floats : nullptr, g, s.cin, s.cout, p.chunks_per_split);
         };
         switch (p.nm) {
             case 4: go27(std::integral_constant<int, 4>{}); break;
@@ -738,7 +748,11 @@ int bwd_weight(const Shape& s, const uint16_t* x, const uint16_t* dy, float* dw,
     hipLaunchKernelGGL(k_reduce_w, dim3(gx), dim3(256), 0, st, part, p.partials, S, p.T, s.cin, s.cout,
                        dw, 2);
     NX_CHECK_LAUNCH("k_reduce_w");
-    if (db) {
+    if (db && p.all_taps) {
+        hipLaunchKernelGGL(k_reduce_cols, dim3(s.cout), dim3(256), 0, st, part + p.part_floats,
+                           (int)p.splits, s.cout, db);
+        NX_CHECK_LAUNCH("k_reduce_cols");
+    } else if (db) {
         // the column partials reuse the workspace after the weight reduction
         const uint32_t per = (g.V + kColBlocks - 1) / kColBlocks;
         hipLaunchKernelGGL(k_colsum, dim3(kColBlocks), dim3(256), 0, st, dy, g.V, s.cout, per, part);

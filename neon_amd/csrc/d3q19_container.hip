@@ -28,6 +28,7 @@
 // creation; if any wall value differs it leaves wall cells unwritten.
 // Both paths evaluate the same expression tree as nx_d3q19_step_dense, so
 // the populations agree bit for bit.
+#include <cstring>
 #include <vector>
 
 #include "dense_common.hpp"
@@ -44,6 +45,10 @@ struct nx_container_s {
     uint64_t n_boundary = 0;       // bulk cells on the full-pull path
     uint64_t n_interior = 0;
     int copy_walls = 0;            // 1: fin and fout wall values are identical
+    // graph of 2 * kPairSteps steps alternating this container and `pair_with`
+    hipGraphExec_t pair_graph = nullptr;
+    const nx_container_s* pair_with = nullptr;
+    nx_tuning pair_tuning[2] = {};
 };
 
 namespace nx {
@@ -714,8 +719,61 @@ extern "C" int nx_container_info(nx_container c, uint64_t* n_interior, uint64_t*
 extern "C" int nx_container_destroy(nx_container c)
 {
     if (!c) return NX_OK;
+    if (c->pair_graph) hipGraphExecDestroy(c->pair_graph);
     if (c->interior) hipFree(c->interior);
     delete c;
+    return NX_OK;
+}
+
+// Ping-pong steps a, b, a, b, ... (LbmIterationD3Q19::run called n times,
+// LbmIteration.h:56-69).  On a created stream, runs of 2 * kPairSteps steps
+// replay one hipGraph (built at the first call for this pair and launch
+// configuration, kept in `a`): one launch instead of 64 kernel launches.
+// The default stream cannot be captured: there every step is launched.
+namespace {
+constexpr int kPairSteps = 32;
+}
+
+extern "C" int nx_container_run_pair(nx_container a, nx_container b, int32_t n_steps, nx_stream s)
+{
+    if (!a || !b) return set_error(NX_ERR_INVALID, "null container");
+    if (n_steps < 0) return set_error(NX_ERR_INVALID, "n_steps must be >= 0");
+    if (a->d.storage != b->d.storage || a->d.compute != b->d.compute || a->fin != b->fout ||
+        a->fout != b->fin)
+        return set_error(NX_ERR_INVALID, "run_pair: b must step b's input back into a's input");
+    const hipStream_t st = (hipStream_t)s;
+    const nx_tuning& t = g_tuning[a->d.storage][a->d.compute];
+    int32_t done = 0;
+    if (st != nullptr && n_steps >= 2 * kPairSteps) {
+        const bool same = a->pair_graph && a->pair_with == b &&
+                          !memcmp(&a->pair_tuning[0], &t, sizeof(t));
+        if (!same) {
+            if (a->pair_graph) hipGraphExecDestroy(a->pair_graph);
+            a->pair_graph = nullptr;
+            hipGraph_t g;
+            NX_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            int e = 0;
+            for (int k = 0; k < kPairSteps && !e; k++) {
+                e = run_container(*a, t, st);
+                if (!e) e = run_container(*b, t, st);
+            }
+            const hipError_t ec = hipStreamEndCapture(st, &g);
+            if (e) {
+                if (ec == hipSuccess) hipGraphDestroy(g);
+                return e;
+            }
+            NX_CHECK_HIP(ec);
+            const hipError_t ei = hipGraphInstantiate(&a->pair_graph, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            NX_CHECK_HIP(ei);
+            a->pair_with = b;
+            a->pair_tuning[0] = t;
+        }
+        for (; done + 2 * kPairSteps <= n_steps; done += 2 * kPairSteps)
+            NX_CHECK_HIP(hipGraphLaunch(a->pair_graph, st));
+    }
+    for (; done < n_steps; done++)
+        if (int e = run_container(done % 2 == 0 ? *a : *b, t, st)) return e;
     return NX_OK;
 }
 

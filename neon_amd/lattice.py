@@ -343,10 +343,12 @@ class DenseLattice(_Lattice):
         return int(a.value), int(b.value), bool(w.value)
 
     def run(self, nsteps: int, omega: float, force=(0.0, 0.0, 0.0), stream=None) -> None:
+        """nsteps container steps in one call (hipGraph replays on a created
+        stream, LbmIterationD3Q19::run)."""
         cs = self._containers(omega, force)
-        for _ in range(int(nsteps)):
-            check(lib.nx_container_run(cs[self.parity], stream))
-            self.parity ^= 1
+        n = int(nsteps)
+        check(lib.nx_container_run_pair(cs[self.parity], cs[self.parity ^ 1], n, stream))
+        self.parity ^= n & 1
 
     def run_plain(self, nsteps: int, omega: float, force=(0.0, 0.0, 0.0), stream=None) -> None:
         """Steps through nx_d3q19_step_dense (no container)."""

@@ -350,3 +350,39 @@ def test_container_single_collision_bitmap_equals_plain_step(nx, st, cm, boundar
         a.set_tuning(**t0)
     b.run_plain(7, 1.5, force=force)
     assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
+
+
+@pytest.mark.parametrize("st,cm", [("f32", "f32"), ("f16dev", "f32"), ("f64", "f64")])
+def test_container_graph_replays_equal_eager_steps(nx, st, cm):
+    """Runs of 64 steps replay one captured hipGraph on a created stream;
+    replays separated by eager work (mass reductions, a tuning change that
+    forces a new capture, odd step counts) give the populations of the
+    per-step launches on the default stream."""
+    N = 40
+    s = nx.Stream()
+    g = nx.DenseLattice((N, N, N), st, cm)
+    h = nx.DenseLattice((N, N, N), st, cm)
+    g.setup_cavity(0.05)
+    h.setup_cavity(0.05)
+    omega = 1.75
+    g.run(131, omega, stream=s.handle)           # 2 replays + 3 launches
+    s.sync()
+    h.run(131, omega)
+    assert np.array_equal(g.populations().view(np.uint8), h.populations().view(np.uint8))
+    m0 = g.mass()
+    t0 = g.tuning()
+    try:
+        alt = {**t0, "boundary": 1 if t0["boundary"] != 1 else 2}
+        g.set_tuning(**alt)
+        h.set_tuning(**alt)
+        g.run(200, omega, stream=s.handle)       # recaptured under the new launch configuration
+        s.sync()
+        h.run(200, omega)
+    finally:
+        g.set_tuning(**t0)
+    assert np.array_equal(g.populations().view(np.uint8), h.populations().view(np.uint8))
+    g.run(129, omega, stream=s.handle)
+    s.sync()
+    h.run(129, omega)
+    assert np.array_equal(g.populations().view(np.uint8), h.populations().view(np.uint8))
+    assert abs(g.mass() - m0) < 1e-3 * m0
