@@ -5,7 +5,9 @@
 // All three are implicit GEMMs fed straight from global memory (the 27-fold
 // reuse of every input voxel is served by L1/L2):
 //   forward      M = voxels, N = cout, K = taps*cin      (A: 8 channels of one
-//                tap per lane = one 16-byte load; B: 8 weights, 16 bytes)
+//                tap per lane = one 16-byte load; B: 8 weights, 16 bytes);
+//                for rows of >= 32 voxels the input block with its halo is
+//                staged in LDS, 16 channels at a time (k_fwd_tile)
 //   bwd-data     the forward on the flipped, transposed weights
 //   bwd-weight   M = cout, N = cin, K = voxels (split over workgroups,
 //                FP32 partials, fixed-order reduction): all 27 taps of a
@@ -163,6 +165,127 @@ __global__ void __launch_bounds__(256)
                 else
                     static_cast<uint16_t*>(y)[(size_t)v * cout + co] = f2bf(s);
             }
+    }
+}
+
+// 3^3 forward (and backward-data) with the input tile in LDS, for rows of
+// >= 32 voxels.  A workgroup computes a 32 x 4 x 4 block of output voxels
+// (x, y, z).  For each slice of 16 input channels it stages the block's input
+// with its one-voxel halo, 34 x 6 x 6 voxels x 16 channels (39 KB), in
+// channels-last order (one 16-byte LDS write per load); each wave then
+// takes 4 rows of 32 voxels, reads every A fragment (8 channels of one tap
+// for one voxel) with one ds_read_b128, and uses each weight fragment for
+// all of its rows.  Each input voxel is read from memory ~2x instead of 27x.
+// CINC = 16 fixes the channel count (one slice, fewer registers); 0 reads it
+// from the argument.
+template <int NT, int CINC>
+__global__ void __launch_bounds__(256)
+    k_fwd_tile(const uint16_t* __restrict__ x, const uint16_t* __restrict__ w,
+               const float* __restrict__ bias, void* __restrict__ y, Geo g, int cin_arg, int cout, int flags)
+{
+    const int cin = CINC ? CINC : cin_arg;           // CINC: channel count fixed at compile time
+    constexpr int TX = 32, TY = 4, TZ = 4, HX = TX + 2, HY = TY + 2, HZ = TZ + 2, CS = 16;
+    constexpr int RW = TY * TZ / 4;                  // rows per wave
+    constexpr int KS_ = 27 * CS, KC = (KS_ + 31) / 32;
+    constexpr int ITEMS = HX * HY * HZ * (CS / 8);   // 16-byte pieces of one slice's halo tile
+    __shared__ __attribute__((aligned(16))) uint16_t tile[HX * HY * HZ * CS];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r16 = lane & 15, grp = lane >> 4;
+    const int co0 = (int)blockIdx.y * NT * 16;
+    const int K = 27 * cin;
+    // tile origin: blocks x fastest, then y, then z (z spans batch * depth)
+    const int nbx = g.W / TX, nby = g.H / TY, nbz = g.D / TZ;
+    const int bx = (int)(blockIdx.x % (uint32_t)nbx);
+    const int by = (int)((blockIdx.x / (uint32_t)nbx) % (uint32_t)nby);
+    const int bzall = (int)(blockIdx.x / (uint32_t)(nbx * nby));   // (sample, z block)
+    const int sample = bzall / nbz, z0 = (bzall % nbz) * TZ, y0 = by * TY, x0 = bx * TX;
+    const size_t sample_base = (size_t)sample * g.D * g.H * g.W;
+
+    const uint16_t* wrow[NT];
+#pragma unroll
+    for (int n = 0; n < NT; n++) wrow[n] = w + (size_t)(co0 + n * 16 + r16) * K;
+    f32x4 acc[RW][2][NT];
+#pragma unroll
+    for (int j = 0; j < RW; j++)
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int n = 0; n < NT; n++) acc[j][t][n] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    for (int c0 = 0; c0 < cin; c0 += CS) {
+        if (c0) __syncthreads();                     // the previous slice's reads are done
+        for (int it = tid; it < ITEMS; it += 256) {
+            const int piece = it & 1, hv = it >> 1;
+            const int hx = hv % HX, hy = (hv / HX) % HY, hz = hv / (HX * HY);
+            const int gx = x0 - 1 + hx, gy = y0 - 1 + hy, gz = z0 - 1 + hz;
+            uint4 v = uint4{0u, 0u, 0u, 0u};
+            if ((unsigned)gx < (unsigned)g.W && (unsigned)gy < (unsigned)g.H && (unsigned)gz < (unsigned)g.D)
+                v = *reinterpret_cast<const uint4*>(
+                    x + (sample_base + ((size_t)gz * g.H + gy) * g.W + gx) * cin + c0 + 8 * piece);
+            *reinterpret_cast<uint4*>(tile + (size_t)hv * CS + 8 * piece) = v;
+        }
+        __syncthreads();
+
+        // K step kc of the slice: lane group grp holds tap (32 kc + 8 grp) / 16,
+        // channels c0 + (8 grp) % 16 ..+7; past the last tap it contributes
+        // zeros -- the MFMAs run for every lane (a lane left out of one would
+        // not receive its accumulator rows)
+        auto fetch_b = [&](int kc, bf16x8* b) {
+            const int kk = kc * 32 + 8 * grp;
+            const int tap = kk / CS, ci = kk % CS;
+#pragma unroll
+            for (int n = 0; n < NT; n++)
+                b[n] = kk < KS_ ? ld8(wrow[n] + tap * cin + c0 + ci) : bf16x8{};
+        };
+        bf16x8 b0[NT], b1[NT];
+        fetch_b(0, b0);
+        for (int kc = 0; kc < KC; kc++) {
+            if (kc + 1 < KC) fetch_b(kc + 1, b1);
+            const int kk = kc * 32 + 8 * grp;
+            const bool kok = kk < KS_;
+            const int tap = kok ? kk / CS : 26, ci = kok ? kk % CS : 0;
+            int dz, dyo, dx;
+            tap_offset<3>(tap, dz, dyo, dx);
+#pragma unroll
+            for (int j = 0; j < RW; j++) {
+                const int row = wave * RW + j;
+                const int ly = row % TY, lz = row / TY;
+#pragma unroll
+                for (int t = 0; t < 2; t++) {
+                    const int hv = ((lz + dz + 1) * HY + (ly + dyo + 1)) * HX + (16 * t + r16 + dx + 1);
+                    const bf16x8 a = kok ? ld8(tile + (size_t)hv * CS + ci) : bf16x8{};
+#pragma unroll
+                    for (int n = 0; n < NT; n++) acc[j][t][n] = mfma(a, b0[n], acc[j][t][n]);
+                }
+            }
+#pragma unroll
+            for (int n = 0; n < NT; n++) b0[n] = b1[n];
+        }
+    }
+
+    const bool relu = flags & kRelu, f32out = flags & kOutF32;
+#pragma unroll
+    for (int j = 0; j < RW; j++) {
+        const int row = wave * RW + j;
+        const int ly = row % TY, lz = row / TY;
+        const size_t vrow = sample_base + ((size_t)(z0 + lz) * g.H + (y0 + ly)) * g.W + x0;
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            const int co = co0 + n * 16 + r16;
+            const float bv = bias ? bias[co] : 0.f;
+#pragma unroll
+            for (int t = 0; t < 2; t++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const size_t v = vrow + 16 * t + 4 * grp + r;
+                    float sv = acc[j][t][n][r] + bv;
+                    if (relu) sv = sv > 0.f ? sv : 0.f;
+                    if (f32out)
+                        static_cast<float*>(y)[v * cout + co] = sv;
+                    else
+                        static_cast<uint16_t*>(y)[v * cout + co] = f2bf(sv);
+                }
+        }
     }
 }
 
@@ -581,6 +704,28 @@ int launch_fwd(const Shape& s, const uint16_t* x, const uint16_t* w, const float
 {
     const Geo g = geo_of(s);
     if (g.V == 0) return NX_OK;
+    // LDS-tiled kernel: 3^3, rows of >= 32 voxels, blocks of 32 x 4 x 4
+    if (s.ks == 3 && s.w >= 32 && s.h >= 4 && s.d >= 4) {
+        const int nt = tiles_of(s.cout / 16, 4);
+        const dim3 grid((unsigned)((size_t)s.n * (s.d / 4) * (s.h / 4) * (s.w / 32)),
+                        (unsigned)(s.cout / (16 * nt)));
+        auto go = [&](auto NTc) {
+            constexpr int NT = decltype(NTc)::value;
+            if (s.cin == 16)
+                hipLaunchKernelGGL((k_fwd_tile<NT, 16>), grid, dim3(256), 0, st, x, w, bias, y, g, s.cin, s.cout,
+                                   flags);
+            else
+                hipLaunchKernelGGL((k_fwd_tile<NT, 0>), grid, dim3(256), 0, st, x, w, bias, y, g, s.cin, s.cout,
+                                   flags);
+        };
+        switch (nt) {
+            case 4: go(std::integral_constant<int, 4>{}); break;
+            case 3: go(std::integral_constant<int, 3>{}); break;
+            case 2: go(std::integral_constant<int, 2>{}); break;
+            default: go(std::integral_constant<int, 1>{}); break;
+        }
+        return NX_OK;
+    }
     const int nt = tiles_of(s.cout / 16, 8);
     const dim3 grid((g.V + 127u) / 128u, (unsigned)(s.cout / (16 * nt)));
     return by_ks(s.ks, [&](auto KS) {

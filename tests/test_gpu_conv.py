@@ -34,8 +34,18 @@ CASES = [  # (shape n,d,h,w), cin, cout, ks
     ((3, 4, 2, 4), 96, 48, 3),
 ]
 
+# rows of >= 32 voxels: the all-taps weight-gradient kernel (k_bwd_w27),
+# including 1, 3 and 4 output-channel tiles and several input-channel blocks
+WIDE = [((1, 4, 4, 32), 16, 16, 3), ((2, 2, 4, 64), 48, 16, 3), ((1, 4, 2, 32), 32, 64, 3),
+        ((1, 2, 2, 32), 16, 48, 3), ((2, 2, 2, 64), 96, 32, 3)]
+# blocks of 32 x 4 x 4 voxels: the LDS-tiled forward, 16 input channels at a
+# time (k_fwd_tile; also the backward-data pass of these layers)
+TILED = [((1, 4, 4, 32), 16, 48, 3), ((2, 4, 4, 32), 32, 64, 3), ((1, 8, 8, 64), 16, 16, 3),
+         ((1, 4, 4, 64), 32, 32, 3), ((1, 4, 4, 32), 48, 32, 3), ((2, 4, 4, 64), 48, 16, 3),
+         ((1, 4, 8, 32), 96, 32, 3)]
 
-@pytest.mark.parametrize("shape,cin,cout,ks", CASES)
+
+@pytest.mark.parametrize("shape,cin,cout,ks", CASES + WIDE + TILED)
 def test_conv_fwd_f32_vs_oracle(nx, shape, cin, cout, ks):
     from neon_amd import surrogate as S
     rng = np.random.default_rng(cin * 7 + cout)
@@ -50,7 +60,7 @@ def test_conv_fwd_f32_vs_oracle(nx, shape, cin, cout, ks):
     assert np.all(np.abs(got - ref) <= 1e-5 * bound + 1e-30)
 
 
-@pytest.mark.parametrize("shape,cin,cout,ks", CASES[:4])
+@pytest.mark.parametrize("shape,cin,cout,ks", CASES[:4] + TILED[:2])
 def test_conv_fwd_relu_bf16(nx, shape, cin, cout, ks):
     from neon_amd import surrogate as S
     rng = np.random.default_rng(11)
@@ -66,7 +76,7 @@ def test_conv_fwd_relu_bf16(nx, shape, cin, cout, ks):
     assert np.all(got >= 0)
 
 
-@pytest.mark.parametrize("shape,cin,cout,ks", CASES)
+@pytest.mark.parametrize("shape,cin,cout,ks", CASES + WIDE + TILED)
 def test_conv_bwd_data_vs_oracle(nx, shape, cin, cout, ks):
     from neon_amd import surrogate as S
     rng = np.random.default_rng(cin + 3 * cout)
@@ -78,12 +88,6 @@ def test_conv_bwd_data_vs_oracle(nx, shape, cin, cout, ks):
     ref = U.conv_bwd_data(g, w, ks)
     bound = U.conv_bwd_data(np.abs(g), np.abs(w), ks)
     assert np.all(np.abs(got - ref) <= 1e-5 * bound + 1e-30)
-
-
-# rows of >= 32 voxels: the all-taps weight-gradient kernel (k_bwd_w27),
-# including 1, 3 and 4 output-channel tiles and several input-channel blocks
-WIDE = [((1, 4, 4, 32), 16, 16, 3), ((2, 2, 4, 64), 48, 16, 3), ((1, 4, 2, 32), 32, 64, 3),
-        ((1, 2, 2, 32), 16, 48, 3), ((2, 2, 2, 64), 96, 32, 3)]
 
 
 @pytest.mark.parametrize("shape,cin,cout,ks", CASES + WIDE)
