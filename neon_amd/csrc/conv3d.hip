@@ -2,8 +2,11 @@
 // matrix cores, FP32 accumulation (the learned warm start's network).  No
 // counterpart in the reference.
 //
-// All three are implicit GEMMs fed straight from global memory (the 27-fold
-// reuse of every input voxel is served by L1/L2):
+// All three are implicit GEMMs.  On grids whose rows hold >= 32 voxels the
+// operands are staged in LDS (the input block with its halo for the forward,
+// a row chunk's 27 shifted tiles for the weight gradient); smaller grids
+// read their fragments straight from global memory (the 27-fold reuse of
+// every input voxel served by L1/L2):
 //   forward      M = voxels, N = cout, K = taps*cin      (A: 8 channels of one
 //                tap per lane = one 16-byte load; B: 8 weights, 16 bytes);
 //                for rows of >= 32 voxels the input block with its halo is
