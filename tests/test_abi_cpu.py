@@ -46,6 +46,22 @@ def test_argument_errors_without_device_work():
     b = abi.BlockDesc()
     assert abi.lib.nx_d3q19_step_block(C.byref(b), None, None, None, C.byref(p),
                                        None) == abi.NX_ERR_INVALID
+    # Zou-He step: deviation storage and a body force are refused up front
+    d.dim[0] = 8
+    vals = (C.c_double * 6)(1.0, 1.0, 0, 0, 0, 0)
+    d.storage, d.compute = abi.NX_F32DEV, abi.NX_COMPUTE_F32
+    assert abi.lib.nx_d3q19_step_dense_zouhe(C.byref(d), C.c_void_p(16), C.c_void_p(32), C.c_void_p(64),
+                                             C.byref(p), vals, None) == abi.NX_ERR_UNSUPPORTED
+    d.storage = abi.NX_F32
+    pf = abi.LbmParams()
+    pf.force[0] = 1e-5
+    assert abi.lib.nx_d3q19_step_dense_zouhe(C.byref(d), C.c_void_p(16), C.c_void_p(32), C.c_void_p(64),
+                                             C.byref(pf), vals, None) == abi.NX_ERR_UNSUPPORTED
+    assert abi.lib.nx_d3q19_step_dense_zouhe(C.byref(d), C.c_void_p(16), C.c_void_p(32), C.c_void_p(64),
+                                             C.byref(p), None, None) == abi.NX_ERR_INVALID
+    # container pair / batch variant: null handles and bad arguments
+    assert abi.lib.nx_container_run_pair(None, None, 4, None) == abi.NX_ERR_INVALID
+    assert abi.lib.nx_batch_set_variant(None, 1, 1) == abi.NX_ERR_INVALID
     try:
         abi.check(abi.NX_ERR_INVALID)
     except abi.NeonError as e:
