@@ -352,6 +352,57 @@ def test_container_single_collision_bitmap_equals_plain_step(nx, st, cm, boundar
     assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
 
 
+LONG_ROW_COMBOS = [("f16dev", "f32", 3, 1), ("f32", "f32", 1, 1), ("f32dev", "f32", 2, 1), ("f64", "f64", 0, 0)]
+
+
+@pytest.mark.parametrize("dim", [(100, 7, 6), (128, 6, 5), (192, 5, 6)])
+@pytest.mark.parametrize("st,cm,sto,cmo", LONG_ROW_COMBOS)
+def test_container_edge_long_rows_cavity(nx, orc, dim, st, cm, sto, cmo):
+    """boundary = 3 with rows of >= 64 cells, as at the benchmark size: the
+    x-end cells sit in the first and last wavefront of a row (rows of 100
+    also put both ends of a row boundary in one wavefront); bit for bit
+    against the oracle."""
+    g, o = _pair(nx, orc, dim, st, cm, sto, cmo)
+    g.setup_cavity(0.05)
+    o.setup_cavity(0.05)
+    t0 = g.tuning()
+    try:
+        g.set_tuning(boundary=3)
+        g.run(8, 1.7)
+    finally:
+        g.set_tuning(**t0)
+    o.run(8, 1.7)
+    assert np.array_equal(_graw(g), _raw(o))
+
+
+@pytest.mark.parametrize("wg", [256, 512])
+@pytest.mark.parametrize("st,cm", [("f16dev", "f32"), ("f32", "f32"), ("f64", "f64")])
+def test_container_edge_long_rows_obstacles_equal_plain_step(nx, st, cm, wg):
+    """boundary = 3 with rows of 130 cells and obstacles in the outer layers
+    (links and walls the flags, not the box, decide), an open x face,
+    periodic y and a body force: equal to the single-kernel step bit for
+    bit."""
+    dim = (130, 9, 8)
+    X, Y, Z = dim
+    rng = np.random.default_rng(21)
+    cls = np.full((Z, Y, X), 2, np.uint32)
+    cls[rng.random((Z, Y, X)) < 0.08] = 0
+    cls[:, :, 0] = 1
+    a = nx.DenseLattice(dim, st, cm, periodic=2)
+    b = nx.DenseLattice(dim, st, cm, periodic=2)
+    for lat in (a, b):
+        lat.set_classes(cls.reshape(-1), ulb=0.04)
+    force = (2e-5, 1e-5, 0.0)
+    t0 = a.tuning()
+    try:
+        a.set_tuning(wg=wg, boundary=3)
+        a.run(7, 1.5, force=force)
+    finally:
+        a.set_tuning(**t0)
+    b.run_plain(7, 1.5, force=force)
+    assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
+
+
 @pytest.mark.parametrize("st,cm", [("f32", "f32"), ("f16dev", "f32"), ("f64", "f64")])
 def test_container_graph_replays_equal_eager_steps(nx, st, cm):
     """Runs of 64 steps replay one captured hipGraph on a created stream;
