@@ -33,6 +33,15 @@
 
 #include "dense_common.hpp"
 
+namespace nx {
+namespace dense {
+// the x-end wall value of each population (raw storage bits)
+struct XWall {
+    uint64_t v[Q];
+};
+}  // namespace dense
+}  // namespace nx
+
 struct nx_container_s {
     nx_dense_desc d;
     nx::dense::DGeom g;
@@ -45,6 +54,11 @@ struct nx_container_s {
     uint64_t n_boundary = 0;       // bulk cells on the full-pull path
     uint64_t n_interior = 0;
     int copy_walls = 0;            // 1: fin and fout wall values are identical
+    // 1: in the rows away from the y/z faces (2 <= y <= Y-3, 2 <= z <= Z-3)
+    // every x = 0 and x = X-1 cell is a non-bulk cell holding the same value
+    // per population (xwall.v), in both fields
+    int xwall_uniform = 0;
+    nx::dense::XWall xwall{};
     // graph of 2 * kPairSteps steps alternating this container and `pair_with`
     hipGraphExec_t pair_graph = nullptr;
     const nx_container_s* pair_with = nullptr;
@@ -140,6 +154,45 @@ __global__ void __launch_bounds__(kCWG)
                                __builtin_bit_cast(U, b[q * g.plane + i]);
         }
     if (__ballot(diff) && (threadIdx.x & 63) == 0) mismatch[blockIdx.x] = 1u;
+}
+
+template <typename T>
+__device__ __forceinline__ uint64_t raw_bits(T v)
+{
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(uint64_t, v);
+    else if constexpr (sizeof(T) == 4) return __builtin_bit_cast(uint32_t, v);
+    else return __builtin_bit_cast(uint16_t, v);
+}
+
+template <typename T>
+__device__ __forceinline__ T from_raw(uint64_t u)
+{
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, u);
+    else if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, (uint32_t)u);
+    else return __builtin_bit_cast(T, (uint16_t)u);
+}
+
+// The x-end cells of the rows away from the y/z faces (x in {0, X-1},
+// 2 <= y <= Y-3, 2 <= z <= Z-3): 1 in *mismatch unless every one is a
+// non-bulk cell holding, per population, the value of the first of them,
+// whose values go to ref[0..18].
+template <typename T>
+__global__ void __launch_bounds__(kCWG)
+    k_xwall_check(const T* __restrict__ f, const uint32_t* __restrict__ flags, DGeom g,
+                  uint32_t* mismatch, uint64_t* ref)
+{
+    const uint32_t t = blockIdx.x * kCWG + threadIdx.x;
+    const uint32_t ny = (uint32_t)g.Y - 4u, nz = (uint32_t)g.Z - 4u;
+    const uint32_t i0 = 2u * (uint32_t)g.X + 2u * g.XY;
+    if (t < Q) ref[t] = raw_bits(f[(uint64_t)t * g.plane + i0]);
+    if (t >= 2u * ny * nz) return;
+    const uint32_t r = t >> 1;
+    const uint32_t x = (t & 1u) ? (uint32_t)g.X - 1u : 0u;
+    const uint32_t i = x + (uint32_t)g.X * (2u + r % ny) + g.XY * (2u + r / ny);
+    bool bad = cls_of(flags[i]) == NX_CELL_BULK;
+    for (int q = 0; q < Q; q++)
+        bad = bad || raw_bits(f[(uint64_t)q * g.plane + i]) != raw_bits(f[(uint64_t)q * g.plane + i0]);
+    if (bad) *mismatch = 1u;
 }
 
 // WG threads, CPT cells per thread (cells base + c*WG), LP = load policy
@@ -398,21 +451,35 @@ __global__ void __launch_bounds__(WG)
 // the interior bitmap; a non-interior lane off the box edge (an obstacle
 // inside the box) fetches its flags and takes the general pull.  The
 // arithmetic is that of k_fused_fix, so results agree bit for bit.
+//
+// xw = 1 (checked when the container is made: every x-end cell of the rows
+// away from the y/z faces is a non-bulk cell holding the values cw): in
+// those rows the x = 0 and x = X-1 lanes load nothing and store cw, and the
+// x = 1 / x = X-2 lanes fetch only the five populations pulled from the wall
+// column (c_x = +1 / -1); further links there take the general pull.
 template <int S, int CM, bool FORCE, bool PER, int WG, int LP>
 __global__ void __launch_bounds__(WG)
     k_fused_edge(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
                  const uint32_t* __restrict__ flags, const uint32_t* __restrict__ bits, DGeom g, Par p,
-                 int remap, int copy_walls)
+                 int remap, int copy_walls, int xw, XWall cw)
 {
     using T = typename Stor<S>::T;
     using P = typename Stor<S>::P;
     using R = typename Real<S, CM>::R;
+    // masks of the populations with c_x = +1 and c_x = -1
+    constexpr uint32_t MP = (1u << 10) | (1u << 13) | (1u << 14) | (1u << 15) | (1u << 16);
+    constexpr uint32_t MM = (1u << 0) | (1u << 3) | (1u << 4) | (1u << 5) | (1u << 6);
     const uint32_t wg = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t i = wg * (uint32_t)WG + threadIdx.x;
     const uint32_t ic = min(i, g.n - 1u);
     int x, y, z;
     coords(g, ic, x, y, z);
-    const bool edge = x <= 1 || x >= g.X - 2 || y <= 1 || y >= g.Y - 2 || z <= 1 || z >= g.Z - 2;
+    const bool yz = y <= 1 || y >= g.Y - 2 || z <= 1 || z >= g.Z - 2;
+    const bool edge = yz || x <= 1 || x >= g.X - 2;
+    const bool xlo = x == 1;
+    const bool known_wall = xw && !yz && (x == 0 || x == g.X - 1);
+    const bool part = xw && !yz && (xlo || x == g.X - 2);
+    const bool full = edge && !known_wall && !part;
     uint32_t word = bits[ic >> 5];
     uint32_t fl = 0u;
     T v[Q], a[Q], w[Q];
@@ -424,15 +491,20 @@ __global__ void __launch_bounds__(WG)
         a[q] = T(0);
         w[q] = T(0);
     }
-    if (edge) {
+    if (edge && !known_wall) {
         fl = flags[ic];
+        // a part lane needs only the populations pulled from the wall column
+        // (c_x = +1 at x = 1, c_x = -1 at x = X-2); its other speculative
+        // loads re-read its own population 9 (a line the streaming load
+        // fetches anyway) and their values go unused
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             if (q == 9) continue;
+            const bool need = !part || (CX[q] == 1 && xlo) || (CX[q] == -1 && !xlo);
             const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
-            const T* po = fin + (uint64_t)OPP[q] * g.plane;
+            const T* po = fin + (uint64_t)(need ? OPP[q] : 9) * g.plane;
             a[q] = ld_plane<LP>(po, ic);
-            w[q] = ld_plane<LP>(po, min(ic - (uint32_t)off, g.n - 1u));
+            w[q] = ld_plane<LP>(po, need ? min(ic - (uint32_t)off, g.n - 1u) : ic);
         }
     }
 #pragma unroll
@@ -449,13 +521,13 @@ __global__ void __launch_bounds__(WG)
     else if (!edge) fl = flags[i];
     const bool bulk = cls_of(fl) == NX_CELL_BULK;
     const uint32_t m = (bulk && !interior) ? (fl & NX_MASK_BITS) : 0u;
-    bool general = bulk && !interior && (!edge || (fl & NX_FLAG_OPEN));
+    bool general = bulk && !interior && (!edge || (fl & NX_FLAG_OPEN) || (part && (m & ~(xlo ? MP : MM))));
     if (PER && bulk && !interior && !general)
         general = ((g.periodic & NX_PERIODIC_X) && (x == 0 || x == g.X - 1)) ||
                   ((g.periodic & NX_PERIODIC_Y) && (y == 0 || y == g.Y - 1)) ||
                   ((g.periodic & NX_PERIODIC_Z) && (z == 0 || z == g.Z - 1));
     if (!bulk && !copy_walls) return;
-    if (!bulk && !edge) {
+    if (!bulk && !full && !known_wall) {
 #pragma unroll
         for (int q = 0; q < Q; q++)
             if (q != 9) a[q] = fin[(uint64_t)OPP[q] * g.plane + i];
@@ -474,7 +546,8 @@ __global__ void __launch_bounds__(WG)
     // one store per population for every lane (wall lanes store their copy)
 #pragma unroll
     for (int q = 0; q < Q; q++) {
-        const T o = bulk ? Stor<S>::template enc<R>(out[q], q) : (q == 9 ? v[9] : a[OPP[q]]);
+        const T o = bulk ? Stor<S>::template enc<R>(out[q], q)
+                         : known_wall ? from_raw<T>(cw.v[q]) : (q == 9 ? v[9] : a[OPP[q]]);
         st_plane(fout + q * g.plane, i, o);
     }
 }
@@ -532,7 +605,7 @@ void launch_fused(const nx_tuning& t, const nx_container_s& c, hipStream_t s)
             if (t.boundary == 3)
                 hipLaunchKernelGGL((k_fused_edge<S, CM, FORCE, PER, WG, LP>), dim3(nb), dim3(WG), 0, s,
                                    (const T*)c.fin, (T*)c.fout, c.flags, c.interior, c.g, c.p,
-                                   t.xcd_remap, c.copy_walls);
+                                   t.xcd_remap, c.copy_walls, c.xwall_uniform, c.xwall);
             else if (t.boundary == 2)
                 hipLaunchKernelGGL((k_fused_fix<S, CM, FORCE, PER, WG, LP, true>), dim3(nb), dim3(WG), 0, s,
                                    (const T*)c.fin, (T*)c.fout, c.flags, c.interior, c.g, c.p,
@@ -694,6 +767,35 @@ extern "C" int nx_d3q19_iteration_dense(const nx_dense_desc* d, const void* fin,
     c->copy_walls = 1;
     for (uint32_t k = 0; k < nwg_c; k++)
         if (cnt[k]) c->copy_walls = 0;
+    // uniform x-end walls (the edge-speculating kernel's short path)
+    if (c->copy_walls && d->dim[0] >= 4 && d->dim[1] >= 5 && d->dim[2] >= 5) {
+        const uint32_t items = 2u * (d->dim[1] - 4) * (d->dim[2] - 4);
+        // d_icnt (8 bytes per classification workgroup) is free again: the
+        // mismatch word and, when it holds >= 19 words, the reference values
+        uint64_t* d_ref = nullptr;
+        if (nwg_c >= Q) d_ref = d_icnt;
+        else if ((e = hipMalloc(&d_ref, Q * sizeof(uint64_t))) != hipSuccess)
+            return fail(hip_error(e, "hipMalloc"));
+        auto done = [&](int code) {
+            if (d_ref != d_icnt) hipFree(d_ref);
+            return code == NX_OK ? code : fail(code);
+        };
+        if ((e = hipMemset(d_cnt, 0, 4)) != hipSuccess) return done(hip_error(e, "hipMemset"));
+        by_type(d, [&](auto sc, auto) {
+            constexpr int S = decltype(sc)::value;
+            using T = typename Stor<S>::T;
+            hipLaunchKernelGGL((k_xwall_check<T>), dim3((items + kCWG - 1) / kCWG), dim3(kCWG), 0, 0,
+                               (const T*)fin, flags, c->g, d_cnt, d_ref);
+        });
+        if ((e = hipGetLastError()) != hipSuccess) return done(hip_error(e, "k_xwall_check"));
+        uint32_t bad = 1;
+        if ((e = hipMemcpy(&bad, d_cnt, 4, hipMemcpyDeviceToHost)) != hipSuccess)
+            return done(hip_error(e, "hipMemcpy"));
+        if ((e = hipMemcpy(c->xwall.v, d_ref, sizeof(c->xwall.v), hipMemcpyDeviceToHost)) != hipSuccess)
+            return done(hip_error(e, "hipMemcpy"));
+        c->xwall_uniform = bad ? 0 : 1;
+        done(NX_OK);
+    }
     hipFree(d_cnt);
     hipFree(d_icnt);
     *out = c;
