@@ -499,12 +499,22 @@ __global__ void __launch_bounds__(WG)
         // fetches anyway) and their values go unused
 #pragma unroll
         for (int q = 0; q < Q; q++) {
-            if (q == 9) continue;
+            if (q == 9 || CX[q] == 0) continue;
             const bool need = !part || (CX[q] == 1 && xlo) || (CX[q] == -1 && !xlo);
             const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
             const T* po = fin + (uint64_t)(need ? OPP[q] : 9) * g.plane;
             a[q] = ld_plane<LP>(po, ic);
             w[q] = ld_plane<LP>(po, need ? min(ic - (uint32_t)off, g.n - 1u) : ic);
+        }
+        if (!part) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                if (q == 9 || CX[q] != 0) continue;
+                const int off = CX[q] + CY[q] * g.X + CZ[q] * (int)g.XY;
+                const T* po = fin + (uint64_t)OPP[q] * g.plane;
+                a[q] = ld_plane<LP>(po, ic);
+                w[q] = ld_plane<LP>(po, min(ic - (uint32_t)off, g.n - 1u));
+            }
         }
     }
 #pragma unroll
