@@ -223,8 +223,13 @@ int nx_d3q19_moments_dense(const nx_dense_desc* d, const void* fin,
  * fields hold identical values in every non-bulk cell (as the reference's
  * set-up leaves them, RunCavityTwoPop.cu:168-206) the kernel also writes
  * those unchanged wall values into fout, so every cache line is written
- * whole; info reports this as copies_walls.  The two fields of a ping-pong
- * pair take one container each (LbmIteration.h:41-42). */
+ * whole; info reports this as copies_walls.  Creation also notes whether
+ * the walls at both x ends (rows at least two cells from the y and z faces)
+ * hold one common value per population, which lets the edge-speculating
+ * kernel (boundary 3) store that value without loading it.  Wall values are
+ * thus read when the container is made: after changing them, make the
+ * container again.  The two fields of a ping-pong pair take one container
+ * each (LbmIteration.h:41-42). */
 int nx_d3q19_iteration_dense(const nx_dense_desc* d, const void* fin, void* fout,
                              const uint32_t* flags, const nx_lbm_params* p, nx_container* c);
 int nx_container_run(nx_container c, nx_stream s);

@@ -567,9 +567,9 @@ __global__ void __launch_bounds__(WG)
 // cavity on MI355X -- the single-collision kernel for 8- and 4-byte storage
 // (FP32 35.4k MLUPS with the interior bitmap, 35.3k without, 32.9k for the
 // bitmap + full-pull kernel; FP64 17.2k), the edge-speculating kernel for
-// binary16 (61.7k vs 51.9k for the bitmap + full-pull kernel; for 4- and
-// 8-byte storage its 36 extra registers cost more than the round trip it
-// saves: FP32 34.5k, FP64 16.6k).
+// binary16 (65.6k with uniform x-end walls, vs 51.9k for the bitmap +
+// full-pull kernel); 4- and 8-byte storage keep the bitmap kernels, whose
+// lower register count matters more there (FP32: 35.2k either way).
 nx_tuning g_tuning[4][3] = {
     {{256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}, {256, 1, 0, 0, 1, 1}},
     {{256, 1, 0, 0, 1, 2}, {256, 1, 0, 0, 1, 2}, {256, 1, 0, 0, 1, 2}},

@@ -202,6 +202,9 @@ class _Lattice:
     def set_populations(self, arr: np.ndarray, which: int | None = None) -> None:
         arr = np.ascontiguousarray(arr, dtype=self.np_storage).reshape(-1)
         assert arr.size == 19 * self.n_slots
+        drop = getattr(self, "_drop_containers", None)
+        if drop is not None:
+            drop()                  # containers hold what they found on the walls
         if which is None:
             for b in self.pop:
                 b.upload(arr)

@@ -403,6 +403,42 @@ def test_container_edge_long_rows_obstacles_equal_plain_step(nx, st, cm, wg):
     assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
 
 
+@pytest.mark.parametrize("uniform", [True, False])
+@pytest.mark.parametrize("st,cm", [("f16dev", "f32"), ("f32", "f32"), ("f32dev", "f32"), ("f64", "f64")])
+def test_container_edge_uniform_x_walls_equal_plain_step(nx, st, cm, uniform):
+    """boundary = 3 with bounce-back walls at both x ends: their lanes store
+    the walls' common values and the cells beside them fetch only the five
+    wall-column links.  Obstacles at x = 2 give those cells further links
+    (general pull), obstacles at x = 1 are walls beside the wall (copied from
+    their own values), periodic y and a body force.  uniform=False puts one
+    moving-wall cell (other values) into the x = 0 layer, so every x-end
+    lane speculates in full.  Equal to the single-kernel step bit for bit."""
+    dim = (100, 10, 9)
+    X, Y, Z = dim
+    rng = np.random.default_rng(33)
+    cls = np.full((Z, Y, X), 2, np.uint32)
+    cls[rng.random((Z, Y, X)) < 0.06] = 0
+    cls[:, :, 0] = 0
+    cls[:, :, X - 1] = 0
+    cls[3, 4, 2] = cls[5, 6, X - 3] = cls[4, 2, 2] = 0
+    cls[4, 5, 1] = cls[6, 3, X - 2] = 0
+    if not uniform:
+        cls[4, 4, 0] = 1
+    a = nx.DenseLattice(dim, st, cm, periodic=2)
+    b = nx.DenseLattice(dim, st, cm, periodic=2)
+    for lat in (a, b):
+        lat.set_classes(cls.reshape(-1), ulb=0.04)
+    force = (2e-5, 1e-5, 0.0)
+    t0 = a.tuning()
+    try:
+        a.set_tuning(boundary=3)
+        a.run(9, 1.5, force=force)
+    finally:
+        a.set_tuning(**t0)
+    b.run_plain(9, 1.5, force=force)
+    assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
+
+
 @pytest.mark.parametrize("st,cm", [("f32", "f32"), ("f16dev", "f32"), ("f64", "f64")])
 def test_container_graph_replays_equal_eager_steps(nx, st, cm):
     """Runs of 64 steps replay one captured hipGraph on a created stream;
@@ -437,3 +473,30 @@ def test_container_graph_replays_equal_eager_steps(nx, st, cm):
     h.run(129, omega)
     assert np.array_equal(g.populations().view(np.uint8), h.populations().view(np.uint8))
     assert abs(g.mass() - m0) < 1e-3 * m0
+
+
+def test_container_remade_after_wall_values_change(nx):
+    """Containers are made from the wall values they find (equal in both
+    fields, uniform x ends); set_populations drops them, so a changed x-end
+    wall value is seen by the next run: equal to the single-kernel step."""
+    dim = (96, 8, 7)
+    a = nx.DenseLattice(dim, "f16dev", "f32")
+    b = nx.DenseLattice(dim, "f16dev", "f32")
+    for lat in (a, b):
+        lat.setup_cavity(0.05)
+    t0 = a.tuning()
+    try:
+        a.set_tuning(boundary=3)
+        a.run(4, 1.6)
+        b.run_plain(4, 1.6)
+        p = a.populations()
+        X, Y, Z = dim
+        cell = 0 + X * 3 + X * Y * 4                # an x = 0 wall cell away from the y/z faces
+        p[3 * X * Y * Z + cell] = np.float16(0.25).view(np.uint16) if p.dtype == np.uint16 else 0.25
+        for lat in (a, b):
+            lat.set_populations(p)
+        a.run(5, 1.6)
+        b.run_plain(5, 1.6)
+    finally:
+        a.set_tuning(**t0)
+    assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
