@@ -33,15 +33,6 @@
 
 #include "dense_common.hpp"
 
-namespace nx {
-namespace dense {
-// the x-end wall value of each population (raw storage bits)
-struct XWall {
-    uint64_t v[Q];
-};
-}  // namespace dense
-}  // namespace nx
-
 struct nx_container_s {
     nx_dense_desc d;
     nx::dense::DGeom g;
@@ -154,22 +145,6 @@ __global__ void __launch_bounds__(kCWG)
                                __builtin_bit_cast(U, b[q * g.plane + i]);
         }
     if (__ballot(diff) && (threadIdx.x & 63) == 0) mismatch[blockIdx.x] = 1u;
-}
-
-template <typename T>
-__device__ __forceinline__ uint64_t raw_bits(T v)
-{
-    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(uint64_t, v);
-    else if constexpr (sizeof(T) == 4) return __builtin_bit_cast(uint32_t, v);
-    else return __builtin_bit_cast(uint16_t, v);
-}
-
-template <typename T>
-__device__ __forceinline__ T from_raw(uint64_t u)
-{
-    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, u);
-    else if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, (uint32_t)u);
-    else return __builtin_bit_cast(T, (uint16_t)u);
 }
 
 // The x-end cells of the rows away from the y/z faces (x in {0, X-1},

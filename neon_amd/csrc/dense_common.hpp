@@ -39,6 +39,28 @@ struct Par {
     double g[3];
 };
 
+// The common wall value of each population at the x ends of a box (raw
+// storage bits), for the edge-speculating steps.
+struct XWall {
+    uint64_t v[Q];
+};
+
+template <typename T>
+__device__ __forceinline__ uint64_t raw_bits(T v)
+{
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(uint64_t, v);
+    else if constexpr (sizeof(T) == 4) return __builtin_bit_cast(uint32_t, v);
+    else return __builtin_bit_cast(uint16_t, v);
+}
+
+template <typename T>
+__device__ __forceinline__ T from_raw(uint64_t u)
+{
+    if constexpr (sizeof(T) == 8) return __builtin_bit_cast(T, u);
+    else if constexpr (sizeof(T) == 4) return __builtin_bit_cast(T, (uint32_t)u);
+    else return __builtin_bit_cast(T, (uint16_t)u);
+}
+
 // Upstream index of population q of cell i (x - c_q), with periodic wrap.
 // Returns false when the upstream cell lies outside the grid.
 __device__ __forceinline__ bool upstream(const DGeom& g, int x, int y, int z, int q, uint32_t& j)
