@@ -403,10 +403,11 @@ def test_container_edge_long_rows_obstacles_equal_plain_step(nx, st, cm, wg):
     assert np.array_equal(a.populations().view(np.uint8), b.populations().view(np.uint8))
 
 
+@pytest.mark.parametrize("boundary", [1, 2, 3])
 @pytest.mark.parametrize("uniform", [True, False])
 @pytest.mark.parametrize("st,cm", [("f16dev", "f32"), ("f32", "f32"), ("f32dev", "f32"), ("f64", "f64")])
-def test_container_edge_uniform_x_walls_equal_plain_step(nx, st, cm, uniform):
-    """boundary = 3 with bounce-back walls at both x ends: their lanes store
+def test_container_edge_uniform_x_walls_equal_plain_step(nx, st, cm, uniform, boundary):
+    """boundary = 1, 2 and 3 with bounce-back walls at both x ends: their lanes store
     the walls' common values and the cells beside them fetch only the five
     wall-column links.  Obstacles at x = 2 give those cells further links
     (general pull), obstacles at x = 1 are walls beside the wall (copied from
@@ -431,7 +432,7 @@ def test_container_edge_uniform_x_walls_equal_plain_step(nx, st, cm, uniform):
     force = (2e-5, 1e-5, 0.0)
     t0 = a.tuning()
     try:
-        a.set_tuning(boundary=3)
+        a.set_tuning(boundary=boundary)
         a.run(9, 1.5, force=force)
     finally:
         a.set_tuning(**t0)
