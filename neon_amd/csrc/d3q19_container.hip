@@ -469,9 +469,9 @@ __global__ void __launch_bounds__(WG)
     if (edge && !known_wall) {
         fl = flags[ic];
         // a part lane needs only the populations pulled from the wall column
-        // (c_x = +1 at x = 1, c_x = -1 at x = X-2); its other speculative
-        // loads re-read its own population 9 (a line the streaming load
-        // fetches anyway) and their values go unused
+        // (c_x = +1 at x = 1, c_x = -1 at x = X-2): its loads for the other
+        // x direction re-read its own population 9 (a line the streaming load
+        // fetches anyway, values unused), and it skips the c_x = 0 ones
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             if (q == 9 || CX[q] == 0) continue;
