@@ -618,7 +618,8 @@ struct LbmContainers
             // The library container classifies the cells when it is created,
             // so it is bound at the first run: the reference builds its
             // iteration before the cell types are set (RunCavityTwoPop.cu:77-84
-            // vs 208-241).  Cell types must not change after that run.
+            // vs 208-241).  Cell types and wall values must not change after
+            // that run (the container notes the walls it finds).
             struct Lazy
             {
                 nx_dense_desc   d;
