@@ -254,6 +254,13 @@ int nx_d3q19_set_tuning_dense(const nx_dense_desc* d, const nx_tuning* t);
 int nx_container_autotune(nx_container c, nx_stream s, nx_tuning* best, float* best_ms);
 
 /* FP64 reductions on the GPU (wave64 DPP + two-level tree; deterministic).
+ * The per-workgroup partials live in one library-owned scratch buffer per
+ * device, shared by every mass and residual entry point (dense and block)
+ * and by nx_conv3d_bwd_data_bf16 / nx_conv3d_bwd_weight_bf16: calls of these
+ * on one device must be ordered (one stream, or synchronised between
+ * streams); two of them in flight at once on different streams race on the
+ * scratch.  The buffer grows on demand (hipMalloc after a device-wide
+ * synchronisation).
  * mass: out[0] = sum over bulk cells of sum_q f_q  (device double*).
  * residual: u from the post-stream moments of fin; out[0] = sum |u-u_prev|^2,
  *   out[1] = sum |u|^2 over bulk cells, then u_prev <- u
@@ -334,6 +341,10 @@ int nx_batch_init(nx_batch b, const float* rho, const float* u, nx_stream s);
 int nx_batch_constrain(nx_batch b, const float* u, int32_t iterations, const uint8_t* members,
                        nx_stream s);
 int nx_batch_step(nx_batch b, int32_t k_steps, nx_stream s);
+/* nx_batch_run replays on the batch's own stream and first waits for the
+ * whole device (hipDeviceSynchronize), so work the caller queued on any
+ * stream (nx_batch_init, nx_batch_step, ...) is complete before it starts;
+ * it returns after its last replay has finished. */
 int nx_batch_run(nx_batch b, int32_t max_steps, int32_t check_every, double tol,
                  int32_t* steps, double* residual, uint32_t* n_unconverged);
 /* Launch of the persistent step: tiles of wg (256 or 512) cells, wgs_per_cu
@@ -350,6 +361,13 @@ int nx_batch_set_launch(nx_batch b, int32_t wg, int32_t wgs_per_cu);
 int nx_batch_set_variant(nx_batch b, int32_t xcd_chunks, int32_t edge);
 int nx_batch_get_launch(nx_batch b, int32_t* wg, int32_t* wgs_per_cu, int32_t* xcd_chunks,
                         int32_t* edge);
+/* Workgroups of the persistent step for n_tiles tiles on n_cu CUs with
+ * wgs_per_cu per CU (0: one per tile): min(n_tiles, n_cu * wgs_per_cu) in
+ * 64-bit arithmetic, at least 1.  Host-only, no device work.
+ * nx_batch_launch_grid reports the grid and whether tiles are dealt in XCD
+ * chunks for the batch's current launch. */
+int nx_batch_grid_size(uint32_t n_tiles, int32_t n_cu, int32_t wgs_per_cu, uint32_t* grid);
+int nx_batch_launch_grid(nx_batch b, uint32_t* grid, int32_t* xcd_chunked);
 int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int32_t* best_wgs_per_cu,
                       float* best_ms);
 int nx_batch_destroy(nx_batch b);

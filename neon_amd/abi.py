@@ -143,6 +143,8 @@ SIGNATURES = {
     "nx_batch_get_launch": (_I, [_P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)]),
     "nx_batch_autotune": (_I, [_P, _P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(C.c_float)]),
     "nx_batch_destroy": (_I, [_P]),
+    "nx_batch_grid_size": (_I, [_U32, _I32, _I32, C.POINTER(_U32)]),
+    "nx_batch_launch_grid": (_I, [_P, C.POINTER(_U32), C.POINTER(_I32)]),
     "nx_batch_constrain": (_I, [_P, _P, _I32, _P, _P]),
     "nx_conv3d_fwd_bf16": (_I, [_P, _P, _P, _P, C.POINTER(_I32), _I32, _I32, _I32, _I32, _P]),
     "nx_conv3d_bwd_data_bf16": (_I, [_P, _P, _P, C.POINTER(_I32), _I32, _I32, _I32, _I32, _P]),

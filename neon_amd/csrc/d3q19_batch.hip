@@ -416,10 +416,19 @@ size_t elem_size(int storage)
     return storage == NX_F64 ? 8 : (storage == NX_F16DEV ? 2 : 4);
 }
 
+// Persistent grid: wgs_per_cu workgroups per CU, at most one per tile;
+// wgs_per_cu == 0 asks for one workgroup per tile.  The product is taken in
+// 64 bits so that no count of CUs and workgroups can wrap.
+unsigned grid_size(uint32_t n_tiles, int n_cu, int wgs_per_cu)
+{
+    if (wgs_per_cu <= 0) return std::max(1u, n_tiles);
+    const uint64_t cap = (uint64_t)std::max(n_cu, 1) * (uint64_t)wgs_per_cu;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(n_tiles, cap));
+}
+
 unsigned grid_of(const nx_batch_s& b)
 {
-    const unsigned cap = (unsigned)(b.n_cu * b.wgs_per_cu);
-    return std::max(1u, std::min<unsigned>(b.n_tiles, cap));
+    return grid_size(b.n_tiles, b.n_cu, b.wgs_per_cu);
 }
 
 // XCD chunking needs a persistent grid that is a multiple of the 8 XCDs
@@ -736,6 +745,10 @@ extern "C" int nx_batch_run(nx_batch b, int32_t max_steps, int32_t check_every, 
                          "nx_batch_run: check_every must be even and >= 2, tol > 0");
     if (!b->graph_stream) NX_CHECK_HIP(hipStreamCreateWithFlags(&b->graph_stream, hipStreamNonBlocking));
     const hipStream_t st = b->graph_stream;
+    // the replays run on the batch's own non-blocking stream: whatever the
+    // caller queued on its streams (nx_batch_step, nx_batch_init, ...) must
+    // have finished before they touch the arenas
+    NX_CHECK_HIP(hipDeviceSynchronize());
     if (!b->graph || b->graph_k != check_every || b->graph_tol != tol) {
         drop_graph(*b);
         hipGraph_t g;
@@ -781,7 +794,7 @@ extern "C" int nx_batch_set_launch(nx_batch b, int32_t wg, int32_t wgs_per_cu)
         build_tiles(*b, wg);
         if (int e = upload_tiles(*b)) return e;
     }
-    b->wgs_per_cu = wgs_per_cu ? wgs_per_cu : 1 << 30;
+    b->wgs_per_cu = wgs_per_cu;
     drop_graph(*b);
     return NX_OK;
 }
@@ -793,8 +806,25 @@ extern "C" int nx_batch_set_variant(nx_batch b, int32_t xcd_chunks, int32_t edge
     if (!b) return set_error(NX_ERR_INVALID, "null batch");
     if ((xcd_chunks != 0 && xcd_chunks != 1) || (edge != 0 && edge != 1))
         return set_error(NX_ERR_INVALID, "batch variant: xcd_chunks and edge are 0 or 1");
+    if (b->xcd_chunks != xcd_chunks || b->edge != edge) drop_graph(*b);  // nx_batch_run recaptures
     b->xcd_chunks = xcd_chunks;
     b->edge = edge;
+    return NX_OK;
+}
+
+extern "C" int nx_batch_grid_size(uint32_t n_tiles, int32_t n_cu, int32_t wgs_per_cu, uint32_t* grid)
+{
+    if (!grid || n_cu < 0 || wgs_per_cu < 0)
+        return set_error(NX_ERR_INVALID, "nx_batch_grid_size: n_cu, wgs_per_cu >= 0 and a result pointer");
+    *grid = grid_size(n_tiles, n_cu, wgs_per_cu);
+    return NX_OK;
+}
+
+extern "C" int nx_batch_launch_grid(nx_batch b, uint32_t* grid, int32_t* xcd_chunked)
+{
+    if (!b) return set_error(NX_ERR_INVALID, "null batch");
+    if (grid) *grid = grid_of(*b);
+    if (xcd_chunked) *xcd_chunked = chunked(*b) ? 1 : 0;
     return NX_OK;
 }
 

@@ -499,6 +499,12 @@ class CavityBatch:
         check(lib.nx_batch_info(self.h, C.byref(nm), C.byref(nc), C.byref(nt)))
         self.n_cells, self.n_tiles = int(nc.value), int(nt.value)
 
+    def tiles(self) -> int:
+        """Tiles of the current launch (they depend on its workgroup size)."""
+        nt = C.c_uint32()
+        check(lib.nx_batch_info(self.h, None, None, C.byref(nt)))
+        return int(nt.value)
+
     def init(self, rho=None, u=None, stream=None):
         """Equilibrium start from device buffers (arena cell order) or rest."""
         check(lib.nx_batch_init(self.h, rho.ptr if rho is not None else None,
@@ -535,7 +541,11 @@ class CavityBatch:
     def launch(self) -> dict:
         v = [C.c_int32() for _ in range(4)]
         check(lib.nx_batch_get_launch(self.h, *(C.byref(x) for x in v)))
-        return dict(zip(("wg", "wgs_per_cu", "xcd_chunks", "edge"), (int(x.value) for x in v)))
+        out = dict(zip(("wg", "wgs_per_cu", "xcd_chunks", "edge"), (int(x.value) for x in v)))
+        grid, chunked = C.c_uint32(), C.c_int32()
+        check(lib.nx_batch_launch_grid(self.h, C.byref(grid), C.byref(chunked)))
+        out["grid"], out["chunked"] = int(grid.value), int(chunked.value)
+        return out
 
     def autotune(self, stream=None):
         w, c, ms = C.c_int32(), C.c_int32(), C.c_float()
@@ -553,6 +563,7 @@ class CavityBatch:
         n = 19 * d.dim[0] * d.dim[1] * d.dim[2]
         out = np.empty(n, _NP_STORAGE[self.storage])
         check(lib.nx_d2h(out.ctypes.data, p, out.nbytes, None))
+        check(lib.nx_stream_sync(None))
         return out
 
     def moments(self, k: int):

@@ -286,24 +286,32 @@ def loss_of(pred: np.ndarray, y: np.ndarray, w: np.ndarray) -> float:
 
 
 CONSTRAIN_ITERATIONS = 10
+PROBE_INTERVALS = 2      # check intervals per start in the fallback test
 
 
 def warm_start(batch: CavityBatch, rho: np.ndarray, u: np.ndarray, check_every: int,
                iterations: int = CONSTRAIN_ITERATIONS):
     """Initialise `batch` from the predicted (rho, u): equilibrium populations,
     then `iterations` constrained double steps (nx_batch_constrain) so that
-    density and non-equilibrium parts match the field.  A member whose
-    residual after one further check interval is not below the rest start's
-    falls back to the rest start.  Leaves the batch initialised (0 steps
-    counted); returns the boolean fallback mask.  Cost beyond the returned
-    state: 4 * iterations + 2 * check_every steps of every member."""
+    density and non-equilibrium parts match the field.
+
+    Fallback test: the predicted start and the rest start each run
+    PROBE_INTERVALS check intervals, and a member falls back to rest when the
+    predicted start's residual over the last interval,
+    r = ||u(t) - u(t - K)|| / ||u(t)||, is not below the rest start's over the
+    same interval.  The first interval cannot rank the two: the rest start's
+    previous velocity is 0, which makes its first residual exactly 1.  Leaves
+    the batch initialised (0 steps counted); returns the boolean fallback
+    mask.  Cost beyond the returned state: warm_start_overhead steps of every
+    member."""
     dr, du = DeviceBuffer.from_array(rho), DeviceBuffer.from_array(u)
     never = 1e-150            # no member retires during the probe intervals
+    probe = PROBE_INTERVALS * check_every
     batch.init(dr, du)
     batch.constrain(du, iterations)
-    _, res_w, _ = batch.run(max_steps=check_every, check_every=check_every, tol=never)
+    _, res_w, _ = batch.run(max_steps=probe, check_every=check_every, tol=never)
     batch.init()
-    _, res_c, _ = batch.run(max_steps=check_every, check_every=check_every, tol=never)
+    _, res_c, _ = batch.run(max_steps=probe, check_every=check_every, tol=never)
     fallback = ~(res_w < res_c)
     if fallback.any():
         rho = rho.copy()
@@ -323,8 +331,9 @@ def warm_start(batch: CavityBatch, rho: np.ndarray, u: np.ndarray, check_every: 
 
 
 def warm_start_overhead(check_every: int, iterations: int = CONSTRAIN_ITERATIONS) -> int:
-    """Steps of every member spent by warm_start before the free iteration."""
-    return 4 * iterations + 2 * check_every
+    """Steps of every member spent by warm_start before the free iteration:
+    the constrained start twice (probe and final) and both probes."""
+    return 4 * iterations + 2 * PROBE_INTERVALS * check_every
 
 
 def forward_flops(net: UNet) -> float:

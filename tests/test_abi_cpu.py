@@ -80,3 +80,28 @@ def test_pop_bytes():
     b = abi.BlockDesc()
     b.n_blocks, b.storage = 3, abi.NX_F16DEV
     assert abi.lib.nx_pop_bytes_block(C.byref(b)) == 3 * 19 * 512 * 2
+
+
+def test_batch_grid_size():
+    """Persistent grid of the batched step: min(tiles, CUs x workgroups per
+    CU) in 64-bit arithmetic; 0 workgroups per CU means one per tile."""
+    from neon_amd import abi
+
+    def grid(tiles, cus, per_cu):
+        g = C.c_uint32()
+        assert abi.lib.nx_batch_grid_size(tiles, cus, per_cu, C.byref(g)) == abi.NX_OK
+        return g.value
+
+    tiles = 688_128                         # 512 cavities of 32..96 cells, 256-cell tiles
+    assert grid(tiles, 256, 0) == tiles     # one workgroup per tile
+    assert grid(tiles, 304, 0) == tiles
+    assert grid(tiles, 256, 2) == 512
+    assert grid(tiles, 256, 16) == 4096
+    assert grid(1000, 256, 1 << 30) == 1000   # 256 * 2^30 does not wrap to 0
+    assert grid(tiles, 2 ** 31 - 1, 2 ** 31 - 1) == tiles
+    assert grid(5, 304, 16) == 5
+    assert grid(0, 256, 0) == 1
+    g = C.c_uint32()
+    assert abi.lib.nx_batch_grid_size(10, 256, -1, C.byref(g)) == abi.NX_ERR_INVALID
+    assert abi.lib.nx_batch_grid_size(10, 256, 2, None) == abi.NX_ERR_INVALID
+    assert abi.lib.nx_batch_launch_grid(None, C.byref(g), None) == abi.NX_ERR_INVALID

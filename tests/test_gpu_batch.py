@@ -36,7 +36,11 @@ def test_batch_members_equal_standalone(nx, st, launch):
     b = nx.CavityBatch(ms, st)
     b.set_launch(*launch[:2])
     b.set_variant(xcd_chunks=launch[2], edge=launch[3])
-    assert b.launch()["edge"] == launch[3] and b.launch()["xcd_chunks"] == launch[2]
+    lc = b.launch()
+    assert lc["edge"] == launch[3] and lc["xcd_chunks"] == launch[2]
+    assert lc["wgs_per_cu"] == launch[1]
+    if launch[1] == 0:                      # one workgroup per tile
+        assert lc["grid"] == b.tiles() and lc["chunked"] == 0
     b.init()
     b.step(14)
     for k, m in enumerate(ms):
@@ -87,3 +91,22 @@ def test_batch_warm_start_from_converged_fields(nx):
     print("cold", steps, "warm", steps_w)
     assert left == 0
     assert (steps_w + 20 < steps / 2).all(), (steps, steps_w)
+
+
+def test_batch_variant_change_recaptures_graph(nx):
+    """nx_batch_run after nx_batch_set_variant runs the new variant: results
+    stay those of the standalone step and the grid reported is the new one."""
+    ms = make_members(nx, seed=7, sizes=(12, 14))
+    b = nx.CavityBatch(ms, "f32")
+    b.set_launch(256, 4)
+    b.set_variant(xcd_chunks=0, edge=0)
+    b.init()
+    b.run(max_steps=20, check_every=10, tol=1e-30)
+    b.set_variant(xcd_chunks=0, edge=1)
+    b.set_launch(256, 0)
+    assert b.launch()["grid"] == b.tiles()
+    b.init()
+    steps, _, _ = b.run(max_steps=20, check_every=10, tol=1e-30)
+    assert (steps == 20).all()
+    for k, m in enumerate(ms):
+        assert np.array_equal(b.populations(k), standalone(nx, m, "f32", 20).populations()), k

@@ -51,3 +51,16 @@ def test_warm_start_beats_rest_on_held_out_cavities(nx):
     assert fb2.all()
     steps_f, _, left_f = warm.run(max_steps=30000, check_every=every, tol=tol)
     assert left_f == 0 and np.array_equal(steps_f, steps_c)
+
+    # the vortex mirrored top to bottom (the right flow in the wrong place:
+    # the return flow under the lid) must fall back as well, while the right
+    # field at 30% of its speed is closer to steady than rest and is kept
+    u_mirror = np.concatenate([
+        uk.reshape(3, d[2], d[1], d[0])[:, :, ::-1, :].reshape(-1)
+        for uk, d in zip(np.split(u, np.cumsum([3 * d[0] * d[1] * d[2] for d in warm.dims])[:-1]),
+                         warm.dims)])
+    fb3 = S.warm_start(warm, rho, u_mirror, every)
+    fb4 = S.warm_start(warm, rho, 0.3 * u, every)
+    print("fallback: mirrored", int(fb3.sum()), "scaled 0.3", int(fb4.sum()), "of", len(test_ms))
+    assert fb3.mean() > 0.9
+    assert fb4.mean() < 0.1
