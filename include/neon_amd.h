@@ -270,6 +270,20 @@ int nx_d3q19_mass_dense(const nx_dense_desc* d, const void* fin,
 int nx_d3q19_residual_dense(const nx_dense_desc* d, const void* fin,
                             const uint32_t* flags, float* u_prev, double* out,
                             nx_stream s);
+/* Allocation-free variant of nx_d3q19_residual_dense: same arguments, same
+ * kernels and bit-identical results, but its workgroup partials live in a
+ * static device array of the library (at most 2048 partials of two doubles,
+ * 32 KiB per GPU, held until the process ends) instead of the growing
+ * scratch above, so it calls no hipMalloc, hipFree or allocating library
+ * routine.  No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving or convergence
+ * loops, harnesses that forbid allocation).  The array's device address is
+ * looked up by nx_init (or by the first call on a device nx_init did not
+ * select).  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (different streams, or host threads) race. */
+int nx_d3q19_residual_dense_noalloc(const nx_dense_desc* d, const void* fin,
+                                    const uint32_t* flags, float* u_prev, double* out,
+                                    nx_stream s);
 
 /* ---- block-sparse D3Q19 (bGrid, 8^3 blocks) -----------------------------
  * Host builder restating bGrid's constructor for one partition

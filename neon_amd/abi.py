@@ -120,6 +120,7 @@ SIGNATURES = {
     "nx_d3q19_moments_dense": (_I, [_DP, _P, _P, _P, _P, _P]),
     "nx_d3q19_mass_dense": (_I, [_DP, _P, _P, _P, _P]),
     "nx_d3q19_residual_dense": (_I, [_DP, _P, _P, _P, _P, _P]),
+    "nx_d3q19_residual_dense_noalloc": (_I, [_DP, _P, _P, _P, _P, _P]),
     "nx_bgrid_create": (_I, [C.POINTER(_I32), _P, _I32, C.POINTER(_P)]),
     "nx_bgrid_create_cb": (_I, [C.POINTER(_I32), _P, _P, _I32, C.POINTER(_P)]),
     "nx_bgrid_info": (_I, [_P, C.POINTER(_U32), C.POINTER(C.c_uint64)]),

@@ -10,6 +10,8 @@
 // lines).  Each population is read once and written once per step:
 // 2*19*sizeof(storage) bytes per cell update.
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "dense_common.hpp"
 
@@ -422,17 +424,13 @@ extern "C" int nx_d3q19_mass_dense(const nx_dense_desc* d, const void* fin, cons
     return finish_reduce((const double*)part, nb, 1, out, st);
 }
 
-extern "C" int nx_d3q19_residual_dense(const nx_dense_desc* d, const void* fin,
-                                       const uint32_t* flags, float* u_prev, double* out,
-                                       nx_stream s)
+namespace {
+
+// Residual kernels + the fixed-order second stage, with the nb workgroup
+// partials (two doubles each) in `part`.
+int residual_launch(const nx_dense_desc* d, const DGeom& g, const void* fin, const uint32_t* flags,
+                    float* u_prev, double* out, void* part, int nb, hipStream_t st)
 {
-    if (int e = check(d)) return e;
-    if (!fin || !flags || !u_prev || !out) return set_error(NX_ERR_INVALID, "null pointer");
-    const DGeom g = make_geom(d);
-    const int nb = (int)std::min<uint32_t>(nwg(g), kRedGrid);
-    void* part = nullptr;
-    if (int e = scratch(2 * sizeof(double) * nb, &part)) return e;
-    const hipStream_t st = (hipStream_t)s;
     if (int e = by_type(d, [&](auto sc, auto cc) {
             constexpr int S = decltype(sc)::value;
             constexpr int CM = decltype(cc)::value;
@@ -447,5 +445,83 @@ extern "C" int nx_d3q19_residual_dense(const nx_dense_desc* d, const void* fin,
         return e;
     NX_CHECK_LAUNCH("k_residual");
     return finish_reduce((const double*)part, nb, 2, out, st);
+}
+
+}  // namespace
+
+extern "C" int nx_d3q19_residual_dense(const nx_dense_desc* d, const void* fin,
+                                       const uint32_t* flags, float* u_prev, double* out,
+                                       nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!fin || !flags || !u_prev || !out) return set_error(NX_ERR_INVALID, "null pointer");
+    const DGeom g = make_geom(d);
+    const int nb = (int)std::min<uint32_t>(nwg(g), kRedGrid);
+    void* part = nullptr;
+    if (int e = scratch(2 * sizeof(double) * nb, &part)) return e;
+    return residual_launch(d, g, fin, flags, u_prev, out, part, nb, (hipStream_t)s);
+}
+
+// ---- allocation-free residual ---------------------------------------------
+// The workgroup partials of nx_d3q19_residual_dense_noalloc live in this
+// file-scope device array: at most kRedGrid reduction workgroups of two
+// doubles each (32 KiB), held by every GPU the library runs on until the
+// process ends.  Every partial is written before the second stage reads it,
+// so nothing needs clearing between calls.  In host code the array's name is
+// only a stand-in: kernels receive its device address, looked up once per
+// device with hipGetSymbolAddress.
+constexpr int kNoAllocDoubles = 2 * kRedGrid;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kMaxDevices] = {};
+}  // namespace
+
+// Looks up this device's copy of the array (the current device); called by
+// nx_init and, for a device nx_init did not select, by the entry point's
+// first call there.  hipGetSymbolAddress allocates nothing.
+int nx::dense_noalloc_setup()
+{
+    int dev = 0;
+    NX_CHECK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(NX_ERR_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[dev]) return NX_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(::scratch));
+    if (e != hipSuccess) return hip_error(e, "hipGetSymbolAddress(scratch)");
+    if (!ptr) return set_error(NX_ERR_HIP, "hipGetSymbolAddress(scratch): null address");
+    g_noalloc_ptr[dev] = (double*)ptr;
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_residual_dense_noalloc(const nx_dense_desc* d, const void* fin,
+                                               const uint32_t* flags, float* u_prev, double* out,
+                                               nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!fin || !flags || !u_prev || !out) return set_error(NX_ERR_INVALID, "null pointer");
+    const DGeom g = make_geom(d);
+    const int nb = (int)std::min<uint32_t>(nwg(g), kRedGrid);
+    if (2 * nb > kNoAllocDoubles)
+        return set_error(NX_ERR_INVALID, "nx_d3q19_residual_dense_noalloc: " + std::to_string(nb) +
+                                             " reduction partials exceed the static scratch of " +
+                                             std::to_string(kNoAllocDoubles / 2));
+    int dev = 0;
+    NX_CHECK_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return set_error(NX_ERR_INVALID, "device index out of range");
+    double* part = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        part = g_noalloc_ptr[dev];
+    }
+    if (!part) {
+        if (int e = dense_noalloc_setup()) return e;
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        part = g_noalloc_ptr[dev];
+    }
+    return residual_launch(d, g, fin, flags, u_prev, out, part, nb, (hipStream_t)s);
 }
 

@@ -490,6 +490,9 @@ int set_error(int code, const std::string& msg);
 int hip_error(hipError_t e, const char* what);
 // per-device reduction scratch of at least `bytes`, owned by the library
 int scratch(size_t bytes, void** ptr);
+// device address of the allocation-free residual's static scratch, looked up
+// for the current device (d3q19_dense.hip)
+int dense_noalloc_setup();
 int finish_reduce(const double* partials, int n, int nv, double* out, hipStream_t s);
 
 }  // namespace nx

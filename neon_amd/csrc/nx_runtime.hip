@@ -99,7 +99,7 @@ int nx_init(int device)
                                              " not present (" + std::to_string(n) + " devices)");
     NX_CHECK_HIP(hipSetDevice(device));
     NX_CHECK_HIP(hipFree(nullptr));  // create the context now
-    return NX_OK;
+    return dense_noalloc_setup();
 }
 
 int nx_device_count(int* count)

@@ -59,6 +59,9 @@ def test_argument_errors_without_device_work():
                                              C.byref(pf), vals, None) == abi.NX_ERR_UNSUPPORTED
     assert abi.lib.nx_d3q19_step_dense_zouhe(C.byref(d), C.c_void_p(16), C.c_void_p(32), C.c_void_p(64),
                                              C.byref(p), None, None) == abi.NX_ERR_INVALID
+    # the allocation-free residual checks its arguments like the residual
+    assert abi.lib.nx_d3q19_residual_dense_noalloc(C.byref(d), C.c_void_p(16), C.c_void_p(32),
+                                                   None, C.c_void_p(64), None) == abi.NX_ERR_INVALID
     # container pair / batch variant: null handles and bad arguments
     assert abi.lib.nx_container_run_pair(None, None, 4, None) == abi.NX_ERR_INVALID
     assert abi.lib.nx_batch_set_variant(None, 1, 1) == abi.NX_ERR_INVALID
