@@ -10,9 +10,14 @@ resident in HBM when the timed region starts.
 
 Prints ONE JSON line: MLUPS over fluid cells (value), the roofline of the step
 kernel (algorithmic bytes 2*19*sizeof(storage) per fluid cell / average launch
-time from HIP events on the launch stream, vs 8 TB/s), the velocity error
-against a binary64 GPU run of the same case, and the CPU baseline (the CPU
-oracle, all host threads, on a bounded sample of the same workload).
+time from HIP events on the launch stream, vs 8 TB/s), the rate of the same
+step with the reference's float-path collision (mlups_compute_ref), the
+velocity error against a binary64 GPU run of the same case at a fixed horizon
+of 600 steps from rest (whatever --steps is), and two CPU baselines on a
+bounded sample of the same workload: cpu_baseline, the CPU oracle on one
+thread in the reference's serial z-y-x order (Neon's CPU launcher has its
+OpenMP pragmas commented out), and cpu_baseline_all_cores, the same oracle
+with OpenMP on every host thread.
 """
 from __future__ import annotations
 
@@ -61,6 +66,17 @@ BATCH = {
 }
 
 
+# The same 512 batched cavities run to steady state: FP32 deviation storage
+# (binary32 f - w_q; raw binary32 populations stall near a residual of 1e-5 on
+# small cavities), nx_batch_run to r = ||u_t - u_{t-K}|| / ||u_t|| < 1e-6 with
+# K = 100 (FP64 sums over bulk cells), converged members retire; capped at a
+# stated number of steps per member (--steps overrides the cap).
+CONVERGE = {
+    # name: (members, storage, tol, check interval K, default step cap)
+    "batch512_converge": (512, "f32dev", 1e-6, 100, 100000),
+}
+
+
 # Duct with Zou-He ends: X x Y x Y cells along x, bounce-back walls on the
 # four side faces, density faces rho_in / rho_out on the x planes (the
 # flow-over-sphere benchmark's inlet/outlet cells), nu = 0.02; stepped by
@@ -91,27 +107,77 @@ def traffic_from_profiles(kernel_tag: str):
         return None
 
 
-def cpu_baseline(N, storage, compute, omega, budget_s=12.0, max_steps=50):
-    """Oracle (the C restatement of the reference step) on the host, all
-    threads, a few steps of the same grid."""
+def cpu_baseline(N, storage, compute, omega, budget_s=12.0, max_steps=50, threads=0):
+    """Oracle (the C restatement of the reference step) on the host, a few
+    steps of the same grid.  threads=1: one host thread walking z, y, x in
+    order -- the reference's CPU launcher, whose OpenMP pragmas are commented
+    out (libNeonSet/include/Neon/set/LambdaExecutor.h:80-96, :119-144);
+    threads=0: every host thread (OpenMP over z slabs)."""
     from oracle import oracle as orc
     st = {"f64": orc.F64, "f32": orc.F32, "f32dev": orc.F32DEV, "f16dev": orc.F16DEV}[storage]
     cm = {"f64": orc.COMPUTE_F64, "f32": orc.COMPUTE_F32, "ref": orc.COMPUTE_REF}[compute]
-    o = orc.DenseOracle((N, N, N), st, cm)
-    o.setup_cavity(0.05)
-    o.run(1, omega)                       # first touch
-    steps, t0 = 0, time.perf_counter()
-    while True:
-        o.run(1, omega)
-        steps += 1
-        el = time.perf_counter() - t0
-        if el > budget_s or steps >= max_steps:
-            break
+    orc.set_threads(threads)
+    try:
+        o = orc.DenseOracle((N, N, N), st, cm)
+        o.setup_cavity(0.05)
+        o.run(1, omega)                       # first touch
+        steps, t0 = 0, time.perf_counter()
+        while True:
+            o.run(1, omega)
+            steps += 1
+            el = time.perf_counter() - t0
+            if el > budget_s or steps >= max_steps:
+                break
+        cores = int(orc.lib.or_get_threads())
+    finally:
+        orc.set_threads(0)
     fluid = (N - 2) ** 3
-    return {"value": fluid * steps / el / 1e6, "unit": "MLUPS", "cores": int(orc.lib.or_get_threads()),
+    how = ("one thread, z-y-x loop as the reference's serial CPU launcher" if threads == 1
+           else "OpenMP over z, all host threads")
+    return {"value": fluid * steps / el / 1e6, "unit": "MLUPS", "cores": cores,
             "kind": "port",
             "sample": f"{steps} steps of the {N}^3 cavity ({storage} storage, {compute} compute), "
-                      f"{el:.1f} s, CPU oracle (C restatement of LbmTools.h, OpenMP over z)"}
+                      f"{el:.1f} s, CPU oracle (C restatement of LbmTools.h, {how})"}
+
+
+ERROR_HORIZON = 600   # steps from rest for the velocity error (100 warm-up + 500 timed)
+
+
+def velocity_error(nx, N, storage, compute, ulb, omega, steps=ERROR_HORIZON):
+    """Velocity of the bench's storage/compute against a binary64 run of the
+    same cavity, both `steps` steps from rest (independent of --steps)."""
+    lat = nx.DenseLattice((N, N, N), storage, compute)
+    lat.setup_cavity(ulb)
+    lat.run(steps, omega)
+    _, u = lat.moments()
+    del lat
+    ref = nx.DenseLattice((N, N, N), "f64", "f64")
+    ref.setup_cavity(ulb)
+    ref.run(steps, omega)
+    _, u64 = ref.moments()
+    del ref
+    d = u.astype(np.float64) - u64
+    return {"steps": steps, "max_rel": float(np.abs(d).max() / np.abs(u64).max()),
+            "l2_rel": float(np.sqrt((d * d).sum() / (u64 * u64).sum())),
+            "norms": "max |u - u64| / max |u64| and ||u - u64||_2 / ||u64||_2 over all cells",
+            "reference": "binary64 GPU run of the same case (bit-exact with the CPU oracle)"}
+
+
+def compute_ref_rate(nx, N, storage, ulb, omega, steps, stream):
+    """MLUPS of the same container step with the reference's float-path
+    collision (NX_COMPUTE_REF: LbmTools.h:209-281 with its double literals)."""
+    lat = nx.DenseLattice((N, N, N), storage, "ref")
+    lat.setup_cavity(ulb, stream=stream.handle)
+    lat.run(20, omega, stream=stream.handle)
+    stream.sync()
+    e0, e1 = nx.Event(), nx.Event()
+    e0.record(stream)
+    lat.run(steps, omega, stream=stream.handle)
+    e1.record(stream)
+    e1.sync()
+    ms = e0.elapsed_ms(e1)
+    return {"value": round((N - 2) ** 3 * steps / (ms * 1e-3) / 1e6, 1), "unit": "MLUPS", "steps": steps,
+            "ms_per_step": round(ms / steps, 5), "compute": "ref (binary32 with the reference's binary64 literals)"}
 
 
 def copy_calibration(nx):
@@ -275,18 +341,9 @@ def batched(nx, a):
     warmup += warmup % 2
     ulb = 0.05
     nx.init(0)
-    rng = np.random.default_rng(2025)
     t0 = time.perf_counter()
-    members, fluid = [], 0
-    for k in range(n):
-        e = int(rng.choice(np.arange(32, 97, 8)))
-        re = float(rng.uniform(50.0, 1000.0))
-        nb = int(rng.integers(0, 5))
-        cls = nx.cavity_classes((e, e, e), nb, 4, 16, seed=int(rng.integers(0, 2 ** 63)))
-        fluid += int((cls == 2).sum())
-        nu = ulb * e / re
-        members.append({"dim": (e, e, e), "classes": cls, "omega": 1.0 / (3.0 * nu + 0.5),
-                        "ulb": ulb, "ulid": (1.0, 0.0, 0.0)})
+    members = nx.batch_cavities(n, seed=2025, ulb=ulb)
+    fluid = sum(int((m["classes"] == 2).sum()) for m in members)
     b = nx.CavityBatch(members, storage)
     t_setup = time.perf_counter() - t0
     if a.edge is not None:
@@ -338,12 +395,87 @@ def batched(nx, a):
     print(json.dumps(out), flush=True)
 
 
+def batch_converge(nx, a):
+    n, storage, tol, every, cap = CONVERGE[a.workload]
+    max_steps = cap if a.steps is None else a.steps
+    max_steps -= max_steps % every
+    chunk = 5000
+    ulb = 0.05
+    nx.init(0)
+    t0 = time.perf_counter()
+    members = nx.batch_cavities(n, seed=2025, ulb=ulb)
+    fluid = np.array([int((m["classes"] == 2).sum()) for m in members])
+    edge = np.array([m["dim"][0] for m in members])
+    re = np.array([m["re"] for m in members])
+    b = nx.CavityBatch(members, storage)
+    b.init()
+    t_setup = time.perf_counter() - t0
+    m0 = np.array([b.mass(k) for k in range(n)])
+    nx.sync()
+    tw = time.perf_counter()
+    done, left = 0, n
+    while done < max_steps and left > 0:
+        k = min(chunk, max_steps - done)
+        steps, res, left = b.run(max_steps=k, check_every=every, tol=tol)
+        done += k
+        print(f"batch512_converge: {done} steps, {n - left} of {n} converged, "
+              f"{time.perf_counter() - tw:.1f} s", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - tw
+    m1 = np.array([b.mass(k) for k in range(n)])
+    conv = np.isfinite(res) & (res < tol)
+    div = ~np.isfinite(res)              # the flow blew up: the member stopped stepping
+    run_on = ~conv & ~div
+    updates = float((fluid * steps).sum())
+    bins = list(range(32, 97, 8))
+    by_edge = {str(e): {"members": int((edge == e).sum()), "converged": int(conv[edge == e].sum()),
+                        "diverged": int(div[edge == e].sum()),
+                        "median_steps": int(np.median(steps[(edge == e) & conv])) if conv[edge == e].any() else None}
+               for e in bins}
+    re_bins = [(50, 200), (200, 400), (400, 600), (600, 800), (800, 1000)]
+    by_re = {f"{lo}-{hi}": {"members": int(((re >= lo) & (re < hi)).sum()),
+                            "converged": int(conv[(re >= lo) & (re < hi)].sum()),
+                            "diverged": int(div[(re >= lo) & (re < hi)].sum())} for lo, hi in re_bins}
+    # cell Reynolds number U dx / nu = Re / edge: BGK loses stability as tau -> 1/2
+    cell_re = re / edge
+    hist_edges = [0, 2000, 5000, 10000, 20000, 50000, 100000, 200000, 500000, 1000000]
+    h, _ = np.histogram(steps[conv], bins=hist_edges)
+    ok = np.isfinite(m1)
+    drift = np.abs(m1[ok] - m0[ok]) / m0[ok]
+    out = {
+        "metric": METRIC, "value": round(updates / wall / 1e6, 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": int(steps.max()), "warmup": 0, "ms_per_step": None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": storage,
+        "data": "synthetic (seeded cavities with box obstacles, started at rest)",
+        "config": {"workload": f"{n} batched lid-driven cavities, edge U{{32..96}} step 8, Re U[50,1000], "
+                               f"0-4 boxes U{{4..16}}, {storage} storage, run to residual {tol:g} "
+                               f"(K={every}) or {max_steps} steps, converged members retire",
+                   "members": n, "fluid_cells": int(fluid.sum()), "setup_s": round(t_setup, 2),
+                   "launch": b.launch(), "device": nx.device_name()},
+        "wall_s": round(wall, 2),
+        "converged": int(conv.sum()), "unconverged": int(n - conv.sum()),
+        "diverged": int(div.sum()),
+        "cell_reynolds": {"diverged_min": float(cell_re[div].min()) if div.any() else None,
+                          "stable_max": float(cell_re[~div].max())},
+        "step_cap": max_steps,
+        "steps_histogram": {"bin_edges": hist_edges, "converged_members": h.tolist()},
+        "by_edge": by_edge, "by_re": by_re,
+        "residual_of_unconverged": {"min": float(res[run_on].min()) if run_on.any() else None,
+                                    "median": float(np.median(res[run_on])) if run_on.any() else None,
+                                    "max": float(res[run_on].max()) if run_on.any() else None},
+        "mass_drift_rel": {"members": int(ok.sum()), "max": float(drift.max()),
+                           "median": float(np.median(drift))},
+        "cell_updates": updates,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="cavity256_f32", choices=sorted(WORKLOADS) + sorted(POROUS) + sorted(BATCH) + sorted(ZOUHE))
+    ap.add_argument("--workload", default="cavity256_f32",
+                    choices=sorted(WORKLOADS) + sorted(POROUS) + sorted(BATCH) + sorted(CONVERGE) + sorted(ZOUHE))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--autotune", action="store_true",
@@ -375,6 +507,8 @@ def main():
         return porous(nx, a)
     if a.workload in BATCH:
         return batched(nx, a)
+    if a.workload in CONVERGE:
+        return batch_converge(nx, a)
     if a.workload in ZOUHE:
         return zouhe_duct(nx, a)
 
@@ -422,22 +556,20 @@ def main():
     tag = f"d3q19_dense_{storage}_{N}"
     traffic = traffic_from_profiles(tag)
 
+    launch = lat.tuning()
+    del lat
+    ref_rate = None
+    if storage == "f32" and compute == "f32":
+        ref_rate = compute_ref_rate(nx, N, storage, ulb, omega, min(steps, 200), stream)
+
     err = None
     if not a.no_error and storage != "f64":
-        _, u = lat.moments(stream=stream.handle)
-        ref = nx.DenseLattice((N, N, N), "f64", "f64")
-        ref.setup_cavity(ulb)
-        ref.run(warmup + steps, omega)
-        _, u64 = ref.moments()
-        d = u.astype(np.float64) - u64
-        err = {"max_rel": float(np.abs(d).max() / np.abs(u64).max()),
-               "l2_rel": float(np.sqrt((d * d).sum() / (u64 * u64).sum())),
-               "reference": "binary64 GPU run of the same case (bit-exact with the CPU oracle)"}
-        del ref
+        err = velocity_error(nx, N, storage, compute, ulb, omega)
 
-    cpu = None
+    cpu = cpu_all = None
     if not a.no_cpu:
-        cpu = cpu_baseline(N, storage, compute, omega)
+        cpu = cpu_baseline(N, storage, compute, omega, threads=1, max_steps=40)
+        cpu_all = cpu_baseline(N, storage, compute, omega)
 
     out = {
         "metric": METRIC,
@@ -456,15 +588,17 @@ def main():
                                f"{storage} storage + {compute} collision, Re={re:g}, "
                                f"lid u=({ulb},0,0), {steps} timed steps after {warmup} warm-up",
                    "grid": "dGrid", "N": N, "storage": storage, "compute": compute,
-                   "omega": omega, "device": nx.device_name(), "launch": lat.tuning()},
+                   "omega": omega, "device": nx.device_name(), "launch": launch},
         "mlups_ref_convention": round(mlups_ref, 1),
+        "mlups_compute_ref": ref_rate,
         "wall_s": round(wall, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic,
-                     "kernel": f"{dense_kernel_name(lat.tuning())}<{storage},{compute}> ({b_alg} B/fluid cell)"},
+                     "kernel": f"{dense_kernel_name(launch)}<{storage},{compute}> ({b_alg} B/fluid cell)"},
         "velocity_error_vs_f64": err,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
     }
     print(json.dumps(out), flush=True)
 

@@ -5,4 +5,4 @@ the C ABI of include/neon_amd.h).  This package only binds it.
 """
 from . import abi  # noqa: F401  (raises ImportError when the library is not built)
 from .lattice import (BlockGrid, BlockLattice, CavityBatch, DenseLattice, DeviceBuffer,  # noqa: F401
-                      Event, Stream, cavity_classes, device_name, init, params, spheres, sync)
+                      Event, Stream, batch_cavities, cavity_classes, device_name, init, params, spheres, sync)

@@ -59,6 +59,26 @@ def cavity_classes(dim, n_boxes: int = 0, emin: int = 4, emax: int = 16, seed: i
     return out
 
 
+def batch_cavities(n: int = 512, seed: int = 2025, edges=tuple(range(32, 97, 8)),
+                   re_range=(50.0, 1000.0), boxes=(0, 4), box_edges=(4, 16), ulb: float = 0.05):
+    """The batched-cavity workload: n lid-driven cubes, edge drawn from
+    `edges`, Re uniform in re_range, boxes[0]..boxes[1] seeded bounce-back
+    boxes with edges uniform in box_edges, lid ulb along x, nu = ulb * edge /
+    Re.  One generator draws (edge, Re, box count, box seed) per member in
+    that order.  Returns member dicts for CavityBatch (plus "re")."""
+    rng = np.random.default_rng(seed)
+    members = []
+    for _ in range(int(n)):
+        e = int(rng.choice(np.asarray(edges)))
+        re = float(rng.uniform(*re_range))
+        nb = int(rng.integers(boxes[0], boxes[1] + 1))
+        cls = cavity_classes((e, e, e), nb, box_edges[0], box_edges[1], seed=int(rng.integers(0, 2 ** 63)))
+        nu = ulb * e / re
+        members.append({"dim": (e, e, e), "classes": cls, "omega": 1.0 / (3.0 * nu + 0.5),
+                        "ulb": ulb, "ulid": (1.0, 0.0, 0.0), "re": re, "boxes": nb})
+    return members
+
+
 class DeviceBuffer:
     """Owning device allocation."""
 
@@ -565,6 +585,14 @@ class CavityBatch:
         check(lib.nx_d2h(out.ctypes.data, p, out.nbytes, None))
         check(lib.nx_stream_sync(None))
         return out
+
+    def mass(self, k: int) -> float:
+        """FP64 mass of member k's latest populations (nx_d3q19_mass_dense on
+        its dense view)."""
+        d, p, f = self.member(k)
+        out = DeviceBuffer(8)
+        check(lib.nx_d3q19_mass_dense(C.byref(d), p, f, out.ptr, None))
+        return float(out.download(np.float64, 1)[0])
 
     def moments(self, k: int):
         d, p, f = self.member(k)

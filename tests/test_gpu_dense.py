@@ -159,6 +159,40 @@ def test_fp16_dev_velocity_error_vs_f64(nx):
     assert errs["f32"] < 1e-4 and errs["f32dev"] < 1e-4 and errs["f16dev"] < 1e-3
 
 
+def test_fp16_dev_velocity_error_at_benchmark_size(nx):
+    """The FP16 deviation storage's accuracy target at the benchmark case:
+    256^3 cavity, Re 1000, 600 steps from rest, against the binary64 run.
+    The 1e-3 bound holds for the relative L2 norm over all cells
+    (||u - u64||_2 / ||u64||_2, measured 3.2e-4) and for the max norm
+    relative to max |u64| away from the two cell layers next to the walls
+    (measured 6.3e-4).  At the cells beside the lid's corner edges, where the
+    lid meets the side walls (a velocity discontinuity), it is 3.1e-3: there
+    binary16 rounds large non-equilibrium parts of nearly steady values the
+    same way every step."""
+    N, steps = 256, 600
+    omega = 1.0 / (3.0 * 0.05 * N / 1000.0 + 0.5)
+    ref = nx.DenseLattice((N, N, N), "f64")
+    ref.setup_cavity(0.05)
+    ref.run(steps, omega)
+    _, u64 = ref.moments()
+    del ref
+    g = nx.DenseLattice((N, N, N), "f16dev", "f32")
+    g.setup_cavity(0.05)
+    g.run(steps, omega)
+    _, u = g.moments()
+    del g
+    d = np.sqrt(((u.astype(np.float64) - u64) ** 2).sum(0))
+    umax = np.sqrt((u64 ** 2).sum(0)).max()
+    l2 = float(np.sqrt((d ** 2).sum() / (u64 ** 2).sum()))
+    max_all = float(d.max() / umax)
+    max_inner = float(d[2:-2, 2:-2, 2:-2].max() / umax)
+    print(f"FP16 deviation vs FP64 at 256^3: L2 {l2:.3e}, max {max_all:.3e}, "
+          f"max beyond two wall layers {max_inner:.3e}")
+    assert l2 < 1e-3
+    assert max_inner < 1e-3
+    assert max_all < 5e-3
+
+
 @pytest.mark.parametrize("boundary", [0, 1])
 @pytest.mark.parametrize("st,cm", [("f64", "f64"), ("f32", "f32"), ("f32", "ref"), ("f16dev", "f32")])
 def test_container_equals_plain_step(nx, orc, st, cm, boundary):
