@@ -77,6 +77,18 @@ CONVERGE = {
 }
 
 
+# Two-level refined lid-driven cavity (the lbmMultiRes lid problem at depth
+# 2, BGK D3Q19): 256^3 domain, level 0 = the shell within a quarter of the
+# half width of the faces (sdfCube > -1/4), level 1 the interior; ulb 0.04,
+# Re 1000 on the top level's length.  A step is one coarse step: collide,
+# store, stream at both levels, level 0 twice.
+MRES = {
+    # name: (N, level-0 sdf threshold, storage, default steps, default warmup)
+    "mres256_f32": (256, -0.25, "f32", 100, 10),
+    "mres256_f64": (256, -0.25, "f64", 50, 5),
+}
+
+
 # Duct with Zou-He ends: X x Y x Y cells along x, bounce-back walls on the
 # four side faces, density faces rho_in / rho_out on the x planes (the
 # flow-over-sphere benchmark's inlet/outlet cells), nu = 0.02; stepped by
@@ -469,13 +481,72 @@ def batch_converge(nx, a):
     print(json.dumps(out), flush=True)
 
 
+def mres_cavity(nx, a):
+    N, thr, storage, dsteps, dwarm = MRES[a.workload]
+    steps = dsteps if a.steps is None else a.steps
+    warmup = dwarm if a.warmup is None else a.warmup
+    from neon_amd import mgrid as mg
+    nx.init(0)
+    dim = (N, N, N)
+    t0 = time.perf_counter()
+    lam0, lam1 = mg.cavity_lambdas(dim, (0.0, thr, -1.0))
+    re = 1000.0
+    ulb, om0 = mg.cavity_parameters(dim, re, storage)
+    g = mg.MGrid(dim, lam0, lam1, storage)
+    g.setup_cavity(ulb)
+    t_setup = time.perf_counter() - t0
+    info = g.info()
+    stream = nx.Stream()
+    g.run(warmup, om0, stream=stream.handle)
+    stream.sync()
+    e0, e1 = nx.Event(), nx.Event()
+    nx.sync()
+    tw = time.perf_counter()
+    e0.record(stream)
+    g.run(steps, om0, stream=stream.handle)
+    e1.record(stream)
+    e1.sync()
+    wall = time.perf_counter() - tw
+    ms = e0.elapsed_ms(e1)
+    ms_step = ms / steps
+    leaf0, leaf1 = info["leaf"]
+    # the reference's count (lbmMultiRes.h:385-391): level l's leaves updated
+    # 2^(depth-1-l) times per coarse step
+    updates = 2 * leaf0 + leaf1
+    mlups = updates * steps / (ms * 1e-3) / 1e6
+    emlups = 2 * N ** 3 * steps / (ms * 1e-3) / 1e6
+    s_b = STORAGE_BYTES[storage]
+    b_alg = 4 * 19 * s_b                       # collide r+w and stream r+w of 19 populations
+    achieved = updates * b_alg / (ms_step * 1e-3) / 1e9
+    out = {
+        "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
+        "steps": steps, "warmup": warmup, "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": storage,
+        "data": "synthetic (two-level refined lid-driven cavity started at rest)",
+        "config": {"workload": f"two-level refined lid-driven cavity {N}^3 (level 0: sdfCube > {thr}, level 1 "
+                               f"inside), D3Q19 BGK, {storage}, ulb {ulb:.4g}, Re {re:g}, {steps} timed coarse "
+                               f"steps after {warmup}",
+                   "leaves": info["leaf"], "active": info["active"], "blocks": info["blocks"],
+                   "store_links": info["store_links"], "setup_s": round(t_setup, 2), "omega0": om0,
+                   "device": nx.device_name()},
+        "mlups_effective": round(emlups, 1),
+        "wall_s": round(wall, 4),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": f"k_mr_collide + k_mr_stream<{storage}> ({b_alg} B per leaf update)"},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="cavity256_f32",
-                    choices=sorted(WORKLOADS) + sorted(POROUS) + sorted(BATCH) + sorted(CONVERGE) + sorted(ZOUHE))
+                    choices=sorted(WORKLOADS) + sorted(POROUS) + sorted(BATCH) + sorted(CONVERGE) + sorted(ZOUHE)
+                    + sorted(MRES))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-error", action="store_true", help="skip the FP64 comparison run")
     ap.add_argument("--autotune", action="store_true",
@@ -511,6 +582,8 @@ def main():
         return batch_converge(nx, a)
     if a.workload in ZOUHE:
         return zouhe_duct(nx, a)
+    if a.workload in MRES:
+        return mres_cavity(nx, a)
 
     N, storage, compute, re, dsteps, dwarm = WORKLOADS[a.workload]
     steps = dsteps if a.steps is None else a.steps

@@ -95,6 +95,7 @@ typedef struct nx_event_s* nx_event;
 typedef struct nx_bgrid_s* nx_bgrid;
 typedef struct nx_container_s* nx_container;
 typedef struct nx_batch_s* nx_batch;
+typedef struct nx_mgrid_s* nx_mgrid;
 
 /* Dense grid: one partition of a dGrid (dGrid_imp.h:7-185). */
 typedef struct {
@@ -385,6 +386,47 @@ int nx_batch_launch_grid(nx_batch b, uint32_t* grid, int32_t* xcd_chunked);
 int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int32_t* best_wgs_per_cu,
                       float* best_ms);
 int nx_batch_destroy(nx_batch b);
+
+/* ---- two-level grid refinement (Neon's mGrid with depth 2, BGK D3Q19) ---
+ * Level 0 is the finest level (the domain dim[3]); a level-1 voxel v covers
+ * the level-0 voxels 2v + {0,1}^3.  Each level is stored as a bGrid of 8^3
+ * blocks (AoSoA populations, as nx_d3q19_step_block).
+ * nx_mgrid_create replaces the mGrid constructor for depth 2
+ *   (libNeonDomain/src/domain/details/mGrid/mGrid.cpp:89-264): lam0 is the
+ *   level-0 activation lambda at every level-0 voxel, lam1 the level-1
+ *   lambda at every level-1 voxel (evaluated at its level-0 origin 2v),
+ *   uint8, x fastest.  Every 2^3 group holding an active voxel becomes
+ *   active (and so does its parent), and refined level-1 voxels whose 26
+ *   neighbours are all refined are removed.  storage NX_F64 or NX_F32.
+ * nx_mgrid_state: the state byte of every voxel of level 0 / level 1
+ *   (dense, x fastest): bit 0 active, bit 1 refined (has children), bits
+ *   2..3 the NX_CELL_* class once nx_mgrid_cavity has run.
+ * nx_mgrid_cavity replaces initLidDrivenCavity + initSumStore
+ *   (apps/lbmMultiRes/lidDrivenCavity.h:8-78, init.h:8-86): leaf voxels of
+ *   level 0 on the domain faces are bounce-back, y = Y-1 the lid moving at
+ *   (ulb, 0, 0); populations w_q / 0 / -6 w_q c_q.ulid in both fields; the
+ *   coalescence weights 1 / (2 n) of every refined level-1 voxel.
+ * nx_mgrid_step replaces nonUniformTimestepRecursive for two levels
+ *   (apps/lbmMultiRes/lbmMultiRes.h:141-263) with the baseline operation
+ *   order and the fine store: collide level 1 (collide.h:8-149), twice
+ *   (collide level 0, store level 0 into level 1 (store.h:98-163), stream
+ *   level 0 with explosion (stream.h:170-225)), stream level 1 with
+ *   coalescence.  omega0 is the level-1 relaxation; level 0 uses
+ *   2 omega0 / (4 - omega0) (computeOmega, util.h:40-45).  The fine-to-coarse
+ *   sums are added in the fine voxels' z, y, x order (the reference adds
+ *   with atomics in no fixed order), so results are deterministic.
+ * nx_mgrid_fields copies fin (which 0) or fout (which 1) of one level into
+ *   a host array [19][Z][Y][X] of that level's dimensions; entries of
+ *   inactive voxels are not written.  Synchronises the device. */
+int nx_mgrid_create(const int32_t dim[3], const uint8_t* lam0, const uint8_t* lam1, int32_t storage,
+                    nx_mgrid* g);
+int nx_mgrid_info(nx_mgrid g, uint32_t n_blocks[2], uint64_t n_active[2], uint64_t n_leaf[2],
+                  uint64_t* n_store);
+int nx_mgrid_state(nx_mgrid g, uint8_t* st0, uint8_t* st1);
+int nx_mgrid_cavity(nx_mgrid g, double ulb);
+int nx_mgrid_step(nx_mgrid g, double omega0, int32_t n_steps, nx_stream s);
+int nx_mgrid_fields(nx_mgrid g, int32_t level, int32_t which, void* dense);
+int nx_mgrid_destroy(nx_mgrid g);
 
 /* ---- geometry (benchmark workloads; no counterpart in the reference) ------
  * Union of n seeded spheres, centres uniform in the box, radii uniform in

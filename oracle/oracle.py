@@ -69,6 +69,11 @@ _SIG = {
     "or_moments_block": (None, [_P, _P, _P, _P, _P]),
     "or_mass_block": (_D, [_P, _P, _P]),
     "or_run_block": (None, [_P, _P, _P, _P, _D, _P, C.c_int]),
+    "or_mres_dims": (None, [_P, _P]),
+    "or_mres_build": (None, [_P, _P, _P, _P, _P]),
+    "or_mres_setup": (None, [_P, C.c_int, _D, _P, _P, _P, _P, _P, _P, _P]),
+    "or_mres_omega": (_D, [_D, C.c_int, C.c_int]),
+    "or_mres_step": (None, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _D]),
 }
 
 
@@ -274,3 +279,59 @@ class BlockOracle:
     def active_slots(self):
         bits = np.unpackbits(self.masks.view(np.uint8), bitorder="little")
         return bits.reshape(-1).astype(bool)
+
+
+MR_ACTIVE, MR_REFINED, MR_CLS_SHIFT = 1, 2, 2
+
+
+class MresOracle:
+    """Two-level grid refinement LBM (mres_oracle.c, restating
+    apps/lbmMultiRes for depth 2, BGK D3Q19).  lam0 / lam1: the activation
+    lambdas of level 0 (every level-0 voxel) and level 1 (every level-1
+    voxel, evaluated at its level-0 origin), uint8, x fastest.  Fields are
+    dense per level, [19][z][y][x]; st0 / st1 the per-voxel state bytes."""
+
+    def __init__(self, dim, lam0, lam1, storage=F64):
+        self.dim = tuple(int(v) for v in dim)
+        self._d0 = (C.c_int32 * 3)(*self.dim)
+        d1 = (C.c_int32 * 3)()
+        lib.or_mres_dims(self._d0, d1)
+        self.dim1 = tuple(int(v) for v in d1)
+        self._d1 = d1
+        self.n0 = int(np.prod(self.dim))
+        self.n1 = int(np.prod(self.dim1))
+        self.storage = storage
+        lam0 = np.ascontiguousarray(lam0, np.uint8).reshape(-1)
+        lam1 = np.ascontiguousarray(lam1, np.uint8).reshape(-1)
+        assert lam0.size == self.n0 and lam1.size == self.n1
+        self.st0 = np.zeros(self.n0, np.uint8)
+        self.st1 = np.zeros(self.n1, np.uint8)
+        lib.or_mres_build(self._d0, _ptr(lam0), _ptr(lam1), _ptr(self.st0), _ptr(self.st1))
+
+    def active(self, level):
+        return ((self.st1 if level else self.st0) & MR_ACTIVE) != 0
+
+    def refined(self):
+        return (self.st1 & MR_REFINED) != 0
+
+    def setup_cavity(self, ulb):
+        dt = NP_STORAGE[self.storage]
+        self.fin = [np.zeros(19 * self.n0, dt), np.zeros(19 * self.n1, dt)]
+        self.fout = [np.zeros(19 * self.n0, dt), np.zeros(19 * self.n1, dt)]
+        self.ss1 = np.zeros(19 * self.n1, np.float32)
+        lib.or_mres_setup(self._d0, self.storage, float(ulb), _ptr(self.st0), _ptr(self.st1),
+                          _ptr(self.fin[0]), _ptr(self.fout[0]), _ptr(self.fin[1]), _ptr(self.fout[1]),
+                          _ptr(self.ss1))
+
+    def classes(self, level):
+        return ((self.st1 if level else self.st0) >> MR_CLS_SHIFT) & 3
+
+    def run(self, n, omega0):
+        for _ in range(int(n)):
+            lib.or_mres_step(self._d0, self.storage, _ptr(self.st0), _ptr(self.st1),
+                             _ptr(self.fin[0]), _ptr(self.fout[0]), _ptr(self.fin[1]),
+                             _ptr(self.fout[1]), _ptr(self.ss1), float(omega0))
+
+    @staticmethod
+    def omega(omega0, level, num_levels=2):
+        return float(lib.or_mres_omega(float(omega0), int(level), int(num_levels)))

@@ -1,0 +1,75 @@
+"""Two-level grid refinement on the GPU (nx_mgrid_*, the lbmMultiRes BGK
+path for depth 2) against the CPU oracle: every population of both fields of
+both levels bit for bit after several coarse steps, for the reference's two
+field types; and at 256^3 the properties the cavity keeps (mass, z-mirror
+symmetry)."""
+import numpy as np
+import pytest
+
+import neon_amd.mgrid as mg
+
+pytestmark = pytest.mark.gpu
+
+CASES = [((48, 48, 48), 1 / 3, 100.0, 30), ((37, 30, 41), 0.3, 50.0, 20), ((64, 64, 64), 0.25, 400.0, 12),
+         ((20, 26, 18), 0.45, 10.0, 40)]
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("dim,a,re,steps", CASES)
+def test_mgrid_equals_oracle(nx, orc, storage, dim, a, re, steps):
+    lam0, lam1 = mg.cavity_lambdas(dim, (0.0, -a, -1.0))
+    ulb, om0 = mg.cavity_parameters(dim, re, storage)
+    print(f"mgrid {dim} a={a:.3f} {storage} Re={re} omega0={om0:.6f} steps={steps}", flush=True)
+    g = mg.MGrid(dim, lam0, lam1, storage)
+    g.setup_cavity(ulb)
+    m = orc.MresOracle(dim, lam0, lam1, orc.F64 if storage == "f64" else orc.F32)
+    m.setup_cavity(ulb)
+    s0, s1 = g.state()
+    assert np.array_equal(s0, m.st0) and np.array_equal(s1, m.st1)
+    assert g.info()["store_links"] > 0
+    g.run(steps, om0)
+    m.run(steps, om0)
+    for lvl in (0, 1):
+        act = np.tile(m.active(lvl), 19)
+        for which, ref in ((0, m.fin[lvl]), (1, m.fout[lvl])):
+            got = g.fields(lvl, which)
+            n_diff = int(np.count_nonzero(got[act].view(np.uint8) != ref[act].view(np.uint8)))
+            assert n_diff == 0, (lvl, which, n_diff, np.abs(got[act] - ref[act]).max())
+
+
+def test_mgrid_256_properties(nx, orc):
+    """256^3 two-level cavity (level-0 shell a quarter of the half width),
+    binary64, 100 coarse steps: finite, the bulk mass of the leaves changes
+    by less than 1e-3 relative (the interface coupling does not conserve mass
+    exactly), and the flow is z-mirror symmetric (u_x even, u_z odd) to
+    round-off."""
+    N = 256
+    dim = (N, N, N)
+    lam0, lam1 = mg.cavity_lambdas(dim, (0.0, -0.25, -1.0))
+    ulb, om0 = mg.cavity_parameters(dim, 1000.0, "f64")
+    g = mg.MGrid(dim, lam0, lam1, "f64")
+    g.setup_cavity(ulb)
+    s0, s1 = g.state()
+    c, _, _ = orc.lattice()
+    bulk0 = ((s0 & 1) == 1) & (((s0 >> 2) & 3) == 2)
+    leaf1 = (s1 & 3) == 1
+
+    def mass():
+        f0 = g.fields(0).reshape(19, -1)
+        f1 = g.fields(1).reshape(19, -1)
+        return f0[:, bulk0].sum() + 8 * f1[:, leaf1].sum()
+
+    m0 = mass()
+    g.run(100, om0)
+    m1 = mass()
+    print(f"mgrid 256^3: leaves {g.info()['leaf']}, mass {m0:.10e} -> {m1:.10e}")
+    assert np.isfinite(m1) and abs(m1 - m0) / m0 < 1e-3
+    f0 = g.fields(0).reshape(19, N, N, N)
+    rho = f0.sum(0)
+    ux = np.tensordot(c[:, 0].astype(float), f0, 1) / np.where(rho == 0, 1, rho)
+    uz = np.tensordot(c[:, 2].astype(float), f0, 1) / np.where(rho == 0, 1, rho)
+    b = bulk0.reshape(N, N, N)
+    assert np.abs(ux[b]).max() > 1e-3
+    sym = b & b[::-1]
+    assert np.abs(ux - ux[::-1])[sym].max() < 1e-9 * np.abs(ux[b]).max()
+    assert np.abs(uz + uz[::-1])[sym].max() < 1e-9 * np.abs(ux[b]).max()
