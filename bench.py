@@ -211,6 +211,48 @@ def copy_calibration(nx):
     print(json.dumps({"copy_bytes_per_launch": 2 * n, "ms": ms, "GBs": 2 * n / ms / 1e6}))
 
 
+def shifted_copy_calibration(nx, N=256, reps=20):
+    """The dense step's access pattern without arithmetic (nx_shifted_copy:
+    19 planes read at the upstream cells, 19 written) on an N^3 grid for 2-,
+    4- and 8-byte elements: the ceiling of the pull-streaming step on SoA
+    planes, beside the plain 1 GiB copy."""
+    import ctypes as C
+    from neon_amd import abi
+    s = nx.Stream()
+    out = {"grid": [N, N, N]}
+    for eb in (2, 4, 8):
+        nbytes = 19 * N ** 3 * eb
+        a, b = nx.DeviceBuffer(nbytes), nx.DeviceBuffer(nbytes)
+        a.zero()
+        dim = (C.c_int32 * 3)(N, N, N)
+        for _ in range(3):
+            abi.check(abi.lib.nx_shifted_copy(b.ptr, a.ptr, dim, eb, s.handle))
+        e0, e1 = nx.Event(), nx.Event()
+        e0.record(s)
+        for _ in range(reps):
+            abi.check(abi.lib.nx_shifted_copy(b.ptr, a.ptr, dim, eb, s.handle))
+        e1.record(s)
+        e1.sync()
+        ms = e0.elapsed_ms(e1) / reps
+        out[f"{eb}B"] = {"us": round(ms * 1e3, 1), "GBs": round(2 * nbytes / ms / 1e6, 1),
+                         "frac_of_8TBs": round(2 * nbytes / ms / 1e6 / HBM_PEAK_GBS, 4)}
+        del a, b
+    n = 1 << 30
+    a, b = nx.DeviceBuffer(n), nx.DeviceBuffer(n)
+    a.zero()
+    for _ in range(3):
+        abi.check(abi.lib.nx_copy_kernel(b.ptr, a.ptr, n, s.handle))
+    e0, e1 = nx.Event(), nx.Event()
+    e0.record(s)
+    for _ in range(reps):
+        abi.check(abi.lib.nx_copy_kernel(b.ptr, a.ptr, n, s.handle))
+    e1.record(s)
+    e1.sync()
+    ms = e0.elapsed_ms(e1) / reps
+    out["plain_copy_1GiB"] = {"us": round(ms * 1e3, 1), "GBs": round(2 * n / ms / 1e6, 1)}
+    print(json.dumps({"shifted_copy_calibration": out}), flush=True)
+
+
 def porous(nx, a):
     N, storage, compute, dsteps, dwarm = POROUS[a.workload]
     steps = dsteps if a.steps is None else a.steps
@@ -430,7 +472,7 @@ def batch_converge(nx, a):
         k = min(chunk, max_steps - done)
         steps, res, left = b.run(max_steps=k, check_every=every, tol=tol)
         done += k
-        print(f"batch512_converge: {done} steps, {n - left} of {n} converged, "
+        print(f"batch512_converge: {done} steps, {n - left} of {n} finished (converged or diverged), "
               f"{time.perf_counter() - tw:.1f} s", file=sys.stderr, flush=True)
     wall = time.perf_counter() - tw
     m1 = np.array([b.mass(k) for k in range(n)])
@@ -561,6 +603,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run the plain copy kernel (1 GiB, 10 launches): the "
                          "reference for scaling FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--shifted", action="store_true",
+                    help="with --copy-calibration: time the step's 19-plane shifted copy (no arithmetic) "
+                         "at 256^3 for 2/4/8-byte elements, the ceiling of the pull-streaming step")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
@@ -571,7 +616,11 @@ def main():
     import neon_amd as nx
 
     if a.copy_calibration:
-        copy_calibration(nx)
+        if a.shifted:
+            nx.init(0)
+            shifted_copy_calibration(nx)
+        else:
+            copy_calibration(nx)
         return
 
     if a.workload in POROUS:

@@ -168,6 +168,12 @@ int nx_host_alloc(size_t bytes, void** host); /* pinned */
 /* Plain device-to-device streaming copy kernel (16-byte aligned); no
  * counterpart in the reference: the achievable-bandwidth yardstick. */
 int nx_copy_kernel(void* dst, const void* src, size_t bytes, nx_stream s);
+/* The dense step's access pattern without arithmetic (no counterpart in the
+ * reference): 19 SoA planes of dim[0]*dim[1]*dim[2] elements of elem_bytes
+ * (2, 4 or 8); every cell reads plane q at its upstream cell i - c_q
+ * (clamped to the grid) and writes plane q at i.  Its bandwidth is the
+ * ceiling of the pull-streaming step on this layout. */
+int nx_shifted_copy(void* dst, const void* src, const int32_t dim[3], int32_t elem_bytes, nx_stream s);
 int nx_host_free(void* host);
 
 /* Bytes of one population field (19 cardinalities) for a storage type. */
@@ -379,8 +385,10 @@ int nx_batch_get_launch(nx_batch b, int32_t* wg, int32_t* wgs_per_cu, int32_t* x
 /* Workgroups of the persistent step for n_tiles tiles on n_cu CUs with
  * wgs_per_cu per CU (0: one per tile): min(n_tiles, n_cu * wgs_per_cu) in
  * 64-bit arithmetic, at least 1.  Host-only, no device work.
- * nx_batch_launch_grid reports the grid and whether tiles are dealt in XCD
- * chunks for the batch's current launch. */
+ * nx_batch_launch_grid reports the grid and how the batch's current launch
+ * deals tiles to the XCDs: 0 round robin, 1 persistent grid in XCD chunks,
+ * 2 one workgroup per tile with each XCD taking a contiguous range of tiles
+ * (xcd_chunks = 1 with wgs_per_cu = 0). */
 int nx_batch_grid_size(uint32_t n_tiles, int32_t n_cu, int32_t wgs_per_cu, uint32_t* grid);
 int nx_batch_launch_grid(nx_batch b, uint32_t* grid, int32_t* xcd_chunked);
 int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int32_t* best_wgs_per_cu,

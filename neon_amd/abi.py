@@ -101,6 +101,7 @@ SIGNATURES = {
     "nx_host_alloc": (_I, [_SZ, C.POINTER(_P)]),
     "nx_host_free": (_I, [_P]),
     "nx_copy_kernel": (_I, [_P, _P, _SZ, _P]),
+    "nx_shifted_copy": (_I, [_P, _P, _P, _I32, _P]),
     "nx_d3q19_get_tuning_dense": (_I, [_DP, C.POINTER(Tuning)]),
     "nx_d3q19_set_tuning_dense": (_I, [_DP, C.POINTER(Tuning)]),
     "nx_d3q19_iteration_dense": (_I, [_DP, _P, _P, _P, _PP, C.POINTER(_P)]),

@@ -62,8 +62,8 @@ struct nx_batch_s {
     float* u_prev = nullptr;         // 3 per cell
     // launch configuration
     int wg = 256;
-    int wgs_per_cu = 2;   // the autotuner's choice for 512 cavities of 32..96 cells on MI355X
-    int xcd_chunks = 1;
+    int wgs_per_cu = 0;   // one workgroup per tile (the autotuner's choice for FP32, below)
+    int xcd_chunks = 0;
     int edge = 0;         // edge-speculating step (set per storage type, autotuned)
     int n_cu = 256;
     // captured graph
@@ -210,7 +210,13 @@ __global__ void __launch_bounds__(WG)
         const uint32_t i = (t - M.tile_first) * (uint32_t)WG + threadIdx.x;
         step_cell<S, EDGE>(fin + M.pop_base, fout + M.pop_base, flags + M.cell_base, g, M, i);
     };
-    if (xcd_chunks) {
+    if (xcd_chunks == 2) {
+        // one workgroup per tile: each XCD takes one contiguous range of
+        // tiles in order (the dense step's remap), so a tile's z neighbours
+        // are read through the L2 that fetched them
+        const uint32_t t = xcd_remap(blockIdx.x, gridDim.x);
+        if (t < n_tiles) tile(t);
+    } else if (xcd_chunks) {
         constexpr uint32_t C = kChunkCells / (uint32_t)WG;  // tiles per chunk
         const uint32_t xcd = blockIdx.x & 7u, G8 = gridDim.x >> 3;
         for (uint32_t k = blockIdx.x >> 3;; k += G8) {
@@ -431,11 +437,15 @@ unsigned grid_of(const nx_batch_s& b)
     return grid_size(b.n_tiles, b.n_cu, b.wgs_per_cu);
 }
 
-// XCD chunking needs a persistent grid that is a multiple of the 8 XCDs
-bool chunked(const nx_batch_s& b)
+// How tiles reach the XCDs: 0 dealt round robin (grid-stride), 1 persistent
+// grid in XCD chunks (needs a multiple of the 8 XCDs), 2 one workgroup per
+// tile remapped so that each XCD takes a contiguous range of tiles.
+int chunked(const nx_batch_s& b)
 {
     const unsigned G = grid_of(b);
-    return b.xcd_chunks && G % 8 == 0 && G < b.n_tiles;
+    if (!b.xcd_chunks) return 0;
+    if (G >= b.n_tiles) return 2;
+    return G % 8 == 0 ? 1 : 0;
 }
 
 int launch_steps(nx_batch_s& b, int k_steps, int parity0, hipStream_t s)
@@ -452,7 +462,7 @@ int launch_steps(nx_batch_s& b, int k_steps, int parity0, hipStream_t s)
                     constexpr bool E = decltype(edgec)::value;
                     hipLaunchKernelGGL((k_batch_step<S, WG, E>), dim3(grid_of(b)), dim3(WG), 0, s,
                                        (const T*)fin, (T*)fout, b.flags, b.d_members,
-                                       b.d_tile_member, b.active, b.n_tiles, (int)chunked(b));
+                                       b.d_tile_member, b.active, b.n_tiles, chunked(b));
                 };
                 if (b.wg == 512) {
                     if (b.edge) go(std::integral_constant<int, 512>{}, std::true_type{});
@@ -553,10 +563,11 @@ extern "C" int nx_batch_create(uint32_t n, const int32_t* dims, const uint8_t* c
     b->storage = storage;
     // measured on 512 cavities of 32..96 cells (the autotuner's choices):
     // binary16 takes the edge-speculating step with 16 workgroups per CU
-    // dealt tile by tile (45.7k MLUPS; 32.6k with 2 per CU in XCD chunks,
-    // 19.0k without the speculation), FP32 the flag-dependent fetch with 2
-    // per CU (28.5k; 25.9k with the speculation, whose 36 extra registers
-    // cost more than the round trip they save)
+    // dealt tile by tile (45.1-45.7k MLUPS; 32.6k with 2 per CU in XCD
+    // chunks, 19.0k without the speculation), FP32 the flag-dependent fetch
+    // with one workgroup per tile (32.7k; 28.5k with a persistent grid of 2
+    // per CU in XCD chunks; the speculation's 36 extra registers cost more
+    // than the round trip they save)
     if (storage == NX_F16DEV) {
         b->edge = 1;
         b->wgs_per_cu = 16;
@@ -824,7 +835,7 @@ extern "C" int nx_batch_launch_grid(nx_batch b, uint32_t* grid, int32_t* xcd_chu
 {
     if (!b) return set_error(NX_ERR_INVALID, "null batch");
     if (grid) *grid = grid_of(*b);
-    if (xcd_chunked) *xcd_chunked = chunked(*b) ? 1 : 0;
+    if (xcd_chunked) *xcd_chunked = chunked(*b);
     return NX_OK;
 }
 

@@ -29,13 +29,12 @@
 #include <string>
 #include <vector>
 
-#include "nx_common.hpp"
+#include "block_common.hpp"
 
 namespace nx {
 namespace mres {
 namespace {
 
-constexpr int kBV = 512;
 // state byte of a voxel (host and device)
 constexpr uint32_t kActive = 1u, kRefined = 2u;
 constexpr int kClsShift = 2;
@@ -76,7 +75,7 @@ __global__ void __launch_bounds__(kBV)
     const uint64_t own = (uint64_t)b * (Q * kBV) + v;
     if (st & kRefined) {
 #pragma unroll
-        for (int q = 0; q < Q; q++) fout[own + (uint64_t)q * kBV] = T(0);
+        for (int q = 0; q < Q; q++) __builtin_nontemporal_store(T(0), &fout[own + (uint64_t)q * kBV]);
         return;
     }
     T f[Q], o[Q];
@@ -84,7 +83,7 @@ __global__ void __launch_bounds__(kBV)
     for (int q = 0; q < Q; q++) f[q] = fin[own + (uint64_t)q * kBV];
     bgk<T>(f, (T)omega, o);
 #pragma unroll
-    for (int q = 0; q < Q; q++) fout[own + (uint64_t)q * kBV] = o[q];
+    for (int q = 0; q < Q; q++) __builtin_nontemporal_store(o[q], &fout[own + (uint64_t)q * kBV]);
 }
 
 // store.h:98-163 as a gather: entry e adds its fine sources, in order, to
@@ -110,65 +109,95 @@ struct UpLevel {  // the coarser level, seen from level 0 (explosion)
 };
 
 // stream.h:170-225 (stream fused with coalescence and explosion), pull, for
-// the bulk leaf voxels of one level.  LEVEL 0: interface links read the
-// coarse post-collision population of the uncle, the level-1 voxel holding
-// the upstream position (explosion).  LEVEL 1: interface links read the
-// refined upstream voxel's accumulated fine populations times its weight
-// (coalescence).
-template <int S, int LEVEL>
+// the bulk leaf voxels of one level.  The plain pull of all 19 populations
+// is issued first (upstream voxel through the connectivity row in scalar
+// registers); voxels with another link state then fix those links up:
+// bounce-back (own opposite + the wall's opposite), LEVEL 0 interface links
+// read the coarse post-collision population of the uncle, the level-1 voxel
+// holding the upstream position (explosion), LEVEL 1 interface links scale
+// the refined upstream voxel's accumulated fine populations by its weight
+// (coalescence); links in state "none" keep the value in `out`.
+// COLLIDE: the pulled populations are collided at once with omega and the
+// result written to `out` -- stream followed by the next collide of the
+// same level (collide.h:8-149) in one pass; the results are those of the two
+// kernels in sequence, bit for bit.
+template <int S, int LEVEL, bool COLLIDE>
 __global__ void __launch_bounds__(kBV)
-    k_mr_stream(const typename MS<S>::T* __restrict__ fout, typename MS<S>::T* __restrict__ fin,
+    k_mr_stream(const typename MS<S>::T* __restrict__ fout, typename MS<S>::T* __restrict__ out,
                 const uint32_t* __restrict__ state, const uint64_t* __restrict__ links,
                 const uint32_t* __restrict__ conn, const int32_t* __restrict__ origins,
-                const float* __restrict__ ss, UpLevel up)
+                const float* __restrict__ ss, UpLevel up, double omega)
 {
     using T = typename MS<S>::T;
-    __shared__ uint32_t sconn[27];
     const uint32_t b = blockIdx.x, v = threadIdx.x;
-    if (v < 27) sconn[v] = conn[(uint64_t)b * 27 + v];
-    __syncthreads();
+    const Conn cb = load_conn(conn, b);
     const uint32_t st = state[(uint64_t)b * kBV + v];
     if (!bulk_leaf(st)) return;
     const uint64_t lk = links[(uint64_t)b * kBV + v];
     const int x = (int)(v & 7u), y = (int)((v >> 3) & 7u), z = (int)(v >> 6);
     const uint64_t own = (uint64_t)b * (Q * kBV) + v;
+    T f[Q];
+    for_q(AllQ{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (q == 9) {
+            f[9] = fout[own + 9 * kBV];
+        } else {
+            uint32_t t, nv;
+            upstream<q>(b, cb, x, y, z, t, nv);
+            f[q] = fout[t != kInvalidBlock ? ((uint64_t)t * Q + q) * kBV + nv : own + (uint64_t)q * kBV];
+        }
+    });
 #pragma unroll
-    for (int q = 0; q < Q; q++) {
-        const uint64_t kind = (lk >> (2 * q)) & 3u;
-        if (kind == kSkip) continue;
-        if (q == 9) {
-            fin[own + 9 * kBV] = fout[own + 9 * kBV];
-            continue;
-        }
-        if (LEVEL == 0 && kind == kIface) {
-            const int ux = (origins[3 * (uint64_t)b] + x - CX[q]) >> 1;
-            const int uy = (origins[3 * (uint64_t)b + 1] + y - CY[q]) >> 1;
-            const int uz = (origins[3 * (uint64_t)b + 2] + z - CZ[q]) >> 1;
-            const int bx = ux >> 3, by = uy >> 3, bz = uz >> 3;
-            if (bx < 0 || by < 0 || bz < 0 || bx >= up.nb[0] || by >= up.nb[1] || bz >= up.nb[2]) continue;
-            const uint32_t tb = up.table[((uint64_t)bz * up.nb[1] + by) * up.nb[0] + bx];
-            if (tb == kInvalidBlock) continue;
-            const uint32_t slot = (uint32_t)((ux & 7) + 8 * (uy & 7) + 64 * (uz & 7));
-            fin[own + (uint64_t)q * kBV] =
-                ((const T*)up.fout)[((uint64_t)tb * Q + q) * kBV + slot];
-            continue;
-        }
-        // upstream voxel at -c_q through the block connectivity
-        const int nx_ = x - CX[q], ny = y - CY[q], nz = z - CZ[q];
-        const int dx = nx_ < 0 ? -1 : (nx_ > 7 ? 1 : 0), dy = ny < 0 ? -1 : (ny > 7 ? 1 : 0),
-                  dz = nz < 0 ? -1 : (nz > 7 ? 1 : 0);
-        const uint32_t t = sconn[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
-        if (t == kInvalidBlock) continue;
-        const uint32_t nv = (uint32_t)((nx_ & 7) + 8 * (ny & 7) + 64 * (nz & 7));
-        if (kind == kPlain) {
-            fin[own + (uint64_t)q * kBV] = fout[((uint64_t)t * Q + q) * kBV + nv];
-        } else if (kind == kWall) {
-            const int o = OPP[q];
-            fin[own + (uint64_t)q * kBV] = fout[own + (uint64_t)o * kBV] + fout[((uint64_t)t * Q + o) * kBV + nv];
-        } else {  // LEVEL 1 interface: coalescence
-            const uint64_t a = ((uint64_t)t * Q + q) * kBV + nv;
-            fin[own + (uint64_t)q * kBV] = fout[a] * (T)ss[a];
-        }
+    for (int q = 0; q < Q; q++) nx_pin(f[q]);
+    uint32_t keep = 0;  // links whose value stays as it is in `out`
+    if (lk != 0) {
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (q != 9) {
+                const uint64_t kind = (lk >> (2 * q)) & 3u;
+                if (kind == kPlain) return;
+                uint32_t t, nv;
+                upstream<q>(b, cb, x, y, z, t, nv);
+                bool ok = true;
+                if (kind == kWall) {
+                    constexpr int o = OPP[q];
+                    ok = t != kInvalidBlock;
+                    if (ok) f[q] = fout[own + (uint64_t)o * kBV] + fout[((uint64_t)t * Q + o) * kBV + nv];
+                } else if (kind == kIface) {
+                    if constexpr (LEVEL == 0) {
+                        const int ux = (origins[3 * (uint64_t)b] + x - CX[q]) >> 1;
+                        const int uy = (origins[3 * (uint64_t)b + 1] + y - CY[q]) >> 1;
+                        const int uz = (origins[3 * (uint64_t)b + 2] + z - CZ[q]) >> 1;
+                        const int bx = ux >> 3, by = uy >> 3, bz = uz >> 3;
+                        ok = !(bx < 0 || by < 0 || bz < 0 || bx >= up.nb[0] || by >= up.nb[1] || bz >= up.nb[2]);
+                        const uint32_t tb = ok ? up.table[((uint64_t)bz * up.nb[1] + by) * up.nb[0] + bx]
+                                               : kInvalidBlock;
+                        ok = tb != kInvalidBlock;
+                        const uint32_t slot = (uint32_t)((ux & 7) + 8 * (uy & 7) + 64 * (uz & 7));
+                        if (ok) f[q] = ((const T*)up.fout)[((uint64_t)tb * Q + q) * kBV + slot];
+                    } else {
+                        ok = t != kInvalidBlock;
+                        if (ok) f[q] = f[q] * (T)ss[((uint64_t)t * Q + q) * kBV + nv];
+                    }
+                } else {
+                    ok = false;
+                }
+                if (!ok) {
+                    keep |= 1u << q;
+                    f[q] = out[own + (uint64_t)q * kBV];
+                }
+            }
+        });
+    }
+    if constexpr (COLLIDE) {
+        T o[Q];
+        bgk<T>(f, (T)omega, o);
+#pragma unroll
+        for (int q = 0; q < Q; q++) __builtin_nontemporal_store(o[q], &out[own + (uint64_t)q * kBV]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; q++)
+            if (!((keep >> q) & 1u)) __builtin_nontemporal_store(f[q], &out[own + (uint64_t)q * kBV]);
     }
 }
 
@@ -221,6 +250,7 @@ struct nx_mgrid_s {
     uint32_t* d_first = nullptr;
     uint64_t* d_src = nullptr;
     uint64_t n_store_src = 0;
+    bool fuse_fine = false;  // level 0 without "none" links: stream fused with the next collide
     bool ready = false;
 };
 
@@ -376,9 +406,11 @@ bool has_parent(const nx_mgrid_s& g, int x, int y, int z)
     return c.table[((size_t)bz * c.nb[1] + by) * c.nb[0] + bx] != kInvalidBlock;
 }
 
-// Link states of every bulk leaf voxel (stream.h:170-225 decided per link).
-void links_of(const nx_mgrid_s& g, int l, std::vector<uint64_t>& out)
+// Link states of every bulk leaf voxel (stream.h:170-225 decided per link);
+// returns the number of links in state "none".
+uint64_t links_of(const nx_mgrid_s& g, int l, std::vector<uint64_t>& out)
 {
+    uint64_t n_skip = 0;
     const Level& lv = g.L[l];
     const Level& c = g.L[1];
     out.assign((size_t)kBV * lv.n_blocks, 0);
@@ -405,9 +437,11 @@ void links_of(const nx_mgrid_s& g, int l, std::vector<uint64_t>& out)
                         if (c.active(ux, uy, uz)) kind = kIface;  // explosion
                     }
                     lk |= kind << (2 * q);
+                    n_skip += kind == kSkip;
                 }
                 out[lv.slot(x, y, z)] = lk;
             }
+    return n_skip;
 }
 
 }  // namespace
@@ -491,7 +525,8 @@ extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
                 }
         if (int e = upload((void**)&lv.d_state, st.data(), st.size() * 4)) return e;
         std::vector<uint64_t> lk;
-        links_of(*g, l, lk);
+        const uint64_t n_skip = links_of(*g, l, lk);
+        if (l == 0) g->fuse_fine = n_skip == 0;
         if (int e = upload((void**)&lv.d_links, lk.data(), lk.size() * 8)) return e;
     }
     // fine store gather lists (store.h:98-163) and sumStore counts
@@ -555,23 +590,43 @@ int launch_step(nx_mgrid_s& g, double omega0, hipStream_t st)
     const double om0 = 2 * omega0 / (std::scalbln(1.0, 2) + (1. - std::scalbln(1.0, 1)) * omega0);
     const UpLevel none{nullptr, nullptr, {0, 0, 0}};
     const UpLevel up{c.fout, c.d_table, {c.nb[0], c.nb[1], c.nb[2]}};
+    auto store = [&](const void* fout0) {
+        if (g.n_store)
+            hipLaunchKernelGGL((k_mr_store<S>), dim3((g.n_store + 255) / 256), dim3(256), 0, st, (const T*)fout0,
+                               (T*)c.fout, g.d_target, g.d_first, g.d_src, g.n_store);
+    };
     if (c.n_blocks)
         hipLaunchKernelGGL((k_mr_collide<S>), dim3(c.n_blocks), dim3(kBV), 0, st, (const T*)c.fin, (T*)c.fout,
                            c.d_state, om1);
-    for (int sub = 0; sub < 2; sub++) {
-        if (f.n_blocks)
+    if (f.n_blocks && !g.fuse_fine) {
+        for (int sub = 0; sub < 2; sub++) {
             hipLaunchKernelGGL((k_mr_collide<S>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fin,
                                (T*)f.fout, f.d_state, om0);
-        if (g.n_store)
-            hipLaunchKernelGGL((k_mr_store<S>), dim3((g.n_store + 255) / 256), dim3(256), 0, st,
-                               (const T*)f.fout, (T*)c.fout, g.d_target, g.d_first, g.d_src, g.n_store);
-        if (f.n_blocks)
-            hipLaunchKernelGGL((k_mr_stream<S, 0>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fout,
-                               (T*)f.fin, f.d_state, f.d_links, f.d_conn, f.d_origins, (const float*)nullptr, up);
+            store(f.fout);
+            hipLaunchKernelGGL((k_mr_stream<S, 0, false>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fout,
+                               (T*)f.fin, f.d_state, f.d_links, f.d_conn, f.d_origins, (const float*)nullptr, up,
+                               0.0);
+        }
+    } else if (f.n_blocks) {
+        // level 0, two sub-steps: collide (fin -> fout), store, then stream
+        // fused with the second collide (fout -> fin buffer, which now holds
+        // post-collision values), store, stream (-> the old fout buffer);
+        // afterwards the two buffers have traded roles.  Only when no bulk
+        // leaf of level 0 has a link that keeps its previous value (state
+        // "none"): that value is gone once the fused pass has written.
+        hipLaunchKernelGGL((k_mr_collide<S>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fin, (T*)f.fout,
+                           f.d_state, om0);
+        store(f.fout);
+        hipLaunchKernelGGL((k_mr_stream<S, 0, true>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fout,
+                           (T*)f.fin, f.d_state, f.d_links, f.d_conn, f.d_origins, (const float*)nullptr, up, om0);
+        store(f.fin);
+        hipLaunchKernelGGL((k_mr_stream<S, 0, false>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fin,
+                           (T*)f.fout, f.d_state, f.d_links, f.d_conn, f.d_origins, (const float*)nullptr, up, 0.0);
+        std::swap(f.fin, f.fout);
     }
     if (c.n_blocks)
-        hipLaunchKernelGGL((k_mr_stream<S, 1>), dim3(c.n_blocks), dim3(kBV), 0, st, (const T*)c.fout, (T*)c.fin,
-                           c.d_state, c.d_links, c.d_conn, c.d_origins, (const float*)g.d_ss, none);
+        hipLaunchKernelGGL((k_mr_stream<S, 1, false>), dim3(c.n_blocks), dim3(kBV), 0, st, (const T*)c.fout,
+                           (T*)c.fin, c.d_state, c.d_links, c.d_conn, c.d_origins, (const float*)g.d_ss, none, 0.0);
     NX_CHECK_LAUNCH("mgrid step");
     return NX_OK;
 }

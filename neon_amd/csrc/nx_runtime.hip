@@ -74,6 +74,31 @@ __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uin
         dst[i] = src[i];
 }
 
+// The D3Q19 step's memory access pattern with no arithmetic: every cell of an
+// X*Y*Z grid reads population plane q at its upstream cell (i - c_q, the
+// index clamped to the grid) and writes plane q at i, one thread per cell,
+// 256 cells per workgroup, each XCD walking one contiguous range of cells
+// (as the dense step does).  What this copy streams is the ceiling of the
+// pull-streaming step on SoA planes.
+template <typename E>
+__global__ void __launch_bounds__(256)
+    k_shifted_copy(const E* __restrict__ src, E* __restrict__ dst, int32_t X, int32_t Y, uint32_t n)
+{
+    const uint32_t i = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int64_t XY = (int64_t)X * Y;
+    E v[19];
+#pragma unroll
+    for (int q = 0; q < 19; q++) {
+        const int64_t off = (int64_t)CX[q] + (int64_t)X * CY[q] + XY * CZ[q];
+        int64_t j = (int64_t)i - off;
+        j = j < 0 ? 0 : (j >= (int64_t)n ? (int64_t)n - 1 : j);
+        v[q] = src[(uint64_t)q * n + (uint64_t)j];
+    }
+#pragma unroll
+    for (int q = 0; q < 19; q++) __builtin_nontemporal_store(v[q], &dst[(uint64_t)q * n + i]);
+}
+
 int finish_reduce(const double* partials, int n, int nv, double* out, hipStream_t s)
 {
     if (nv < 1 || nv > 4) return set_error(NX_ERR_INVALID, "finish_reduce: nv out of range");
@@ -225,6 +250,29 @@ int nx_d2d(void* dst, const void* src, size_t bytes, nx_stream s)
 {
     if (!bytes) return NX_OK;
     NX_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, (hipStream_t)s));
+    return NX_OK;
+}
+
+int nx_shifted_copy(void* dst, const void* src, const int32_t dim[3], int32_t elem_bytes, nx_stream s)
+{
+    if (!dst || !src || !dim) return set_error(NX_ERR_INVALID, "null pointer");
+    if (dim[0] < 1 || dim[1] < 1 || dim[2] < 1)
+        return set_error(NX_ERR_INVALID, "nx_shifted_copy: dimensions must be >= 1");
+    const uint64_t n = (uint64_t)dim[0] * dim[1] * dim[2];
+    if (n >= (1ull << 32) / 19) return set_error(NX_ERR_INVALID, "nx_shifted_copy: grid too large");
+    if (dst == src) return set_error(NX_ERR_INVALID, "nx_shifted_copy: src and dst must differ");
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    const hipStream_t st = (hipStream_t)s;
+    switch (elem_bytes) {
+        case 2: hipLaunchKernelGGL(k_shifted_copy<uint16_t>, dim3(nb), dim3(256), 0, st, (const uint16_t*)src,
+                                   (uint16_t*)dst, dim[0], dim[1], (uint32_t)n); break;
+        case 4: hipLaunchKernelGGL(k_shifted_copy<uint32_t>, dim3(nb), dim3(256), 0, st, (const uint32_t*)src,
+                                   (uint32_t*)dst, dim[0], dim[1], (uint32_t)n); break;
+        case 8: hipLaunchKernelGGL(k_shifted_copy<uint64_t>, dim3(nb), dim3(256), 0, st, (const uint64_t*)src,
+                                   (uint64_t*)dst, dim[0], dim[1], (uint32_t)n); break;
+        default: return set_error(NX_ERR_INVALID, "nx_shifted_copy: elem_bytes 2, 4 or 8");
+    }
+    NX_CHECK_LAUNCH("k_shifted_copy");
     return NX_OK;
 }
 

@@ -28,7 +28,8 @@ def standalone(nx, m, st, steps):
 
 
 @pytest.mark.parametrize("st", ["f32", "f64", "f16dev", "f32dev"])
-@pytest.mark.parametrize("launch", [(256, 4, 1, 0), (512, 0, 0, 1), (256, 16, 0, 1), (256, 2, 1, 1)])
+@pytest.mark.parametrize("launch", [(256, 4, 1, 0), (512, 0, 0, 1), (256, 16, 0, 1), (256, 2, 1, 1),
+                                    (256, 0, 1, 1), (512, 0, 1, 0)])
 def test_batch_members_equal_standalone(nx, st, launch):
     """Every launch (wg, workgroups per CU, XCD chunks, edge speculation):
     the members' populations equal the standalone single-kernel step's."""
@@ -40,7 +41,7 @@ def test_batch_members_equal_standalone(nx, st, launch):
     assert lc["edge"] == launch[3] and lc["xcd_chunks"] == launch[2]
     assert lc["wgs_per_cu"] == launch[1]
     if launch[1] == 0:                      # one workgroup per tile
-        assert lc["grid"] == b.tiles() and lc["chunked"] == 0
+        assert lc["grid"] == b.tiles() and lc["chunked"] == (2 if launch[2] else 0)
     b.init()
     b.step(14)
     for k, m in enumerate(ms):

@@ -14,10 +14,28 @@ CASES = [((48, 48, 48), 1 / 3, 100.0, 30), ((37, 30, 41), 0.3, 50.0, 20), ((64, 
          ((20, 26, 18), 0.45, 10.0, 40)]
 
 
+def _lambdas(dim, a):
+    if a == -1:  # level 0 where sdfCube > -0.3, level 1 where sdfCube <= -0.5
+        lam0, _ = mg.cavity_lambdas(dim, (0.0, -0.3, -1.0))
+        _, lam1 = mg.cavity_lambdas(dim, (0.0, -0.5, -1.0))
+        return lam0, lam1
+    if a == -2:  # a level-0 slab 8 <= x < 24 and nothing else: its x faces have no source
+        X, Y, Z = dim
+        lam0 = np.zeros((Z, Y, X), np.uint8)
+        lam0[:, :, 8:24] = 1
+        return lam0.reshape(-1), np.zeros(((Z + 1) // 2) * ((Y + 1) // 2) * ((X + 1) // 2), np.uint8)
+    return mg.cavity_lambdas(dim, (0.0, -a, -1.0))
+
+
 @pytest.mark.parametrize("storage", ["f64", "f32"])
-@pytest.mark.parametrize("dim,a,re,steps", CASES)
+@pytest.mark.parametrize("dim,a,re,steps", CASES + [((40, 40, 40), -1, 30.0, 16), ((32, 24, 20), -2, 20.0, 12)])
 def test_mgrid_equals_oracle(nx, orc, storage, dim, a, re, steps):
-    lam0, lam1 = mg.cavity_lambdas(dim, (0.0, -a, -1.0))
+    """Cavities with a level-0 shell of several widths, odd and even sizes
+    (level 0's stream fused with its next collide); a = -1: the levels'
+    lambdas leave a band between them; a = -2: a level-0 slab whose x faces
+    have neither a level-0 nor a level-1 source, so those links keep their
+    values (the unfused level-0 path)."""
+    lam0, lam1 = _lambdas(dim, a)
     ulb, om0 = mg.cavity_parameters(dim, re, storage)
     print(f"mgrid {dim} a={a:.3f} {storage} Re={re} omega0={om0:.6f} steps={steps}", flush=True)
     g = mg.MGrid(dim, lam0, lam1, storage)
@@ -26,7 +44,7 @@ def test_mgrid_equals_oracle(nx, orc, storage, dim, a, re, steps):
     m.setup_cavity(ulb)
     s0, s1 = g.state()
     assert np.array_equal(s0, m.st0) and np.array_equal(s1, m.st1)
-    assert g.info()["store_links"] > 0
+    assert (g.info()["store_links"] > 0) == (a != -2)  # the slab has no level-1 leaves to feed
     g.run(steps, om0)
     m.run(steps, om0)
     for lvl in (0, 1):
