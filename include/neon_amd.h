@@ -423,13 +423,18 @@ int nx_batch_destroy(nx_batch b);
  *   2 omega0 / (4 - omega0) (computeOmega, util.h:40-45).  The fine-to-coarse
  *   sums are added in the fine voxels' z, y, x order (the reference adds
  *   with atomics in no fixed order), so results are deterministic.
+ * Level 0's stream is fused with its next collide (one pass instead of two,
+ * same results) when no bulk leaf of level 0 has a link without a source;
+ * nx_mgrid_info reports it (fine_fused, after nx_mgrid_cavity) with the
+ * block, active and leaf voxel counts per level and the number of fine
+ * sources of the store.
  * nx_mgrid_fields copies fin (which 0) or fout (which 1) of one level into
  *   a host array [19][Z][Y][X] of that level's dimensions; entries of
  *   inactive voxels are not written.  Synchronises the device. */
 int nx_mgrid_create(const int32_t dim[3], const uint8_t* lam0, const uint8_t* lam1, int32_t storage,
                     nx_mgrid* g);
 int nx_mgrid_info(nx_mgrid g, uint32_t n_blocks[2], uint64_t n_active[2], uint64_t n_leaf[2],
-                  uint64_t* n_store);
+                  uint64_t* n_store, int32_t* fine_fused);
 int nx_mgrid_state(nx_mgrid g, uint8_t* st0, uint8_t* st1);
 int nx_mgrid_cavity(nx_mgrid g, double ulb);
 int nx_mgrid_step(nx_mgrid g, double omega0, int32_t n_steps, nx_stream s);

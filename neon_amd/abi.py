@@ -146,7 +146,7 @@ SIGNATURES = {
     "nx_batch_autotune": (_I, [_P, _P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(C.c_float)]),
     "nx_batch_destroy": (_I, [_P]),
     "nx_mgrid_create": (_I, [_P, _P, _P, _I32, C.POINTER(_P)]),
-    "nx_mgrid_info": (_I, [_P, _P, _P, _P, _P]),
+    "nx_mgrid_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "nx_mgrid_state": (_I, [_P, _P, _P]),
     "nx_mgrid_cavity": (_I, [_P, _D]),
     "nx_mgrid_step": (_I, [_P, _D, _I32, _P]),

@@ -468,8 +468,9 @@ extern "C" int nx_mgrid_create(const int32_t dim[3], const uint8_t* lam0, const 
 }
 
 extern "C" int nx_mgrid_info(nx_mgrid g, uint32_t n_blocks[2], uint64_t n_active[2], uint64_t n_leaf[2],
-                             uint64_t* n_store)
+                             uint64_t* n_store, int32_t* fine_fused)
 {
+    if (g && fine_fused) *fine_fused = g->fuse_fine ? 1 : 0;
     if (!g) return set_error(NX_ERR_INVALID, "null mgrid");
     for (int l = 0; l < 2; l++) {
         uint64_t a = 0, lf = 0;

@@ -93,9 +93,10 @@ class MGrid:
     def info(self) -> dict:
         nb = (C.c_uint32 * 2)()
         na, nl = (C.c_uint64 * 2)(), (C.c_uint64 * 2)()
-        ns = C.c_uint64()
-        check(lib.nx_mgrid_info(self.h, nb, na, nl, C.byref(ns)))
-        return {"blocks": list(nb), "active": list(na), "leaf": list(nl), "store_links": int(ns.value)}
+        ns, fused = C.c_uint64(), C.c_int32()
+        check(lib.nx_mgrid_info(self.h, nb, na, nl, C.byref(ns), C.byref(fused)))
+        return {"blocks": list(nb), "active": list(na), "leaf": list(nl), "store_links": int(ns.value),
+                "fine_fused": bool(fused.value)}
 
     def setup_cavity(self, ulb: float) -> None:
         self._init()
