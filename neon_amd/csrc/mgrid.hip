@@ -120,8 +120,9 @@ struct UpLevel {  // the coarser level, seen from level 0 (explosion)
 // COLLIDE: the pulled populations are collided at once with omega and the
 // result written to `out` -- stream followed by the next collide of the
 // same level (collide.h:8-149) in one pass; the results are those of the two
-// kernels in sequence, bit for bit.
-template <int S, int LEVEL, bool COLLIDE>
+// kernels in sequence, bit for bit.  KEEP = false: the level has no link in
+// state "none" (checked on the host), every population is stored.
+template <int S, int LEVEL, bool COLLIDE, bool KEEP>
 __global__ void __launch_bounds__(kBV)
     k_mr_stream(const typename MS<S>::T* __restrict__ fout, typename MS<S>::T* __restrict__ out,
                 const uint32_t* __restrict__ state, const uint64_t* __restrict__ links,
@@ -194,10 +195,13 @@ __global__ void __launch_bounds__(kBV)
         bgk<T>(f, (T)omega, o);
 #pragma unroll
         for (int q = 0; q < Q; q++) __builtin_nontemporal_store(o[q], &out[own + (uint64_t)q * kBV]);
-    } else {
+    } else if constexpr (KEEP) {
 #pragma unroll
         for (int q = 0; q < Q; q++)
             if (!((keep >> q) & 1u)) __builtin_nontemporal_store(f[q], &out[own + (uint64_t)q * kBV]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &out[own + (uint64_t)q * kBV]);
     }
 }
 
@@ -250,7 +254,8 @@ struct nx_mgrid_s {
     uint32_t* d_first = nullptr;
     uint64_t* d_src = nullptr;
     uint64_t n_store_src = 0;
-    bool fuse_fine = false;  // level 0 without "none" links: stream fused with the next collide
+    bool fuse_fine = false;   // level 0 without "none" links: stream fused with the next collide
+    bool keep_coarse = true;  // level 1 has "none" links
     bool ready = false;
 };
 
@@ -528,6 +533,7 @@ extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
         std::vector<uint64_t> lk;
         const uint64_t n_skip = links_of(*g, l, lk);
         if (l == 0) g->fuse_fine = n_skip == 0;
+        if (l == 1) g->keep_coarse = n_skip != 0;
         if (int e = upload((void**)&lv.d_links, lk.data(), lk.size() * 8)) return e;
     }
     // fine store gather lists (store.h:98-163) and sumStore counts
@@ -604,9 +610,9 @@ int launch_step(nx_mgrid_s& g, double omega0, hipStream_t st)
             hipLaunchKernelGGL((k_mr_collide<S>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fin,
                                (T*)f.fout, f.d_state, om0);
             store(f.fout);
-            hipLaunchKernelGGL((k_mr_stream<S, 0, false>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fout,
-                               (T*)f.fin, f.d_state, f.d_links, f.d_conn, f.d_origins, (const float*)nullptr, up,
-                               0.0);
+            hipLaunchKernelGGL((k_mr_stream<S, 0, false, true>), dim3(f.n_blocks), dim3(kBV), 0, st,
+                               (const T*)f.fout, (T*)f.fin, f.d_state, f.d_links, f.d_conn, f.d_origins,
+                               (const float*)nullptr, up, 0.0);
         }
     } else if (f.n_blocks) {
         // level 0, two sub-steps: collide (fin -> fout), store, then stream
@@ -618,15 +624,18 @@ int launch_step(nx_mgrid_s& g, double omega0, hipStream_t st)
         hipLaunchKernelGGL((k_mr_collide<S>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fin, (T*)f.fout,
                            f.d_state, om0);
         store(f.fout);
-        hipLaunchKernelGGL((k_mr_stream<S, 0, true>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fout,
+        hipLaunchKernelGGL((k_mr_stream<S, 0, true, false>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fout,
                            (T*)f.fin, f.d_state, f.d_links, f.d_conn, f.d_origins, (const float*)nullptr, up, om0);
         store(f.fin);
-        hipLaunchKernelGGL((k_mr_stream<S, 0, false>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fin,
+        hipLaunchKernelGGL((k_mr_stream<S, 0, false, false>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fin,
                            (T*)f.fout, f.d_state, f.d_links, f.d_conn, f.d_origins, (const float*)nullptr, up, 0.0);
         std::swap(f.fin, f.fout);
     }
-    if (c.n_blocks)
-        hipLaunchKernelGGL((k_mr_stream<S, 1, false>), dim3(c.n_blocks), dim3(kBV), 0, st, (const T*)c.fout,
+    if (c.n_blocks && g.keep_coarse)
+        hipLaunchKernelGGL((k_mr_stream<S, 1, false, true>), dim3(c.n_blocks), dim3(kBV), 0, st, (const T*)c.fout,
+                           (T*)c.fin, c.d_state, c.d_links, c.d_conn, c.d_origins, (const float*)g.d_ss, none, 0.0);
+    else if (c.n_blocks)
+        hipLaunchKernelGGL((k_mr_stream<S, 1, false, false>), dim3(c.n_blocks), dim3(kBV), 0, st, (const T*)c.fout,
                            (T*)c.fin, c.d_state, c.d_links, c.d_conn, c.d_origins, (const float*)g.d_ss, none, 0.0);
     NX_CHECK_LAUNCH("mgrid step");
     return NX_OK;
