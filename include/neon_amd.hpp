@@ -10,6 +10,9 @@
 //   Neon::skeleton::Skeleton       libNeonSkeleton/include/Neon/skeleton/Skeleton.h:56-65
 //   LbmContainers, LbmIterationD3Q19
 //                                  benchmarks/lbm-lid-driven-cavity-flow/src/{LbmTools.h,LbmIteration.h}
+// Compiled with hipcc, the header also offers Neon's generic Container API on
+// the dense grid (grid.newContainer, Loader::load, partition getNghData) for
+// a caller's own stencils (end of this file).
 // Header-only, C++17; link with libneon_amd.so.  One process drives one GPU:
 // a Backend takes exactly one device id (the reference's multi-device
 // partitioning collapses to one partition, so there is no halo exchange and
@@ -44,12 +47,19 @@ inline void check(int code, const std::string& where)
     if (code != NX_OK) throw NeonException(where + ": " + nx_last_error());
 }
 
+// index_3d is usable in device lambdas of hipcc-compiled callers
+#if defined(__HIPCC__)
+#define NX_HOST_DEVICE __host__ __device__
+#else
+#define NX_HOST_DEVICE
+#endif
+
 struct index_3d
 {
     int32_t x = 0, y = 0, z = 0;
-    index_3d() = default;
-    index_3d(int32_t x_, int32_t y_, int32_t z_) : x(x_), y(y_), z(z_) {}
-    size_t rMulTyped() const { return size_t(x) * size_t(y) * size_t(z); }
+    NX_HOST_DEVICE index_3d() = default;
+    NX_HOST_DEVICE index_3d(int32_t x_, int32_t y_, int32_t z_) : x(x_), y(y_), z(z_) {}
+    NX_HOST_DEVICE size_t rMulTyped() const { return size_t(x) * size_t(y) * size_t(z); }
 };
 
 enum class Runtime { stream };
@@ -232,6 +242,7 @@ template <> struct ComputeOf<FastFloat> { static constexpr int value = NX_COMPUT
 
 template <typename Grid, typename T, int C>
 class Field;
+class Container;
 
 // ---- dGrid -------------------------------------------------------------------
 
@@ -280,6 +291,16 @@ class dGrid
             for (int y = 0; y < d.y; y++)
                 for (int x = 0; x < d.x; x++) f(index_3d(x, y, z), cell++);
     }
+#if defined(__HIPCC__)
+    // Neon's grid.newContainer(name, loadingLambda) (dGrid_imp.h:218-251):
+    // the loading lambda receives a Loader, loads the fields it touches and
+    // returns the compute lambda, `[=] __device__ (const dIndex& idx) mutable
+    // {...}` (NEON_CUDA_HOST_DEVICE in the reference), which the container
+    // launches once per cell of the box on run().  Compiled only when the
+    // caller's translation unit is compiled with hipcc (defined below).
+    template <typename LoadingLambda>
+    Container newContainer(const std::string& name, LoadingLambda&& loadingLambda) const;
+#endif
     nx_dense_desc desc(int storage, int compute) const
     {
         nx_dense_desc d{};
@@ -447,6 +468,12 @@ class Field
         updateDeviceData(Backend::mainStreamIdx);
         g.getBackend().sync(Backend::mainStreamIdx);
     }
+#if defined(__HIPCC__)
+    // The device view a container's compute lambda captures (Neon's
+    // field.getPartition(execution, setIdx, dataView), dField_imp.h:67-87):
+    // raw pointer, dimensions, cardinality; valid while the field lives.
+    auto getPartition() const;
+#endif
     bool           valid() const { return bool(s_); }
     const GridT&   getGrid() const { return s_->grid; }
     const Backend& getBackend() const { return s_->grid.getBackend(); }
@@ -734,3 +761,149 @@ struct LbmIterationD3Q19
 
 }  // namespace lbm
 }  // namespace neon_amd
+
+#if defined(__HIPCC__)
+// ---- user containers on the dense grid (hipcc translation units only) -------
+// The reference's Container API for a user's own stencil: Loader::load
+// (Loader_imp.h:102-188) returns the field's partition; the partition offers
+// operator()(idx, card), getNghData<dx, dy, dz>(idx, card[, alternative])
+// and getNghData(idx, offset, card) returning NghData{mData, mIsValid}
+// (dPartition.h:174-190, 265-299; NghData.h), getGlobalIndex and
+// cardinality.  A neighbour outside the box is invalid unless the grid is
+// periodic along that axis (then it wraps).
+#include <hip/hip_runtime.h>
+
+namespace neon_amd {
+
+// A cell of the dense grid (dIndex.h): its coordinates in the box.
+struct dIndex
+{
+    int32_t x, y, z;
+};
+
+struct int8_3d
+{
+    int8_t x = 0, y = 0, z = 0;
+    __host__ __device__ int8_3d() = default;
+    __host__ __device__ int8_3d(int8_t x_, int8_t y_, int8_t z_) : x(x_), y(y_), z(z_) {}
+};
+
+template <typename T>
+struct NghData
+{
+    T    mData;
+    bool mIsValid;
+};
+
+// Neon::Pattern: how a container reads a field (Loader_imp.h:102-188)
+enum class Pattern { MAP, STENCIL };
+
+template <typename T>
+struct dPartition
+{
+    T*       mem = nullptr;
+    int32_t  X = 0, Y = 0, Z = 0;
+    int32_t  card = 1;
+    int32_t  periodic = 0;
+    uint64_t n = 0;  // cells
+
+    __host__ __device__ uint64_t cell(const dIndex& i) const
+    {
+        return uint64_t(i.x) + uint64_t(X) * (uint64_t(i.y) + uint64_t(Y) * uint64_t(i.z));
+    }
+    __device__ T& operator()(const dIndex& i, int c) const { return mem[uint64_t(c) * n + cell(i)]; }
+    __device__ int cardinality() const { return card; }
+    __device__ index_3d getGlobalIndex(const dIndex& i) const { return index_3d(i.x, i.y, i.z); }
+    __device__ bool nghIdx(const dIndex& i, int dx, int dy, int dz, dIndex& out) const
+    {
+        int x = i.x + dx, y = i.y + dy, z = i.z + dz;
+        if (periodic & NX_PERIODIC_X) x = (x + X) % X;
+        if (periodic & NX_PERIODIC_Y) y = (y + Y) % Y;
+        if (periodic & NX_PERIODIC_Z) z = (z + Z) % Z;
+        out = dIndex{x, y, z};
+        return x >= 0 && y >= 0 && z >= 0 && x < X && y < Y && z < Z;
+    }
+    __device__ NghData<T> getNghData(const dIndex& i, int8_3d off, int c, T alternative = T()) const
+    {
+        dIndex j;
+        if (!nghIdx(i, off.x, off.y, off.z, j)) return NghData<T>{alternative, false};
+        return NghData<T>{(*this)(j, c), true};
+    }
+    template <int dx, int dy, int dz>
+    __device__ NghData<T> getNghData(const dIndex& i, int c, T alternative = T()) const
+    {
+        return getNghData(i, int8_3d(dx, dy, dz), c, alternative);
+    }
+};
+
+// Packed cell type word (CellType.h) as the partition of a CellType field
+// holds it on the device.
+__device__ inline int cellClassification(uint32_t w) { return int((w >> NX_CLASS_SHIFT) & 3u); }
+__device__ inline uint32_t cellWallNghBitflag(uint32_t w) { return w & ~(3u << NX_CLASS_SHIFT); }
+
+class Loader
+{
+   public:
+    // A STENCIL load of a field the container also writes is refused, as the
+    // reference refuses a stencil load of a non-const field
+    // (Loader_imp.h:114-122): neighbours would be read while written.
+    template <typename FieldT>
+    auto load(FieldT& field, Pattern pattern = Pattern::MAP,
+              StencilSemantic = StencilSemantic::standard) const
+    {
+        if (pattern == Pattern::STENCIL && !std::is_const_v<FieldT>)
+            throw NeonException("Loader::load: STENCIL access needs a const field");
+        return field.getPartition();
+    }
+};
+
+template <typename GridT, typename T, int C>
+auto Field<GridT, T, C>::getPartition() const
+{
+    static_assert(std::is_same_v<GridT, dGrid>, "getPartition: dense grid fields only");
+    dPartition<D> p;
+    const index_3d d = s_->grid.getDimension();
+    p.mem = static_cast<D*>(s_->dev);
+    p.X = d.x;
+    p.Y = d.y;
+    p.Z = d.z;
+    p.card = s_->card;
+    p.periodic = s_->grid.periodic();
+    p.n = d.rMulTyped();
+    return p;
+}
+
+namespace detail {
+// One thread per cell of the box, 256 cells per workgroup along x-fastest
+// cell order: each wavefront's accesses to a SoA plane are contiguous.
+template <typename F>
+__global__ void __launch_bounds__(256) dgrid_launch(F f, int32_t X, int32_t Y, uint64_t n)
+{
+    const uint64_t c = uint64_t(blockIdx.x) * 256u + threadIdx.x;
+    if (c >= n) return;
+    const dIndex i{int32_t(c % uint64_t(X)), int32_t((c / uint64_t(X)) % uint64_t(Y)),
+                   int32_t(c / (uint64_t(X) * uint64_t(Y)))};
+    f(i);
+}
+}  // namespace detail
+
+// The loading lambda runs once, when the container is made (the reference
+// runs it again at every launch to collect the partitions; the partitions
+// here are fixed for the fields' lifetime).
+template <typename LoadingLambda>
+Container dGrid::newContainer(const std::string& name, LoadingLambda&& loadingLambda) const
+{
+    Loader     loader;
+    auto       compute = loadingLambda(loader);
+    const auto d = getDimension();
+    return Container(name, getBackend(), [compute, d, name](nx_stream s) {
+        const uint64_t n = d.rMulTyped();
+        hipLaunchKernelGGL(detail::dgrid_launch<decltype(compute)>, dim3(unsigned((n + 255) / 256)), dim3(256), 0,
+                           reinterpret_cast<hipStream_t>(s), compute, d.x, d.y, n);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) throw NeonException(name + ": " + hipGetErrorString(e));
+    });
+}
+
+}  // namespace neon_amd
+#endif  // __HIPCC__

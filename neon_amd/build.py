@@ -3,9 +3,10 @@
     python neon_amd/build.py            # compile every source, link the library
 
 Each translation unit is compiled separately (in parallel) with hipcc and the
-objects are linked into neon_amd/lib/libneon_amd.so; the C++ programs under
-neon_amd/apps/ (host code on include/neon_amd.hpp) are then built with g++
-into neon_amd/lib/.  Flags:
+objects are linked into neon_amd/lib/libneon_amd.so; the programs under
+neon_amd/apps/ (on include/neon_amd.hpp) are then built into neon_amd/lib/:
+*.cpp (host code) with g++, *.hip (a caller's own device containers) with
+hipcc and the library's device flags.  Flags:
   * --offload-arch=gfx950       MI355X only
   * -ffp-contract=off           no FMA contraction: binary64 kernels match a CPU
                                 evaluation of the same expression tree bit for bit
@@ -71,10 +72,16 @@ def build(verbose: bool = True) -> pathlib.Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
-    for app in sorted(APPS.glob("*.cpp")):
+    for app in sorted(list(APPS.glob("*.cpp")) + list(APPS.glob("*.hip"))):
         exe = LIB.parent / app.stem
-        cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{REPO / 'include'}", str(app), "-o", str(exe) + ".tmp",
-               f"-L{LIB.parent}", "-lneon_amd", "-Wl,-rpath,$ORIGIN"]
+        if app.suffix == ".hip":
+            # a caller's own containers: device code from include/neon_amd.hpp
+            cmd = [HIPCC, "-x", "hip", *DEVICE, "-O3", "-std=c++17", "-ffp-contract=off", "-Wall",
+                   f"-I{REPO / 'include'}", str(app), "-o", str(exe) + ".tmp",
+                   f"-L{LIB.parent}", "-lneon_amd", "-Wl,-rpath,$ORIGIN"]
+        else:
+            cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{REPO / 'include'}", str(app), "-o", str(exe) + ".tmp",
+                   f"-L{LIB.parent}", "-lneon_amd", "-Wl,-rpath,$ORIGIN"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"app build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
