@@ -562,15 +562,16 @@ extern "C" int nx_batch_create(uint32_t n, const int32_t* dims, const uint8_t* c
     auto* b = new nx_batch_s;
     b->storage = storage;
     // measured on 512 cavities of 32..96 cells (the autotuner's choices):
-    // binary16 takes the edge-speculating step with 16 workgroups per CU
-    // dealt tile by tile (45.1-45.7k MLUPS; 32.6k with 2 per CU in XCD
-    // chunks, 19.0k without the speculation), FP32 the flag-dependent fetch
+    // binary16 takes the edge-speculating step with 32 workgroups per CU
+    // dealt tile by tile (46.1k MLUPS; 45.2k with 16 per CU, 44.3k with one
+    // per tile, 32.6k with 2 per CU in XCD chunks, 19.0k without the
+    // speculation), FP32 the flag-dependent fetch
     // with one workgroup per tile (32.7k; 28.5k with a persistent grid of 2
     // per CU in XCD chunks; the speculation's 36 extra registers cost more
     // than the round trip they save)
     if (storage == NX_F16DEV) {
         b->edge = 1;
-        b->wgs_per_cu = 16;
+        b->wgs_per_cu = 32;
         b->xcd_chunks = 0;
     }
     b->n_members = n;
@@ -863,7 +864,7 @@ extern "C" int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int3
     for (int ed : {1, 0})
     for (int xc : {1, 0})
     for (int wg : {256, 512})
-        for (int c : {2, 4, 8, 16, 0}) {
+        for (int c : {2, 4, 8, 16, 32, 0}) {
             if (int e = nx_batch_set_launch(b, wg, c)) return e;
             b->xcd_chunks = xc;
             b->edge = ed;
