@@ -430,7 +430,9 @@ int nx_batch_destroy(nx_batch b);
  * sources of the store.
  * nx_mgrid_fields copies fin (which 0) or fout (which 1) of one level into
  *   a host array [19][Z][Y][X] of that level's dimensions; entries of
- *   inactive voxels are not written.  Synchronises the device. */
+ *   inactive voxels are not written.  Synchronises the device.
+ * A handle is not for concurrent use from several host threads (a step
+ * swaps level 0's two buffers when it is queued). */
 int nx_mgrid_create(const int32_t dim[3], const uint8_t* lam0, const uint8_t* lam1, int32_t storage,
                     nx_mgrid* g);
 int nx_mgrid_info(nx_mgrid g, uint32_t n_blocks[2], uint64_t n_active[2], uint64_t n_leaf[2],

@@ -28,13 +28,16 @@ def _lambdas(dim, a):
 
 
 @pytest.mark.parametrize("storage", ["f64", "f32"])
-@pytest.mark.parametrize("dim,a,re,steps", CASES + [((40, 40, 40), -1, 30.0, 16), ((32, 24, 20), -2, 20.0, 12)])
+@pytest.mark.parametrize("dim,a,re,steps", CASES + [((40, 40, 40), -1, 30.0, 16), ((32, 24, 20), -2, 20.0, 12),
+                                                     ((24, 24, 24), 5.0, 20.0, 10), ((16, 18, 16), 0.0, 20.0, 6)])
 def test_mgrid_equals_oracle(nx, orc, storage, dim, a, re, steps):
     """Cavities with a level-0 shell of several widths, odd and even sizes
     (level 0's stream fused with its next collide); a = -1: the levels'
     lambdas leave a band between them; a = -2: a level-0 slab whose x faces
     have neither a level-0 nor a level-1 source, so those links keep their
-    values (the unfused level-0 path)."""
+    values (the unfused level-0 path); a = 5: every voxel on level 0
+    (level 1 empty); a = 0: level 0 empty (no walls; level-1 links at the
+    domain boundary keep their values)."""
     lam0, lam1 = _lambdas(dim, a)
     ulb, om0 = mg.cavity_parameters(dim, re, storage)
     print(f"mgrid {dim} a={a:.3f} {storage} Re={re} omega0={om0:.6f} steps={steps}", flush=True)
@@ -44,7 +47,7 @@ def test_mgrid_equals_oracle(nx, orc, storage, dim, a, re, steps):
     m.setup_cavity(ulb)
     s0, s1 = g.state()
     assert np.array_equal(s0, m.st0) and np.array_equal(s1, m.st1)
-    assert (g.info()["store_links"] > 0) == (a != -2)  # the slab has no level-1 leaves to feed
+    assert (g.info()["store_links"] > 0) == (a not in (-2, 0.0, 5.0))  # nothing to feed without both levels
     g.run(steps, om0)
     m.run(steps, om0)
     for lvl in (0, 1):
