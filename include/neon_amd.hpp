@@ -263,6 +263,10 @@ class dGrid
     using Field = neon_amd::Field<dGrid, T, C>;
 
     dGrid() = default;
+    // The active-cell lambda is accepted and not used: the reference's dGrid
+    // ignores it as well (every cell of the box is allocated and active,
+    // dGrid_imp.h:7-10).  The stencil only sizes halos, which one partition
+    // does not have.
     dGrid(const Backend& bk, index_3d dim, const std::function<bool(const index_3d&)>& = {},
           const void* /*stencil*/ = nullptr, int periodic = 0)
     {
@@ -566,7 +570,10 @@ struct SkeletonOptions
 };
 
 // A sequence of containers run in order on the main stream (with one
-// partition there is no halo update to insert between them).
+// partition there is no halo update to insert between them).  The
+// SkeletonOptions (OCC, transfer mode) are accepted and have no effect: they
+// choose how halo transfers overlap computation across partitions
+// (Skeleton.h:32-65, skeleton/Options.h), and one partition has none.
 class Skeleton
 {
     Backend                bk_;
