@@ -94,3 +94,43 @@ def test_mgrid_256_properties(nx, orc):
     sym = b & b[::-1]
     assert np.abs(ux - ux[::-1])[sym].max() < 1e-9 * np.abs(ux[b]).max()
     assert np.abs(uz + uz[::-1])[sym].max() < 1e-9 * np.abs(ux[b]).max()
+
+
+def test_mgrid_160_centerline_close_to_reference_uniform(nx):
+    """Physics of the two-level path against the reference's own output: the
+    160^3 cavity at Re 100 (tests/golden/NeonUniformLBM_20000_Y.dat, the
+    uniform dGrid run of RunCavityTwoPop.cu after 20000 steps) run as two
+    levels (level 0 within a quarter of the half width of the faces) for
+    10000 coarse steps with the level-0 relaxation of the uniform run; the
+    centre line u_x(y) at x = z = 80, read from the leaf that covers each
+    level-0 position, stays within 2% of the lid speed of the reference's
+    (measured: largest 1.43% at y = 138, near the interface under the lid;
+    mean 0.22%)."""
+    import pathlib
+    N, RE, ULB, STEPS = 160, 100.0, 0.04, 10000
+    dat = np.loadtxt(pathlib.Path(__file__).resolve().parent / "golden" / "NeonUniformLBM_20000_Y.dat")
+    omega_f = 1.0 / (3.0 * ULB * (N - 2) / RE + 0.5)
+    omega0 = 4.0 * omega_f / (2.0 + omega_f)          # level 0 then relaxes with omega_f
+    dim = (N, N, N)
+    lam0, lam1 = mg.cavity_lambdas(dim, (0.0, -0.25, -1.0))
+    g = mg.MGrid(dim, lam0, lam1, "f64")
+    g.setup_cavity(ULB)
+    g.run(STEPS, omega0)
+    s0, s1 = g.state()
+    f0 = g.fields(0).reshape(19, N, N, N)
+    f1 = g.fields(1).reshape(19, N // 2, N // 2, N // 2)
+    c = np.array([[-1, 0, 0], [0, -1, 0], [0, 0, -1], [-1, -1, 0], [-1, 1, 0], [-1, 0, -1], [-1, 0, 1],
+                  [0, -1, -1], [0, -1, 1], [0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [1, -1, 0],
+                  [1, 0, 1], [1, 0, -1], [0, 1, 1], [0, 1, -1]], float)
+    a0 = (s0.reshape(N, N, N) & 1) == 1
+    ux = np.zeros(N)
+    for y in range(N):
+        if a0[N // 2, y, N // 2]:
+            f = f0[:, N // 2, y, N // 2]
+        else:
+            f = f1[:, N // 4, y // 2, N // 4]
+        ux[y] = (c[:, 0] @ f) / f.sum() if 0 < y < N - 1 else dat[y, 1]
+    dev = np.abs(ux - dat[:, 1]) / ULB
+    print(f"two-level 160^3 centre line vs the reference's uniform run: max |du|/U {dev.max():.4f} "
+          f"at y = {int(dev.argmax())}, mean {dev.mean():.4f}")
+    assert dev.max() < 0.02
