@@ -24,6 +24,7 @@
 #include "neon_amd.h"
 
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <fstream>
 #include <functional>
@@ -147,6 +148,147 @@ inline void writeFile(const std::string& fileName, const std::string& fieldName,
 }
 
 }  // namespace vtk
+
+// XML ImageData (.vti) export, Neon::ioToVTI
+// (libNeonCore/include/Neon/core/tools/io/ioToVti.h:33-44, :663-719).  Each
+// input is a callback (index, component) -> value evaluated over the node
+// box (isNode) or the voxel box, written as one Float64 DataArray: the
+// PointData section first, then CellData (:468-539).  ASCII arrays print
+// every value in fixed notation with 17 decimals followed by a space, a tab
+// after each voxel's components, a newline after each x row and after each
+// z plane (:431-462).  BINARY arrays hold the values as little-endian
+// binary64 behind a UInt32 byte count, base64-encoded as one stream (the
+// VTK XML inline-binary format); the reference's binary branch converts the
+// values to its index type and sizes the buffer in elements (:374-415), so
+// it is not reproduced.
+struct double_3d
+{
+    double x = 0, y = 0, z = 0;
+    double_3d() = default;
+    double_3d(double x_, double y_, double z_) : x(x_), y(y_), z(z_) {}
+};
+
+template <class real_tt = double>
+struct VtiInputData_t
+{
+    std::function<real_tt(const index_3d&, int componentIdx)> func;
+    int                                                      nComponents = 1;
+    std::string                                              fieldName;
+    bool                                                     isNode = false;
+    IoFileType                                               asciiOrBinary = IoFileType::ASCII;
+};
+
+namespace vti {
+
+inline std::string base64(const std::vector<unsigned char>& in)
+{
+    static const char tab[] = "ABCDEFGHIJKLMNOPQRSTUVWXYZabcdefghijklmnopqrstuvwxyz0123456789+/";
+    std::string out;
+    out.reserve((in.size() + 2) / 3 * 4);
+    size_t i = 0;
+    for (; i + 3 <= in.size(); i += 3) {
+        const uint32_t w = (uint32_t(in[i]) << 16) | (uint32_t(in[i + 1]) << 8) | uint32_t(in[i + 2]);
+        for (int s = 18; s >= 0; s -= 6) out += tab[(w >> s) & 63u];
+    }
+    const size_t rest = in.size() - i;
+    if (rest) {
+        uint32_t w = uint32_t(in[i]) << 16;
+        if (rest == 2) w |= uint32_t(in[i + 1]) << 8;
+        out += tab[(w >> 18) & 63u];
+        out += tab[(w >> 12) & 63u];
+        out += rest == 2 ? tab[(w >> 6) & 63u] : '=';
+        out += '=';
+    }
+    return out;
+}
+
+inline std::string extent(const index_3d& box)
+{
+    return "0 " + std::to_string(box.x) + " 0 " + std::to_string(box.y) + " 0 " + std::to_string(box.z);
+}
+
+template <class real_tt>
+inline void dataArray(std::ostream& out, const VtiInputData_t<real_tt>& in, const index_3d& box)
+{
+    out << "<DataArray type=\"Float64\" NumberOfComponents=\"" << in.nComponents << "\" Name=\"" << in.fieldName
+        << "\" format=\"" << (in.asciiOrBinary == IoFileType::ASCII ? "ascii" : "binary") << "\">\n";
+    if (in.asciiOrBinary == IoFileType::ASCII) {
+        char buf[64];
+        for (int z = 0; z < box.z; z++) {
+            for (int y = 0; y < box.y; y++) {
+                for (int x = 0; x < box.x; x++) {
+                    for (int c = 0; c < in.nComponents; c++) {
+                        const real_tt v = in.func(index_3d(x, y, z), c);
+                        if constexpr (std::is_integral_v<real_tt>) out << static_cast<long long>(v) << " ";
+                        else {
+                            std::snprintf(buf, sizeof(buf), "%.17f ", static_cast<double>(v));
+                            out << buf;
+                        }
+                    }
+                    out << "\t";
+                }
+                out << "\n";
+            }
+            out << "\n";
+        }
+        out << "</DataArray>\n\n";
+        return;
+    }
+    const size_t               n = box.rMulTyped() * size_t(in.nComponents);
+    const size_t               bytes = n * sizeof(double);
+    std::vector<unsigned char> raw(4 + bytes);
+    const uint32_t             len = uint32_t(bytes);
+    for (int k = 0; k < 4; k++) raw[size_t(k)] = static_cast<unsigned char>(len >> (8 * k));
+    size_t at = 4;
+    for (int z = 0; z < box.z; z++)
+        for (int y = 0; y < box.y; y++)
+            for (int x = 0; x < box.x; x++)
+                for (int c = 0; c < in.nComponents; c++) {
+                    const double v = static_cast<double>(in.func(index_3d(x, y, z), c));
+                    uint64_t     u;
+                    std::memcpy(&u, &v, sizeof(u));
+                    for (int k = 0; k < 8; k++) raw[at++] = static_cast<unsigned char>(u >> (8 * k));
+                }
+    out << base64(raw) << "\n</DataArray>\n\n";
+}
+
+}  // namespace vti
+
+// Writes `filename` (taken as given, no suffix added) for the inputs over a
+// node box and a voxel box that must differ by one in every direction.
+template <class real_tt = double>
+inline void ioToVTI(const std::vector<VtiInputData_t<real_tt>>& gridsInfo, const std::string& filename,
+                    const index_3d& nodeSpace, const index_3d& voxSpace,
+                    const double_3d& spacingData = double_3d(1, 1, 1), const double_3d& origin = double_3d(0, 0, 0))
+{
+    if (voxSpace.x != nodeSpace.x - 1 || voxSpace.y != nodeSpace.y - 1 || voxSpace.z != nodeSpace.z - 1)
+        throw NeonException("ioToVTI: Inconsistent data. Node space dimension should be bigger than voxel of one unit.");
+    if (voxSpace.x < 0 || voxSpace.y < 0 || voxSpace.z < 0) throw NeonException("ioToVTI: negative voxel box");
+    for (const auto& g : gridsInfo) {
+        const index_3d box = g.isNode ? nodeSpace : voxSpace;
+        if (g.asciiOrBinary == IoFileType::BINARY && box.rMulTyped() * size_t(g.nComponents) * 8 > 0xffffffffull)
+            throw NeonException("ioToVTI: binary array " + g.fieldName + " exceeds the 4 GiB UInt32 header");
+    }
+    std::ofstream out(filename, std::ios::out | std::ios::binary);
+    if (!out.is_open()) throw NeonException("ioToVTI: file " + filename + " could not be opened");
+    const auto d = [](double v) { return std::to_string(v); };
+    out << "<?xml version=\"1.0\"?>\n";
+    out << "<VTKFile type=\"ImageData\" version=\"0.1\" byte_order=\"LittleEndian\">\n";
+    out << "<ImageData WholeExtent=\"" << vti::extent(voxSpace) << " \" Origin=\"" << d(origin.x) << " "
+        << d(origin.y) << " " << d(origin.z) << "\" Spacing=\"" << d(spacingData.x) << " " << d(spacingData.y)
+        << " " << d(spacingData.z) << " \">\n";
+    out << "<Piece Extent=\"" << vti::extent(voxSpace) << "\" >\n";
+    for (const bool nodes : {true, false}) {
+        out << (nodes ? "<PointData>\n" : "<CellData>\n");
+        for (const auto& g : gridsInfo)
+            if (g.isNode == nodes) vti::dataArray(out, g, nodes ? nodeSpace : voxSpace);
+        out << (nodes ? "</PointData>\n" : "</CellData>\n");
+    }
+    out << "</Piece>\n";
+    out << " </ImageData>\n";
+    out << " </VTKFile>\n";
+    if (!out.good()) throw NeonException("ioToVTI: writing " + filename + " failed");
+}
 
 // ---- Backend ---------------------------------------------------------------
 

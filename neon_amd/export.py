@@ -9,11 +9,22 @@
   each value and a newline per x row, BINARY values big-endian followed by a
   newline.  The C++ writer in include/neon_amd.hpp (Field::ioToVtk) produces
   the same bytes.
+* XML ImageData (``write_vti`` / ``read_vti``): the files Neon::ioToVTI
+  writes (libNeonCore/include/Neon/core/tools/io/ioToVti.h:663-719): one
+  Float64 DataArray per field, PointData then CellData; ASCII values in
+  fixed notation with 17 decimals, each followed by a space, a tab after
+  each voxel, a newline after each x row and each z plane; BINARY arrays as
+  little-endian binary64 behind a UInt32 byte count, base64 in one stream.
+  The C++ writer (neon_amd::ioToVTI in include/neon_amd.hpp) produces the
+  same bytes.
 * Centre lines (``centerlines`` / ``write_dat``): the two
   NeonUniformLBM_<iter>_{X,Y}.dat files of the cavity benchmark
   (benchmarks/lbm-lid-driven-cavity-flow/src/RunCavityTwoPop.cu:113-149).
 """
 from __future__ import annotations
+
+import base64
+import xml.etree.ElementTree as ET
 
 import numpy as np
 
@@ -126,6 +137,94 @@ def read_vtk(path: str) -> dict:
                 raise ValueError(f"field {name}: missing METADATA block")
         else:
             raise ValueError(f"unexpected line {s!r}")
+    return out
+
+
+def _vti_extent(box) -> str:
+    return f"0 {box[0]} 0 {box[1]} 0 {box[2]}"
+
+
+def _vti_array(name: str, v: np.ndarray, binary: bool) -> bytes:
+    """One DataArray; v has shape (Z, Y, X, ncomp)."""
+    z, y, x, nc = v.shape
+    head = (f'<DataArray type="Float64" NumberOfComponents="{nc}" Name="{name}" '
+            f'format="{"binary" if binary else "ascii"}">\n').encode()
+    if binary:
+        raw = v.astype("<f8").tobytes()
+        if len(raw) > 0xFFFFFFFF:
+            raise ValueError(f"binary array {name} exceeds the 4 GiB UInt32 header")
+        body = base64.b64encode(len(raw).to_bytes(4, "little") + raw) + b"\n</DataArray>\n\n"
+        return head + body
+    fmt = "%d " if np.issubdtype(v.dtype, np.integer) else "%.17f "
+    vals = v.astype(np.int64 if fmt == "%d " else np.float64).reshape(z, y, x, nc).tolist()
+    parts = []
+    for plane in vals:
+        for row in plane:
+            parts.append("".join("".join(fmt % c for c in vox) + "\t" for vox in row) + "\n")
+        parts.append("\n")
+    return head + "".join(parts).encode() + b"</DataArray>\n\n"
+
+
+def write_vti(path: str, fields, node_space, spacing=(1.0, 1.0, 1.0), origin=(0.0, 0.0, 0.0)) -> str:
+    """Write `path` (as given) with fields = [(name, values, is_node, binary)]:
+    values of shape (Z, Y, X) or (Z, Y, X, ncomp) over the node box
+    node_space = (X, Y, Z) when is_node, else over the voxel box (one less in
+    every direction).  Mirrors Neon::ioToVTI(gridsInfo, filename, nodeSpace,
+    voxSpace, spacingData, origin) (ioToVti.h:663-719)."""
+    node = tuple(int(n) for n in node_space)
+    vox = tuple(n - 1 for n in node)
+    arrays = {True: [], False: []}
+    for name, values, is_node, binary in fields:
+        v = np.asarray(values)
+        if v.ndim == 3:
+            v = v[..., None]
+        box = node if is_node else vox
+        if v.ndim != 4 or v.shape[:3] != (box[2], box[1], box[0]):
+            raise ValueError(f"field {name}: shape {v.shape} does not cover the box {box} (x, y, z)")
+        arrays[bool(is_node)].append(_vti_array(name, v, bool(binary)))
+    f6 = lambda t: " ".join("%f" % float(a) for a in t)
+    out = [b'<?xml version="1.0"?>\n',
+           b'<VTKFile type="ImageData" version="0.1" byte_order="LittleEndian">\n',
+           f'<ImageData WholeExtent="{_vti_extent(vox)} " Origin="{f6(origin)}" Spacing="{f6(spacing)} ">\n'.encode(),
+           f'<Piece Extent="{_vti_extent(vox)}" >\n'.encode(),
+           b"<PointData>\n", *arrays[True], b"</PointData>\n",
+           b"<CellData>\n", *arrays[False], b"</CellData>\n",
+           b"</Piece>\n", b" </ImageData>\n", b" </VTKFile>\n"]
+    with open(path, "wb") as fh:
+        fh.write(b"".join(out))
+    return path
+
+
+def read_vti(path: str) -> dict:
+    """Parse a file written by write_vti / neon_amd::ioToVTI with an XML
+    parser: returns {"whole_extent", "origin", "spacing", "point": {name:
+    array}, "cell": {name: array}}, arrays shaped (Z, Y, X, ncomp) float64."""
+    root = ET.parse(path).getroot()
+    if root.tag != "VTKFile" or root.get("type") != "ImageData" or root.get("byte_order") != "LittleEndian":
+        raise ValueError("not a little-endian ImageData VTK XML file")
+    img = root.find("ImageData")
+    ext = [int(t) for t in img.get("WholeExtent").split()]
+    out = {"whole_extent": tuple(ext), "origin": tuple(float(t) for t in img.get("Origin").split()),
+           "spacing": tuple(float(t) for t in img.get("Spacing").split()), "point": {}, "cell": {}}
+    vox = (ext[1] - ext[0], ext[3] - ext[2], ext[5] - ext[4])
+    piece = img.find("Piece")
+    for sec, tag, box in (("point", "PointData", tuple(n + 1 for n in vox)), ("cell", "CellData", vox)):
+        for da in piece.find(tag).findall("DataArray"):
+            if da.get("type") != "Float64":
+                raise ValueError("only Float64 arrays are written")
+            nc = int(da.get("NumberOfComponents"))
+            n = box[0] * box[1] * box[2] * nc
+            if da.get("format") == "binary":
+                raw = base64.b64decode("".join(da.text.split()))
+                nbytes = int.from_bytes(raw[:4], "little")
+                if nbytes != 8 * n or len(raw) != 4 + nbytes:
+                    raise ValueError(f"array {da.get('Name')}: header {nbytes} bytes, expected {8 * n}")
+                arr = np.frombuffer(raw[4:], dtype="<f8").astype(np.float64)
+            else:
+                arr = np.array(da.text.split(), dtype=np.float64)
+            if arr.size != n:
+                raise ValueError(f"array {da.get('Name')}: {arr.size} values, expected {n}")
+            out[sec][da.get("Name")] = arr.reshape(box[2], box[1], box[0], nc)
     return out
 
 

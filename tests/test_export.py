@@ -132,3 +132,91 @@ def test_centerlines_positions_and_components():
     np.testing.assert_array_equal(y[:, 0], np.arange(n) / n)
     np.testing.assert_array_equal(y[:, 1], np.arange(n) * 0.01)
     np.testing.assert_array_equal(x[:, 1], -np.arange(n) * 0.02)
+
+
+# ---- XML ImageData (.vti) ---------------------------------------------------
+
+@pytest.fixture(scope="module")
+def vti_writer(tmp_path_factory):
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    exe = str(tmp_path_factory.mktemp("vti") / "vti_writer")
+    subprocess.run([gxx, "-O1", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    os.path.join(HERE, "vti_writer.cpp"), "-o", exe], check=True)
+    return exe
+
+
+def _vti_values(box, nc, dtype):
+    x, y, z = box
+    zz, yy, xx, k = np.meshgrid(np.arange(z), np.arange(y), np.arange(x), np.arange(nc), indexing="ij")
+    v = (xx + 10 * yy + 100 * zz).astype(np.float64) / 7.0 + k.astype(np.float64) / 1000.0 - 3.0
+    return v.astype(dtype)
+
+
+VTI_CASES = [((3, 2, 2), "double", 0), ((4, 3, 5), "double", 7), ((4, 3, 2), "float", 5),
+             ((5, 2, 3), "int", 2), ((2, 2, 2), "double", 1)]
+
+
+@pytest.mark.parametrize("node,typ,mask", VTI_CASES, ids=[f"{c[1]}-m{c[2]}-{c[0][0]}{c[0][1]}{c[0][2]}" for c in VTI_CASES])
+def test_vti_cpp_and_python_writers_agree(vti_writer, tmp_path, node, typ, mask):
+    dt = {"double": np.float64, "float": np.float32, "int": np.int32}[typ]
+    cpp = str(tmp_path / "cpp.vti")
+    subprocess.run([vti_writer, cpp, *map(str, node), typ, str(mask)], check=True)
+    vox = tuple(n - 1 for n in node)
+    u, rho, w = _vti_values(vox, 3, dt), _vti_values(node, 1, dt), _vti_values(vox, 2, dt)
+    py = export.write_vti(str(tmp_path / "py.vti"),
+                          [("u", u, False, mask & 1), ("rho", rho, True, mask & 2), ("w", w, False, mask & 4)],
+                          node, spacing=(0.5, 1.0, 2.0), origin=(-1.0, 0.0, 3.25))
+    assert open(cpp, "rb").read() == open(py, "rb").read()
+    back = export.read_vti(py)
+    assert back["whole_extent"] == (0, vox[0], 0, vox[1], 0, vox[2])
+    assert back["origin"] == (-1.0, 0.0, 3.25) and back["spacing"] == (0.5, 1.0, 2.0)
+    for name, want, sec, binary in (("u", u, "cell", mask & 1), ("rho", rho, "point", mask & 2),
+                                    ("w", w, "cell", mask & 4)):
+        got = back[sec][name]
+        if binary or typ != "double":
+            np.testing.assert_array_equal(got, want.astype(np.float64))
+        else:
+            np.testing.assert_allclose(got, want, rtol=1e-15, atol=1e-16)  # 17 decimals
+
+
+def test_vti_layout_matches_reference_writer(tmp_path):
+    """Two voxels along x (node box 3x2x2), a two-component cell field in
+    ASCII, written out by hand from the reference's writer
+    (ioToVti.h:431-462 for the array, :695-718 for the file around it)."""
+    v = np.array([[[[0.5, -1.25], [3.0, 1e-7]]]])
+    path = export.write_vti(str(tmp_path / "a.vti"), [("u", v, False, False)], (3, 2, 2))
+    want = ('<?xml version="1.0"?>\n'
+            '<VTKFile type="ImageData" version="0.1" byte_order="LittleEndian">\n'
+            '<ImageData WholeExtent="0 2 0 1 0 1 " Origin="0.000000 0.000000 0.000000" '
+            'Spacing="1.000000 1.000000 1.000000 ">\n'
+            '<Piece Extent="0 2 0 1 0 1" >\n'
+            '<PointData>\n'
+            '</PointData>\n'
+            '<CellData>\n'
+            '<DataArray type="Float64" NumberOfComponents="2" Name="u" format="ascii">\n'
+            '0.50000000000000000 -1.25000000000000000 \t3.00000000000000000 0.00000010000000000 \t\n'
+            '\n'
+            '</DataArray>\n\n'
+            '</CellData>\n'
+            '</Piece>\n'
+            ' </ImageData>\n'
+            ' </VTKFile>\n')
+    assert open(path).read() == want
+
+
+def test_vti_binary_header_and_encoding(tmp_path):
+    """One node value 1.0 in binary: base64 of the UInt32 byte count 8 and the
+    little-endian binary64 bytes of 1.0, as one stream."""
+    path = export.write_vti(str(tmp_path / "b.vti"), [("r", np.ones((1, 1, 1)), True, True)], (1, 1, 1))
+    body = open(path).read().split('format="binary">\n', 1)[1].split("\n", 1)[0]
+    import base64
+    assert base64.b64decode(body) == b"\x08\x00\x00\x00" + b"\x00" * 6 + b"\xf0\x3f"
+
+
+def test_vti_rejects_bad_boxes(vti_writer, tmp_path):
+    with pytest.raises(ValueError):  # cell values over the node box
+        export.write_vti(str(tmp_path / "c.vti"), [("u", np.zeros((2, 2, 2)), False, False)], (2, 2, 2))
+    r = subprocess.run([vti_writer, str(tmp_path / "d.vti"), "0", "2", "2", "double", "1"], capture_output=True)
+    assert r.returncode == 1 and b"negative voxel box" in r.stderr
