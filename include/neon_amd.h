@@ -395,48 +395,61 @@ int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int32_t* best_w
                       float* best_ms);
 int nx_batch_destroy(nx_batch b);
 
-/* ---- two-level grid refinement (Neon's mGrid with depth 2, BGK D3Q19) ---
- * Level 0 is the finest level (the domain dim[3]); a level-1 voxel v covers
- * the level-0 voxels 2v + {0,1}^3.  Each level is stored as a bGrid of 8^3
- * blocks (AoSoA populations, as nx_d3q19_step_block).
- * nx_mgrid_create replaces the mGrid constructor for depth 2
- *   (libNeonDomain/src/domain/details/mGrid/mGrid.cpp:89-264): lam0 is the
- *   level-0 activation lambda at every level-0 voxel, lam1 the level-1
- *   lambda at every level-1 voxel (evaluated at its level-0 origin 2v),
- *   uint8, x fastest.  Every 2^3 group holding an active voxel becomes
- *   active (and so does its parent), and refined level-1 voxels whose 26
- *   neighbours are all refined are removed.  storage NX_F64 or NX_F32.
- * nx_mgrid_state: the state byte of every voxel of level 0 / level 1
- *   (dense, x fastest): bit 0 active, bit 1 refined (has children), bits
- *   2..3 the NX_CELL_* class once nx_mgrid_cavity has run.
+/* ---- grid refinement (Neon's mGrid with depth 2 to 5, BGK D3Q19) -------
+ * Level 0 is the finest level (the domain dim[3]); level l has voxels of
+ * edge 2^l, ceil(dim / 2^l) of them per axis, and a level-(l+1) voxel v
+ * covers the level-l voxels 2v + {0,1}^3.  Each level is stored as a bGrid of
+ * 8^3 blocks (AoSoA populations, as nx_d3q19_step_block).
+ * nx_mgrid_create_levels replaces the mGrid constructor
+ *   (libNeonDomain/src/domain/details/mGrid/mGrid.cpp:89-345): lams[l] is
+ *   the level-l activation lambda at every level-l voxel (evaluated at its
+ *   level-0 origin v 2^l), uint8, x fastest, for l < depth (2..5).  Every
+ *   2^3 group holding an active voxel becomes active (and so does its
+ *   parent); from the coarsest level down, refined voxels whose in-domain
+ *   neighbours are all refined are removed; the strong-balance pass then
+ *   activates the voxels that keep every neighbour of an active voxel of
+ *   level l covered on level l or l + 1.  storage NX_F64 or NX_F32.
+ * nx_mgrid_create: the same for depth 2 (lam0, lam1).
+ * nx_mgrid_depth: the depth and, if dims is not NULL, the voxels per axis of
+ *   every level, dims[3 * l + axis].
+ * nx_mgrid_state_level: the state byte of every voxel of one level (dense,
+ *   x fastest): bit 0 active, bit 1 refined (has children), bits 2..3 the
+ *   NX_CELL_* class once nx_mgrid_cavity has run.  nx_mgrid_state: levels 0
+ *   and 1 (either pointer may be NULL).
  * nx_mgrid_cavity replaces initLidDrivenCavity + initSumStore
  *   (apps/lbmMultiRes/lidDrivenCavity.h:8-78, init.h:8-86): leaf voxels of
  *   level 0 on the domain faces are bounce-back, y = Y-1 the lid moving at
  *   (ulb, 0, 0); populations w_q / 0 / -6 w_q c_q.ulid in both fields; the
- *   coalescence weights 1 / (2 n) of every refined level-1 voxel.
- * nx_mgrid_step replaces nonUniformTimestepRecursive for two levels
- *   (apps/lbmMultiRes/lbmMultiRes.h:141-263) with the baseline operation
- *   order and the fine store: collide level 1 (collide.h:8-149), twice
- *   (collide level 0, store level 0 into level 1 (store.h:98-163), stream
- *   level 0 with explosion (stream.h:170-225)), stream level 1 with
- *   coalescence.  omega0 is the level-1 relaxation; level 0 uses
- *   2 omega0 / (4 - omega0) (computeOmega, util.h:40-45).  The fine-to-coarse
- *   sums are added in the fine voxels' z, y, x order (the reference adds
- *   with atomics in no fixed order), so results are deterministic.
+ *   coalescence weights 1 / (2 n) of every refined voxel.
+ * nx_mgrid_step runs n steps of the coarsest level, each the reference's
+ *   nonUniformTimestepRecursive (apps/lbmMultiRes/lbmMultiRes.h:141-263)
+ *   with the baseline operation order and the fine store: per level l from
+ *   the top, collide l (collide.h:8-149) and store l into l + 1
+ *   (store.h:98-163), the step of level l - 1, stream l with explosion and
+ *   coalescence (stream.h:170-225), and below the top level all of it once
+ *   more (two level-l steps per level-(l+1) step).  omega0 is the coarsest
+ *   level's relaxation; level l uses computeOmega (util.h:40-45).  The
+ *   fine-to-coarse sums are added in the fine voxels' z, y, x order (the
+ *   reference adds with atomics in no fixed order), so results are
+ *   deterministic.
  * Level 0's stream is fused with its next collide (one pass instead of two,
  * same results) when no bulk leaf of level 0 has a link without a source;
  * nx_mgrid_info reports it (fine_fused, after nx_mgrid_cavity) with the
- * block, active and leaf voxel counts per level and the number of fine
- * sources of the store.
+ * block, active and leaf voxel counts per level (arrays of depth entries)
+ * and the number of fine sources of all stores.
  * nx_mgrid_fields copies fin (which 0) or fout (which 1) of one level into
  *   a host array [19][Z][Y][X] of that level's dimensions; entries of
  *   inactive voxels are not written.  Synchronises the device.
  * A handle is not for concurrent use from several host threads (a step
  * swaps level 0's two buffers when it is queued). */
+int nx_mgrid_create_levels(const int32_t dim[3], int32_t depth, const uint8_t* const* lams, int32_t storage,
+                           nx_mgrid* g);
 int nx_mgrid_create(const int32_t dim[3], const uint8_t* lam0, const uint8_t* lam1, int32_t storage,
                     nx_mgrid* g);
-int nx_mgrid_info(nx_mgrid g, uint32_t n_blocks[2], uint64_t n_active[2], uint64_t n_leaf[2],
-                  uint64_t* n_store, int32_t* fine_fused);
+int nx_mgrid_depth(nx_mgrid g, int32_t* depth, int32_t* dims);
+int nx_mgrid_info(nx_mgrid g, uint32_t* n_blocks, uint64_t* n_active, uint64_t* n_leaf, uint64_t* n_store,
+                  int32_t* fine_fused);
+int nx_mgrid_state_level(nx_mgrid g, int32_t level, uint8_t* st);
 int nx_mgrid_state(nx_mgrid g, uint8_t* st0, uint8_t* st1);
 int nx_mgrid_cavity(nx_mgrid g, double ulb);
 int nx_mgrid_step(nx_mgrid g, double omega0, int32_t n_steps, nx_stream s);

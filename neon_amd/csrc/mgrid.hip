@@ -1,28 +1,31 @@
-// mgrid.hip -- two-level grid refinement LBM (Neon's mGrid with depth 2 and
-// the lbmMultiRes application's BGK D3Q19 scheme) on a stack of two
-// block-sparse levels.
+// mgrid.hip -- grid-refinement LBM (Neon's mGrid, depth 2 to 5, and the
+// lbmMultiRes application's BGK D3Q19 scheme) on a stack of block-sparse
+// levels.
 //
-// Level 0 is the finest level (the domain); a level-1 voxel v covers the
-// level-0 voxels 2v + {0,1}^3.  Each level is a bGrid of 8^3 blocks (the
-// library's builder: blocks ordered bz, by, bx, 27-entry connectivity) with
-// AoSoA populations (b*19 + q)*512 + x + 8y + 64z, and a dense table from
-// block coordinates to block index for the cross-level lookups.
+// Level 0 is the finest level (the domain); level l has voxels of edge 2^l,
+// ceil(dim / 2^l) per axis, and a level-(l+1) voxel v covers the level-l
+// voxels 2v + {0,1}^3.  Each level is a bGrid of 8^3 blocks (the library's
+// builder: blocks ordered bz, by, bx, 27-entry connectivity) with AoSoA
+// populations (b*19 + q)*512 + x + 8y + 64z, and a dense table from block
+// coordinates to block index for the cross-level lookups.
 //
 // Everything that depends only on the geometry is resolved on the host when
-// the cavity is set up: every bulk leaf voxel carries a 2-bit state per
-// population (plain pull, bounce-back link, interface link, no update), the
-// fine-to-coarse accumulation is a gather list whose per-target order is the
-// fine voxels' z, y, x order, and the coalescence weights are a field of
-// level 1.  The kernels then do no searching: per coarse step
-//   collide L1 | 2 x (collide L0, store L0 -> L1, stream L0) | stream L1
-// (lbmMultiRes.h:141-263 for two levels, baseline operation order with the
-// fine store).  Reference semantics restated (file:line in each kernel):
-//   grid      libNeonDomain/src/domain/details/mGrid/mGrid.cpp:89-264
+// the cavity is set up: every bulk leaf voxel carries a 3-bit state per
+// population (plain pull, bounce-back link, explosion link, coalescence
+// link, no update), each fine-to-coarse accumulation is a gather list whose
+// per-target order is the fine voxels' z, y, x order, and the coalescence
+// weights are a field of every level above 0.  The kernels then do no
+// searching.  One coarse step is the reference's recursion
+//   step(l): collide l, store l -> l+1 | step(l-1) | stream l
+//            | unless l is the top: the same once more
+// (lbmMultiRes.h:141-263, baseline operation order with the fine store).
+// Reference semantics restated (file:line in each kernel):
+//   grid      libNeonDomain/src/domain/details/mGrid/mGrid.cpp:89-345, 466-555
 //   set-up    apps/lbmMultiRes/lidDrivenCavity.h:8-78, init.h:8-86
 //   collide   apps/lbmMultiRes/collide.h:8-149
 //   store     apps/lbmMultiRes/store.h:98-163
 //   stream    apps/lbmMultiRes/stream.h:170-225
-//   helpers   apps/lbmMultiRes/util.h:19-45, mPartition_imp.h:186-235
+//   helpers   apps/lbmMultiRes/util.h:19-45, mPartition_imp.h:155-235
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -38,8 +41,9 @@ namespace {
 // state byte of a voxel (host and device)
 constexpr uint32_t kActive = 1u, kRefined = 2u;
 constexpr int kClsShift = 2;
-// per-population link states of a bulk leaf voxel
-constexpr uint64_t kPlain = 0, kWall = 1, kIface = 2, kSkip = 3;
+// per-population link states of a bulk leaf voxel, 3 bits each
+constexpr uint64_t kPlain = 0, kWall = 1, kExpl = 2, kCoal = 3, kSkip = 4;
+constexpr int kMaxDepth = 5;
 
 __host__ __device__ inline int cls_bits(uint32_t st) { return (int)((st >> kClsShift) & 3u); }
 __host__ __device__ inline bool bulk_leaf(uint32_t st)
@@ -61,8 +65,8 @@ struct MS<NX_F32> {
 // ---- kernels ----------------------------------------------------------------
 
 // collide.h:8-149: BGK on every bulk leaf voxel (fin -> fout, same voxel);
-// a refined voxel of level 1 resets its fout, the accumulator of the fine
-// store, to 0.
+// a refined voxel (levels above 0) resets its fout, the accumulator of the
+// fine store, to 0.
 template <int S>
 __global__ void __launch_bounds__(kBV)
     k_mr_collide(const typename MS<S>::T* __restrict__ fin, typename MS<S>::T* __restrict__ fout,
@@ -87,22 +91,22 @@ __global__ void __launch_bounds__(kBV)
 }
 
 // store.h:98-163 as a gather: entry e adds its fine sources, in order, to
-// the level-1 accumulator target[e].
+// the coarse accumulator target[e].
 template <int S>
 __global__ void __launch_bounds__(256)
-    k_mr_store(const typename MS<S>::T* __restrict__ fout0, typename MS<S>::T* __restrict__ fout1,
+    k_mr_store(const typename MS<S>::T* __restrict__ fine, typename MS<S>::T* __restrict__ coarse,
                const uint64_t* __restrict__ target, const uint32_t* __restrict__ first,
                const uint64_t* __restrict__ src, uint32_t n)
 {
     using T = typename MS<S>::T;
     const uint32_t e = blockIdx.x * 256 + threadIdx.x;
     if (e >= n) return;
-    T acc = fout1[target[e]];
-    for (uint32_t k = first[e]; k < first[e + 1]; k++) acc += fout0[src[k]];
-    fout1[target[e]] = acc;
+    T acc = coarse[target[e]];
+    for (uint32_t k = first[e]; k < first[e + 1]; k++) acc += fine[src[k]];
+    coarse[target[e]] = acc;
 }
 
-struct UpLevel {  // the coarser level, seen from level 0 (explosion)
+struct UpLevel {  // the next coarser level, seen from this one (explosion)
     const void* fout;
     const uint32_t* table;
     int32_t nb[3];
@@ -112,17 +116,17 @@ struct UpLevel {  // the coarser level, seen from level 0 (explosion)
 // the bulk leaf voxels of one level.  The plain pull of all 19 populations
 // is issued first (upstream voxel through the connectivity row in scalar
 // registers); voxels with another link state then fix those links up:
-// bounce-back (own opposite + the wall's opposite), LEVEL 0 interface links
-// read the coarse post-collision population of the uncle, the level-1 voxel
-// holding the upstream position (explosion), LEVEL 1 interface links scale
-// the refined upstream voxel's accumulated fine populations by its weight
-// (coalescence); links in state "none" keep the value in `out`.
+// bounce-back (own opposite + the wall's opposite), explosion (the coarse
+// post-collision population of the uncle, the next level's voxel holding
+// the upstream position), coalescence (the refined upstream voxel's
+// accumulated fine populations times its weight); links in state "none"
+// keep the value in `out`.
 // COLLIDE: the pulled populations are collided at once with omega and the
 // result written to `out` -- stream followed by the next collide of the
 // same level (collide.h:8-149) in one pass; the results are those of the two
 // kernels in sequence, bit for bit.  KEEP = false: the level has no link in
 // state "none" (checked on the host), every population is stored.
-template <int S, int LEVEL, bool COLLIDE, bool KEEP>
+template <int S, bool COLLIDE, bool KEEP>
 __global__ void __launch_bounds__(kBV)
     k_mr_stream(const typename MS<S>::T* __restrict__ fout, typename MS<S>::T* __restrict__ out,
                 const uint32_t* __restrict__ state, const uint64_t* __restrict__ links,
@@ -155,7 +159,7 @@ __global__ void __launch_bounds__(kBV)
         for_q(AllQ{}, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
             if constexpr (q != 9) {
-                const uint64_t kind = (lk >> (2 * q)) & 3u;
+                const uint64_t kind = (lk >> (3 * q)) & 7u;
                 if (kind == kPlain) return;
                 uint32_t t, nv;
                 upstream<q>(b, cb, x, y, z, t, nv);
@@ -164,22 +168,21 @@ __global__ void __launch_bounds__(kBV)
                     constexpr int o = OPP[q];
                     ok = t != kInvalidBlock;
                     if (ok) f[q] = fout[own + (uint64_t)o * kBV] + fout[((uint64_t)t * Q + o) * kBV + nv];
-                } else if (kind == kIface) {
-                    if constexpr (LEVEL == 0) {
-                        const int ux = (origins[3 * (uint64_t)b] + x - CX[q]) >> 1;
-                        const int uy = (origins[3 * (uint64_t)b + 1] + y - CY[q]) >> 1;
-                        const int uz = (origins[3 * (uint64_t)b + 2] + z - CZ[q]) >> 1;
-                        const int bx = ux >> 3, by = uy >> 3, bz = uz >> 3;
-                        ok = !(bx < 0 || by < 0 || bz < 0 || bx >= up.nb[0] || by >= up.nb[1] || bz >= up.nb[2]);
-                        const uint32_t tb = ok ? up.table[((uint64_t)bz * up.nb[1] + by) * up.nb[0] + bx]
-                                               : kInvalidBlock;
-                        ok = tb != kInvalidBlock;
-                        const uint32_t slot = (uint32_t)((ux & 7) + 8 * (uy & 7) + 64 * (uz & 7));
-                        if (ok) f[q] = ((const T*)up.fout)[((uint64_t)tb * Q + q) * kBV + slot];
-                    } else {
-                        ok = t != kInvalidBlock;
-                        if (ok) f[q] = f[q] * (T)ss[((uint64_t)t * Q + q) * kBV + nv];
-                    }
+                } else if (kind == kExpl) {
+                    const int ux = (origins[3 * (uint64_t)b] + x - CX[q]) >> 1;
+                    const int uy = (origins[3 * (uint64_t)b + 1] + y - CY[q]) >> 1;
+                    const int uz = (origins[3 * (uint64_t)b + 2] + z - CZ[q]) >> 1;
+                    const int bx = ux >> 3, by = uy >> 3, bz = uz >> 3;
+                    ok = up.table && !(bx < 0 || by < 0 || bz < 0 || bx >= up.nb[0] || by >= up.nb[1] ||
+                                       bz >= up.nb[2]);
+                    const uint32_t tb = ok ? up.table[((uint64_t)bz * up.nb[1] + by) * up.nb[0] + bx]
+                                           : kInvalidBlock;
+                    ok = tb != kInvalidBlock;
+                    const uint32_t slot = (uint32_t)((ux & 7) + 8 * (uy & 7) + 64 * (uz & 7));
+                    if (ok) f[q] = ((const T*)up.fout)[((uint64_t)tb * Q + q) * kBV + slot];
+                } else if (kind == kCoal) {
+                    ok = t != kInvalidBlock && ss;
+                    if (ok) f[q] = f[q] * (T)ss[((uint64_t)t * Q + q) * kBV + nv];
                 } else {
                     ok = false;
                 }
@@ -219,8 +222,10 @@ struct Level {
     uint32_t* d_conn = nullptr;
     int32_t* d_origins = nullptr;
     uint32_t* d_table = nullptr;
+    float* d_ss = nullptr;  // coalescence weights (levels above 0)
     void* fin = nullptr;
     void* fout = nullptr;
+    bool keep = true;  // the level has links in state "none"
 
     size_t nvox() const { return (size_t)dim[0] * dim[1] * dim[2]; }
     size_t lin(int x, int y, int z) const { return (size_t)x + (size_t)dim[0] * ((size_t)y + (size_t)dim[1] * z); }
@@ -240,22 +245,24 @@ struct Level {
     static uint64_t elem(int64_t slot, int q) { return ((uint64_t)(slot / kBV) * Q + q) * kBV + slot % kBV; }
 };
 
+struct Store {  // fine-to-coarse gather list of one level pair (l -> l + 1)
+    uint32_t n = 0;
+    uint64_t n_src = 0;
+    uint64_t* d_target = nullptr;
+    uint32_t* d_first = nullptr;
+    uint64_t* d_src = nullptr;
+};
+
 }  // namespace
 }  // namespace mres
 }  // namespace nx
 
 struct nx_mgrid_s {
     int storage = NX_F64;
-    nx::mres::Level L[2];
-    std::vector<uint8_t> parent_block;  // level-1 8^3 block (dense block coords) holds an active voxel
-    float* d_ss = nullptr;
-    uint32_t n_store = 0;
-    uint64_t* d_target = nullptr;
-    uint32_t* d_first = nullptr;
-    uint64_t* d_src = nullptr;
-    uint64_t n_store_src = 0;
-    bool fuse_fine = false;   // level 0 without "none" links: stream fused with the next collide
-    bool keep_coarse = true;  // level 1 has "none" links
+    int depth = 2;
+    nx::mres::Level L[nx::mres::kMaxDepth];
+    nx::mres::Store store[nx::mres::kMaxDepth];
+    bool fuse_fine = false;  // level 0 without "none" links: stream fused with the next collide
     bool ready = false;
 };
 
@@ -275,64 +282,114 @@ int upload(void** dst, const void* src, size_t bytes)
     return NX_OK;
 }
 
-// Activation of both levels and overlap culling, mGrid.cpp:89-264 for depth
-// 2 (the strong-balance pass, :266-345, cannot change a two-level grid).
-void activate(const int32_t dim[3], const uint8_t* lam0, const uint8_t* lam1, Level& f, Level& c)
+// isRefined (mGrid.cpp:167-199): a child of level-l voxel (x, y, z) is
+// active on level l - 1 (the voxel itself need not be)
+bool any_child(const Level& fine, int x, int y, int z)
 {
-    auto group = [](Level& lv, const uint8_t* lam, int gx, int gy, int gz) {
-        bool any = false;
-        for (int k = 0; k < 8; k++) {
-            const int x = 2 * gx + (k & 1), y = 2 * gy + ((k >> 1) & 1), z = 2 * gz + (k >> 2);
-            if (lv.inside(x, y, z) && (lv.st[lv.lin(x, y, z)] || lam[lv.lin(x, y, z)])) any = true;
-        }
-        if (any)
-            for (int k = 0; k < 8; k++) {
-                const int x = 2 * gx + (k & 1), y = 2 * gy + ((k >> 1) & 1), z = 2 * gz + (k >> 2);
-                if (lv.inside(x, y, z)) lv.st[lv.lin(x, y, z)] = (uint8_t)kActive;
-            }
-        return any;
-    };
-    for (int a = 0; a < 3; a++) {
-        f.dim[a] = dim[a];
-        c.dim[a] = (dim[a] + 1) / 2;
-    }
-    f.st.assign(f.nvox(), 0);
-    c.st.assign(c.nvox(), 0);
-    // level 0: a group (one level-1 voxel) with an active voxel activates its
-    // voxels and its parent
-    for (int gz = 0; gz < c.dim[2]; gz++)
-        for (int gy = 0; gy < c.dim[1]; gy++)
-            for (int gx = 0; gx < c.dim[0]; gx++)
-                if (group(f, lam0, gx, gy, gz)) c.st[c.lin(gx, gy, gz)] = (uint8_t)kActive;
-    // level 1, the top level: groups of 2^3 level-1 voxels
-    for (int gz = 0; gz < (c.dim[2] + 1) / 2; gz++)
-        for (int gy = 0; gy < (c.dim[1] + 1) / 2; gy++)
-            for (int gx = 0; gx < (c.dim[0] + 1) / 2; gx++) group(c, lam1, gx, gy, gz);
-    std::vector<uint8_t> refined(c.nvox(), 0);
-    for (int z = 0; z < c.dim[2]; z++)
-        for (int y = 0; y < c.dim[1]; y++)
-            for (int x = 0; x < c.dim[0]; x++) {
-                bool r = false;
-                for (int k = 0; k < 8 && !r; k++)
-                    r = f.active(2 * x + (k & 1), 2 * y + ((k >> 1) & 1), 2 * z + (k >> 2));
-                refined[c.lin(x, y, z)] = r;
-            }
-    // culling: a refined voxel whose 26 in-domain neighbours are refined goes
-    std::vector<uint8_t> keep = c.st;
-    for (int z = 0; z < c.dim[2]; z++)
-        for (int y = 0; y < c.dim[1]; y++)
-            for (int x = 0; x < c.dim[0]; x++) {
-                const size_t i = c.lin(x, y, z);
-                if (!c.st[i] || !refined[i]) continue;
-                bool all = true;
-                for (int d = 0; d < 27 && all; d++) {
-                    if (d == 13) continue;
-                    const int nx_ = x + d % 3 - 1, ny = y + (d / 3) % 3 - 1, nz = z + d / 9 - 1;
-                    if (c.inside(nx_, ny, nz) && !refined[c.lin(nx_, ny, nz)]) all = false;
+    for (int k = 0; k < 8; k++)
+        if (fine.active(2 * x + (k & 1), 2 * y + ((k >> 1) & 1), 2 * z + (k >> 2))) return true;
+    return false;
+}
+
+// The mGrid constructor (mGrid.cpp:89-345): activation level by level in
+// 2^3 groups, each active group activating its parent; overlap culling from
+// the coarsest level down (a refined voxel whose in-domain neighbours are
+// all refined goes; level l - 1 is still unculled when level l is visited);
+// the strong-balance pass; then the refined flags from the final masks
+// (:466-555).  Returns the number of voxels the strong-balance pass added.
+uint64_t activate(const int32_t dim[3], int depth, const uint8_t* const* lams, Level* L)
+{
+    for (int a = 0; a < 3; a++) L[0].dim[a] = dim[a];
+    for (int l = 1; l < depth; l++)
+        for (int a = 0; a < 3; a++) L[l].dim[a] = (L[l - 1].dim[a] + 1) / 2;
+    for (int l = 0; l < depth; l++) L[l].st.assign(L[l].nvox(), 0);
+    for (int l = 0; l < depth; l++) {
+        Level& lv = L[l];
+        const uint8_t* lam = lams[l];
+        for (int gz = 0; gz < (lv.dim[2] + 1) / 2; gz++)
+            for (int gy = 0; gy < (lv.dim[1] + 1) / 2; gy++)
+                for (int gx = 0; gx < (lv.dim[0] + 1) / 2; gx++) {
+                    bool any = false;
+                    for (int k = 0; k < 8; k++) {
+                        const int x = 2 * gx + (k & 1), y = 2 * gy + ((k >> 1) & 1), z = 2 * gz + (k >> 2);
+                        if (lv.inside(x, y, z) && (lv.st[lv.lin(x, y, z)] || lam[lv.lin(x, y, z)])) any = true;
+                    }
+                    if (!any) continue;
+                    for (int k = 0; k < 8; k++) {
+                        const int x = 2 * gx + (k & 1), y = 2 * gy + ((k >> 1) & 1), z = 2 * gz + (k >> 2);
+                        if (lv.inside(x, y, z)) lv.st[lv.lin(x, y, z)] = (uint8_t)kActive;
+                    }
+                    if (l + 1 < depth) L[l + 1].st[L[l + 1].lin(gx, gy, gz)] = (uint8_t)kActive;
                 }
-                if (all) keep[i] = 0;
-            }
-    for (size_t i = 0; i < c.nvox(); i++) c.st[i] = keep[i] ? (uint8_t)(kActive | (refined[i] ? kRefined : 0u)) : 0;
+    }
+    for (int l = depth - 1; l > 0; l--) {
+        Level& c = L[l];
+        std::vector<uint8_t> refined(c.nvox(), 0);
+        for (int z = 0; z < c.dim[2]; z++)
+            for (int y = 0; y < c.dim[1]; y++)
+                for (int x = 0; x < c.dim[0]; x++) refined[c.lin(x, y, z)] = any_child(L[l - 1], x, y, z);
+        for (int z = 0; z < c.dim[2]; z++)
+            for (int y = 0; y < c.dim[1]; y++)
+                for (int x = 0; x < c.dim[0]; x++) {
+                    const size_t i = c.lin(x, y, z);
+                    if (!c.st[i] || !refined[i]) continue;
+                    bool all = true;
+                    for (int d = 0; d < 27 && all; d++) {
+                        if (d == 13) continue;
+                        const int nx_ = x + d % 3 - 1, ny = y + (d / 3) % 3 - 1, nz = z + d / 9 - 1;
+                        if (c.inside(nx_, ny, nz) && !refined[c.lin(nx_, ny, nz)]) all = false;
+                    }
+                    if (all) c.st[i] = 0;
+                }
+    }
+    // strong balance: the finest active voxel covering each in-domain
+    // neighbour position of an active voxel of level l lies on level l or
+    // l + 1; found on l_n >= l + 2, the voxel of level l_n - 1 covering it
+    // is activated alone and the search continues upwards; passes repeat
+    // until one activates nothing and see their own activations
+    uint64_t added = 0;
+    for (bool again = true; again;) {
+        again = false;
+        for (int l = 0; l < depth; l++) {
+            const Level& lv = L[l];
+            for (int bz = 0; bz < (lv.dim[2] + 1) / 2; bz++)
+                for (int by = 0; by < (lv.dim[1] + 1) / 2; by++)
+                    for (int bx = 0; bx < (lv.dim[0] + 1) / 2; bx++)
+                        for (int k8 = 0; k8 < 8; k8++) {
+                            const int x = 2 * bx + (k8 & 1), y = 2 * by + ((k8 >> 1) & 1), z = 2 * bz + (k8 >> 2);
+                            if (!lv.active(x, y, z)) continue;
+                            for (int d = 0; d < 27; d++) {
+                                if (d == 13) continue;
+                                int p[3] = {x + d % 3 - 1, y + (d / 3) % 3 - 1, z + d / 9 - 1};
+                                if (!lv.inside(p[0], p[1], p[2])) continue;
+                                int prev[3] = {0, 0, 0};
+                                for (int ln = l; ln < depth; ln++) {
+                                    if (L[ln].active(p[0], p[1], p[2])) {
+                                        if (ln <= l + 1) break;
+                                        uint8_t& s = L[ln - 1].st[L[ln - 1].lin(prev[0], prev[1], prev[2])];
+                                        added += (s & kActive) ? 0 : 1;
+                                        s = (uint8_t)kActive;
+                                        again = true;
+                                    }
+                                    for (int a = 0; a < 3; a++) {
+                                        prev[a] = p[a];
+                                        p[a] >>= 1;
+                                    }
+                                }
+                            }
+                        }
+        }
+    }
+    for (int l = 1; l < depth; l++) {
+        Level& c = L[l];
+        for (int z = 0; z < c.dim[2]; z++)
+            for (int y = 0; y < c.dim[1]; y++)
+                for (int x = 0; x < c.dim[0]; x++) {
+                    uint8_t& s = c.st[c.lin(x, y, z)];
+                    if ((s & kActive) && any_child(L[l - 1], x, y, z)) s |= (uint8_t)kRefined;
+                }
+    }
+    return added;
 }
 
 int build_level(Level& lv)
@@ -364,13 +421,13 @@ inline int uoff(int x, int d)
 }
 
 template <typename T>
-void fill_cavity(nx_mgrid_s& g, double ulb, std::vector<T> pop[2])
+void fill_cavity(nx_mgrid_s& g, double ulb, std::vector<T>* pop)
 {
     // lidDrivenCavity.h:8-78: leaf voxels of level 0 on the domain faces are
     // bounce-back, those with y = Y-1 the lid; populations w_q / 0 /
     // -6 w_q (c_q . ulid), ulid = (ulb, 0, 0), evaluated as the reference
     // does in T with binary64 constants
-    for (int l = 0; l < 2; l++) {
+    for (int l = 0; l < g.depth; l++) {
         Level& lv = g.L[l];
         pop[l].assign(19 * (size_t)kBV * lv.n_blocks, T(0));
         for (int z = 0; z < lv.dim[2]; z++)
@@ -401,30 +458,30 @@ void fill_cavity(nx_mgrid_s& g, double ulb, std::vector<T> pop[2])
     }
 }
 
-bool has_parent(const nx_mgrid_s& g, int x, int y, int z)
+// hasParent (mPartition_imp.h:172-208): the memory block (8^3) of level
+// l + 1 holding the parent of level-l voxel (x, y, z) exists
+bool has_parent(const nx_mgrid_s& g, int l, int x, int y, int z)
 {
-    // mPartition_imp.h:199-208: the level-1 memory block (8^3) holding the
-    // parent voxel exists
-    const Level& c = g.L[1];
+    if (l + 1 >= g.depth) return false;
+    const Level& c = g.L[l + 1];
     const int bx = (x >> 1) >> 3, by = (y >> 1) >> 3, bz = (z >> 1) >> 3;
     if (bx >= c.nb[0] || by >= c.nb[1] || bz >= c.nb[2]) return false;
     return c.table[((size_t)bz * c.nb[1] + by) * c.nb[0] + bx] != kInvalidBlock;
 }
 
-// Link states of every bulk leaf voxel (stream.h:170-225 decided per link);
-// returns the number of links in state "none".
+// Link states of every bulk leaf voxel of level l (stream.h:170-225 decided
+// per link); returns the number of links in state "none".
 uint64_t links_of(const nx_mgrid_s& g, int l, std::vector<uint64_t>& out)
 {
     uint64_t n_skip = 0;
     const Level& lv = g.L[l];
-    const Level& c = g.L[1];
     out.assign((size_t)kBV * lv.n_blocks, 0);
     for (int z = 0; z < lv.dim[2]; z++)
         for (int y = 0; y < lv.dim[1]; y++)
             for (int x = 0; x < lv.dim[0]; x++) {
                 const uint8_t st = lv.st[lv.lin(x, y, z)];
                 if (!bulk_leaf(st)) continue;
-                const bool parent = l == 0 && has_parent(g, x, y, z);
+                const bool parent = has_parent(g, l, x, y, z);
                 uint64_t lk = 0;
                 for (int q = 0; q < Q; q++) {
                     if (q == 9) continue;  // own population: plain
@@ -433,15 +490,16 @@ uint64_t links_of(const nx_mgrid_s& g, int l, std::vector<uint64_t>& out)
                     if (lv.active(nx_, ny, nz)) {
                         const uint8_t ns = lv.st[lv.lin(nx_, ny, nz)];
                         if (ns & kRefined)
-                            kind = kIface;  // coalescence (level 1 only)
+                            kind = kCoal;
                         else
                             kind = cls_bits(ns) == NX_CELL_BULK ? kPlain : kWall;
                     } else if (parent) {
+                        const Level& c = g.L[l + 1];
                         const int ux = (x >> 1) + uoff(x, -CX[q]), uy = (y >> 1) + uoff(y, -CY[q]),
                                   uz = (z >> 1) + uoff(z, -CZ[q]);
-                        if (c.active(ux, uy, uz)) kind = kIface;  // explosion
+                        if (c.active(ux, uy, uz)) kind = kExpl;
                     }
-                    lk |= kind << (2 * q);
+                    lk |= kind << (3 * q);
                     n_skip += kind == kSkip;
                 }
                 out[lv.slot(x, y, z)] = lk;
@@ -449,21 +507,35 @@ uint64_t links_of(const nx_mgrid_s& g, int l, std::vector<uint64_t>& out)
     return n_skip;
 }
 
+int check_level(const nx_mgrid_s* g, int32_t level)
+{
+    if (!g) return set_error(NX_ERR_INVALID, "null mgrid");
+    if (level < 0 || level >= g->depth)
+        return set_error(NX_ERR_INVALID, "mgrid: level " + std::to_string(level) + " outside [0, " +
+                                             std::to_string(g->depth) + ")");
+    return NX_OK;
+}
+
 }  // namespace
 
-extern "C" int nx_mgrid_create(const int32_t dim[3], const uint8_t* lam0, const uint8_t* lam1, int32_t storage,
-                               nx_mgrid* out)
+extern "C" int nx_mgrid_create_levels(const int32_t dim[3], int32_t depth, const uint8_t* const* lams,
+                                      int32_t storage, nx_mgrid* out)
 {
-    if (!dim || !lam0 || !lam1 || !out) return set_error(NX_ERR_INVALID, "nx_mgrid_create: null argument");
+    if (!dim || !lams || !out) return set_error(NX_ERR_INVALID, "nx_mgrid_create_levels: null argument");
+    if (depth < 2 || depth > kMaxDepth)
+        return set_error(NX_ERR_INVALID, "nx_mgrid_create_levels: depth in [2, " + std::to_string(kMaxDepth) + "]");
+    for (int l = 0; l < depth; l++)
+        if (!lams[l]) return set_error(NX_ERR_INVALID, "nx_mgrid_create_levels: null lambda array");
     for (int a = 0; a < 3; a++)
         if (dim[a] < 2 || dim[a] > (1 << 14))
-            return set_error(NX_ERR_INVALID, "nx_mgrid_create: dimensions in [2, 16384]");
+            return set_error(NX_ERR_INVALID, "nx_mgrid_create_levels: dimensions in [2, 16384]");
     if (storage != NX_F64 && storage != NX_F32)
         return set_error(NX_ERR_UNSUPPORTED, "nx_mgrid_create: storage NX_F64 or NX_F32 (the reference's types)");
     auto* g = new nx_mgrid_s;
     g->storage = storage;
-    activate(dim, lam0, lam1, g->L[0], g->L[1]);
-    for (int l = 0; l < 2; l++)
+    g->depth = depth;
+    activate(dim, depth, lams, g->L);
+    for (int l = 0; l < depth; l++)
         if (int e = build_level(g->L[l])) {
             nx_mgrid_destroy(g);
             return e;
@@ -472,12 +544,31 @@ extern "C" int nx_mgrid_create(const int32_t dim[3], const uint8_t* lam0, const 
     return NX_OK;
 }
 
-extern "C" int nx_mgrid_info(nx_mgrid g, uint32_t n_blocks[2], uint64_t n_active[2], uint64_t n_leaf[2],
+extern "C" int nx_mgrid_create(const int32_t dim[3], const uint8_t* lam0, const uint8_t* lam1, int32_t storage,
+                               nx_mgrid* out)
+{
+    if (!lam0 || !lam1) return set_error(NX_ERR_INVALID, "nx_mgrid_create: null argument");
+    const uint8_t* lams[2] = {lam0, lam1};
+    return nx_mgrid_create_levels(dim, 2, lams, storage, out);
+}
+
+extern "C" int nx_mgrid_depth(nx_mgrid g, int32_t* depth, int32_t* dims)
+{
+    if (!g) return set_error(NX_ERR_INVALID, "null mgrid");
+    if (depth) *depth = g->depth;
+    if (dims)
+        for (int l = 0; l < g->depth; l++)
+            for (int a = 0; a < 3; a++) dims[3 * l + a] = g->L[l].dim[a];
+    return NX_OK;
+}
+
+extern "C" int nx_mgrid_info(nx_mgrid g, uint32_t* n_blocks, uint64_t* n_active, uint64_t* n_leaf,
                              uint64_t* n_store, int32_t* fine_fused)
 {
     if (g && fine_fused) *fine_fused = g->fuse_fine ? 1 : 0;
     if (!g) return set_error(NX_ERR_INVALID, "null mgrid");
-    for (int l = 0; l < 2; l++) {
+    uint64_t ns = 0;
+    for (int l = 0; l < g->depth; l++) {
         uint64_t a = 0, lf = 0;
         for (uint8_t s : g->L[l].st) {
             a += (s & kActive) ? 1 : 0;
@@ -486,8 +577,17 @@ extern "C" int nx_mgrid_info(nx_mgrid g, uint32_t n_blocks[2], uint64_t n_active
         if (n_blocks) n_blocks[l] = g->L[l].n_blocks;
         if (n_active) n_active[l] = a;
         if (n_leaf) n_leaf[l] = lf;
+        ns += g->store[l].n_src;
     }
-    if (n_store) *n_store = g->n_store_src;
+    if (n_store) *n_store = ns;
+    return NX_OK;
+}
+
+extern "C" int nx_mgrid_state_level(nx_mgrid g, int32_t level, uint8_t* st)
+{
+    if (int e = check_level(g, level)) return e;
+    if (!st) return set_error(NX_ERR_INVALID, "nx_mgrid_state_level: null output");
+    std::copy(g->L[level].st.begin(), g->L[level].st.end(), st);
     return NX_OK;
 }
 
@@ -503,26 +603,25 @@ extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
 {
     if (!g) return set_error(NX_ERR_INVALID, "null mgrid");
     if (!std::isfinite(ulb)) return set_error(NX_ERR_INVALID, "nx_mgrid_cavity: non-finite lid speed");
-    Level& f = g->L[0];
-    Level& c = g->L[1];
-    const size_t elems[2] = {19 * (size_t)kBV * f.n_blocks, 19 * (size_t)kBV * c.n_blocks};
+    const int D = g->depth;
     const size_t es = g->storage == NX_F64 ? 8 : 4;
     // populations
-    std::vector<double> p64[2];
-    std::vector<float> p32[2];
+    std::vector<double> p64[kMaxDepth];
+    std::vector<float> p32[kMaxDepth];
     if (g->storage == NX_F64)
         fill_cavity<double>(*g, ulb, p64);
     else
         fill_cavity<float>(*g, ulb, p32);
-    for (int l = 0; l < 2; l++) {
+    for (int l = 0; l < D; l++) {
         Level& lv = g->L[l];
+        const size_t elems = 19 * (size_t)kBV * lv.n_blocks;
         if (int e = upload((void**)&lv.d_conn, lv.conn.data(), lv.conn.size() * 4)) return e;
         if (int e = upload((void**)&lv.d_origins, lv.origins.data(), lv.origins.size() * 4)) return e;
         if (int e = upload((void**)&lv.d_table, lv.table.data(), lv.table.size() * 4)) return e;
         const void* src = g->storage == NX_F64 ? (const void*)p64[l].data() : (const void*)p32[l].data();
-        if (int e = upload(&g->L[l].fin, src, elems[l] * es)) return e;
-        if (int e = upload(&g->L[l].fout, src, elems[l] * es)) return e;
-        std::vector<uint32_t> st((size_t)kBV * g->L[l].n_blocks, 0);
+        if (int e = upload(&lv.fin, src, elems * es)) return e;
+        if (int e = upload(&lv.fout, src, elems * es)) return e;
+        std::vector<uint32_t> st((size_t)kBV * lv.n_blocks, 0);
         for (int z = 0; z < lv.dim[2]; z++)
             for (int y = 0; y < lv.dim[1]; y++)
                 for (int x = 0; x < lv.dim[0]; x++) {
@@ -533,53 +632,61 @@ extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
         std::vector<uint64_t> lk;
         const uint64_t n_skip = links_of(*g, l, lk);
         if (l == 0) g->fuse_fine = n_skip == 0;
-        if (l == 1) g->keep_coarse = n_skip != 0;
+        lv.keep = n_skip != 0;
         if (int e = upload((void**)&lv.d_links, lk.data(), lk.size() * 8)) return e;
     }
     // fine store gather lists (store.h:98-163) and sumStore counts
-    // (init.h:8-55): fine voxels in z, y, x order, q ascending
-    std::vector<float> cnt(elems[1], 0.f);
-    std::vector<std::pair<uint64_t, uint64_t>> tl;  // (target element, source element)
-    for (int z = 0; z < f.dim[2]; z++)
-        for (int y = 0; y < f.dim[1]; y++)
-            for (int x = 0; x < f.dim[0]; x++) {
-                if (!f.active(x, y, z) || !has_parent(*g, x, y, z)) continue;
-                const int px = x >> 1, py = y >> 1, pz = z >> 1;
-                for (int q = 0; q < Q; q++) {
-                    if (q == 9) continue;
-                    const int ux = uoff(x, CX[q]), uy = uoff(y, CY[q]), uz = uoff(z, CZ[q]);
-                    if (f.active(x + ux, y + uy, z + uz)) continue;
-                    if (!c.active(px + ux, py + uy, pz + uz)) continue;
-                    const int sx = ux - CX[q], sy = uy - CY[q], sz = uz - CZ[q];
-                    const int64_t cs = c.slot(px + sx, py + sy, pz + sz);
-                    if (cs < 0) continue;
-                    cnt[Level::elem(cs, q)] += 1.f;  // initSumStore has no csChild test
-                    if (!f.active(x + sx, y + sy, z + sz)) continue;
-                    tl.emplace_back(Level::elem(cs, q), Level::elem(f.slot(x, y, z), q));
+    // (init.h:8-55) of every level pair: fine voxels in z, y, x order, q
+    // ascending.  The counts take every active fine voxel with a parent,
+    // the store only those without children whose csChild is active.
+    for (int l = 0; l + 1 < D; l++) {
+        const Level& f = g->L[l];
+        const Level& c = g->L[l + 1];
+        std::vector<float> cnt(19 * (size_t)kBV * c.n_blocks, 0.f);
+        std::vector<std::pair<uint64_t, uint64_t>> tl;  // (target element, source element)
+        for (int z = 0; z < f.dim[2]; z++)
+            for (int y = 0; y < f.dim[1]; y++)
+                for (int x = 0; x < f.dim[0]; x++) {
+                    if (!f.active(x, y, z) || !has_parent(*g, l, x, y, z)) continue;
+                    const bool leaf = !(f.st[f.lin(x, y, z)] & kRefined);
+                    const int px = x >> 1, py = y >> 1, pz = z >> 1;
+                    for (int q = 0; q < Q; q++) {
+                        if (q == 9) continue;
+                        const int ux = uoff(x, CX[q]), uy = uoff(y, CY[q]), uz = uoff(z, CZ[q]);
+                        if (f.active(x + ux, y + uy, z + uz)) continue;
+                        if (!c.active(px + ux, py + uy, pz + uz)) continue;
+                        const int sx = ux - CX[q], sy = uy - CY[q], sz = uz - CZ[q];
+                        const int64_t cs = c.slot(px + sx, py + sy, pz + sz);
+                        if (cs < 0) continue;
+                        cnt[Level::elem(cs, q)] += 1.f;
+                        if (!leaf || !f.active(x + sx, y + sy, z + sz)) continue;
+                        tl.emplace_back(Level::elem(cs, q), Level::elem(f.slot(x, y, z), q));
+                    }
                 }
+        for (float& v : cnt)
+            if ((double)v > 0.001) v = g->storage == NX_F64 ? (float)(1.0 / (2.0 * (double)v)) : 1.f / (2.f * v);
+        if (int e = upload((void**)&g->L[l + 1].d_ss, cnt.data(), cnt.size() * 4)) return e;
+        std::stable_sort(tl.begin(), tl.end(),
+                         [](const std::pair<uint64_t, uint64_t>& a, const std::pair<uint64_t, uint64_t>& b) {
+                             return a.first < b.first;
+                         });
+        std::vector<uint64_t> target, src;
+        std::vector<uint32_t> first;
+        for (size_t k = 0; k < tl.size(); k++) {
+            if (k == 0 || tl[k].first != tl[k - 1].first) {
+                target.push_back(tl[k].first);
+                first.push_back((uint32_t)k);
             }
-    for (float& v : cnt)
-        if ((double)v > 0.001) v = g->storage == NX_F64 ? (float)(1.0 / (2.0 * (double)v)) : 1.f / (2.f * v);
-    if (int e = upload((void**)&g->d_ss, cnt.data(), cnt.size() * 4)) return e;
-    std::stable_sort(tl.begin(), tl.end(),
-                     [](const std::pair<uint64_t, uint64_t>& a, const std::pair<uint64_t, uint64_t>& b) {
-                         return a.first < b.first;
-                     });
-    std::vector<uint64_t> target, src;
-    std::vector<uint32_t> first;
-    for (size_t k = 0; k < tl.size(); k++) {
-        if (k == 0 || tl[k].first != tl[k - 1].first) {
-            target.push_back(tl[k].first);
-            first.push_back((uint32_t)k);
+            src.push_back(tl[k].second);
         }
-        src.push_back(tl[k].second);
+        first.push_back((uint32_t)tl.size());
+        Store& sl = g->store[l];
+        sl.n = (uint32_t)target.size();
+        sl.n_src = src.size();
+        if (int e = upload((void**)&sl.d_target, target.data(), target.size() * 8)) return e;
+        if (int e = upload((void**)&sl.d_first, first.data(), first.size() * 4)) return e;
+        if (int e = upload((void**)&sl.d_src, src.data(), src.size() * 8)) return e;
     }
-    first.push_back((uint32_t)tl.size());
-    g->n_store = (uint32_t)target.size();
-    g->n_store_src = src.size();
-    if (int e = upload((void**)&g->d_target, target.data(), target.size() * 8)) return e;
-    if (int e = upload((void**)&g->d_first, first.data(), first.size() * 4)) return e;
-    if (int e = upload((void**)&g->d_src, src.data(), src.size() * 8)) return e;
     g->ready = true;
     return NX_OK;
 }
@@ -587,59 +694,86 @@ extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
 namespace {
 
 template <int S>
-int launch_step(nx_mgrid_s& g, double omega0, hipStream_t st)
-{
+struct Stepper {
     using T = typename MS<S>::T;
-    Level& f = g.L[0];
-    Level& c = g.L[1];
-    // computeOmega (util.h:40-45): omega0 is the coarse level's
-    const double om1 = 2 * omega0 / (std::scalbln(1.0, 1) + (1. - std::scalbln(1.0, 0)) * omega0);
-    const double om0 = 2 * omega0 / (std::scalbln(1.0, 2) + (1. - std::scalbln(1.0, 1)) * omega0);
-    const UpLevel none{nullptr, nullptr, {0, 0, 0}};
-    const UpLevel up{c.fout, c.d_table, {c.nb[0], c.nb[1], c.nb[2]}};
-    auto store = [&](const void* fout0) {
-        if (g.n_store)
-            hipLaunchKernelGGL((k_mr_store<S>), dim3((g.n_store + 255) / 256), dim3(256), 0, st, (const T*)fout0,
-                               (T*)c.fout, g.d_target, g.d_first, g.d_src, g.n_store);
-    };
-    if (c.n_blocks)
-        hipLaunchKernelGGL((k_mr_collide<S>), dim3(c.n_blocks), dim3(kBV), 0, st, (const T*)c.fin, (T*)c.fout,
-                           c.d_state, om1);
-    if (f.n_blocks && !g.fuse_fine) {
-        for (int sub = 0; sub < 2; sub++) {
-            hipLaunchKernelGGL((k_mr_collide<S>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fin,
-                               (T*)f.fout, f.d_state, om0);
-            store(f.fout);
-            hipLaunchKernelGGL((k_mr_stream<S, 0, false, true>), dim3(f.n_blocks), dim3(kBV), 0, st,
-                               (const T*)f.fout, (T*)f.fin, f.d_state, f.d_links, f.d_conn, f.d_origins,
-                               (const float*)nullptr, up, 0.0);
-        }
-    } else if (f.n_blocks) {
-        // level 0, two sub-steps: collide (fin -> fout), store, then stream
-        // fused with the second collide (fout -> fin buffer, which now holds
-        // post-collision values), store, stream (-> the old fout buffer);
-        // afterwards the two buffers have traded roles.  Only when no bulk
-        // leaf of level 0 has a link that keeps its previous value (state
-        // "none"): that value is gone once the fused pass has written.
-        hipLaunchKernelGGL((k_mr_collide<S>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fin, (T*)f.fout,
-                           f.d_state, om0);
-        store(f.fout);
-        hipLaunchKernelGGL((k_mr_stream<S, 0, true, false>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fout,
-                           (T*)f.fin, f.d_state, f.d_links, f.d_conn, f.d_origins, (const float*)nullptr, up, om0);
-        store(f.fin);
-        hipLaunchKernelGGL((k_mr_stream<S, 0, false, false>), dim3(f.n_blocks), dim3(kBV), 0, st, (const T*)f.fin,
-                           (T*)f.fout, f.d_state, f.d_links, f.d_conn, f.d_origins, (const float*)nullptr, up, 0.0);
-        std::swap(f.fin, f.fout);
+    nx_mgrid_s& g;
+    double omega0;
+    hipStream_t st;
+
+    // computeOmega (util.h:40-45): omega0 is the coarsest level's
+    double omega(int l) const
+    {
+        const int il = g.depth - l - 1;
+        return 2 * omega0 / (std::scalbln(1.0, il + 1) + (1. - std::scalbln(1.0, il)) * omega0);
     }
-    if (c.n_blocks && g.keep_coarse)
-        hipLaunchKernelGGL((k_mr_stream<S, 1, false, true>), dim3(c.n_blocks), dim3(kBV), 0, st, (const T*)c.fout,
-                           (T*)c.fin, c.d_state, c.d_links, c.d_conn, c.d_origins, (const float*)g.d_ss, none, 0.0);
-    else if (c.n_blocks)
-        hipLaunchKernelGGL((k_mr_stream<S, 1, false, false>), dim3(c.n_blocks), dim3(kBV), 0, st, (const T*)c.fout,
-                           (T*)c.fin, c.d_state, c.d_links, c.d_conn, c.d_origins, (const float*)g.d_ss, none, 0.0);
-    NX_CHECK_LAUNCH("mgrid step");
-    return NX_OK;
-}
+    UpLevel up(int l) const
+    {
+        if (l + 1 >= g.depth) return UpLevel{nullptr, nullptr, {0, 0, 0}};
+        const Level& c = g.L[l + 1];
+        return UpLevel{c.fout, c.d_table, {c.nb[0], c.nb[1], c.nb[2]}};
+    }
+    void collide(int l) const
+    {
+        const Level& lv = g.L[l];
+        if (lv.n_blocks)
+            hipLaunchKernelGGL((k_mr_collide<S>), dim3(lv.n_blocks), dim3(kBV), 0, st, (const T*)lv.fin,
+                               (T*)lv.fout, lv.d_state, omega(l));
+    }
+    void store(int l, const void* fine) const
+    {
+        const Store& sl = g.store[l];
+        if (l + 1 < g.depth && sl.n)
+            hipLaunchKernelGGL((k_mr_store<S>), dim3((sl.n + 255) / 256), dim3(256), 0, st, (const T*)fine,
+                               (T*)g.L[l + 1].fout, sl.d_target, sl.d_first, sl.d_src, sl.n);
+    }
+    template <bool COLLIDE, bool KEEP>
+    void stream(int l, const void* src, void* dst) const
+    {
+        const Level& lv = g.L[l];
+        if (lv.n_blocks)
+            hipLaunchKernelGGL((k_mr_stream<S, COLLIDE, KEEP>), dim3(lv.n_blocks), dim3(kBV), 0, st, (const T*)src,
+                               (T*)dst, lv.d_state, lv.d_links, lv.d_conn, lv.d_origins, (const float*)lv.d_ss,
+                               up(l), COLLIDE ? omega(l) : 0.0);
+    }
+    void stream(int l) const
+    {
+        const Level& lv = g.L[l];
+        if (lv.keep)
+            stream<false, true>(l, lv.fout, lv.fin);
+        else
+            stream<false, false>(l, lv.fout, lv.fin);
+    }
+    // nonUniformTimestepRecursive (lbmMultiRes.h:141-263)
+    void level(int l) const
+    {
+        Level& lv = g.L[l];
+        if (l == 0 && g.fuse_fine) {
+            // level 0, two sub-steps: collide (fin -> fout), store, then
+            // stream fused with the second collide (fout -> fin buffer, which
+            // now holds post-collision values), store, stream (-> the old
+            // fout buffer); afterwards the two buffers have traded roles.
+            // Only when no bulk leaf of level 0 has a link that keeps its
+            // previous value (state "none"): that value is gone once the
+            // fused pass has written.
+            collide(0);
+            store(0, lv.fout);
+            stream<true, false>(0, lv.fout, lv.fin);
+            store(0, lv.fin);
+            stream<false, false>(0, lv.fin, lv.fout);
+            std::swap(lv.fin, lv.fout);
+            return;
+        }
+        collide(l);
+        store(l, lv.fout);
+        if (l != 0) level(l - 1);
+        stream(l);
+        if (l == g.depth - 1) return;
+        collide(l);
+        store(l, lv.fout);
+        if (l != 0) level(l - 1);
+        stream(l);
+    }
+};
 
 }  // namespace
 
@@ -649,17 +783,19 @@ extern "C" int nx_mgrid_step(nx_mgrid g, double omega0, int32_t n_steps, nx_stre
     if (!g->ready) return set_error(NX_ERR_NOT_INIT, "nx_mgrid_step: call nx_mgrid_cavity first");
     if (!(omega0 > 0.0 && omega0 < 2.0)) return set_error(NX_ERR_INVALID, "nx_mgrid_step: omega0 in (0, 2)");
     for (int k = 0; k < n_steps; k++) {
-        const int e = g->storage == NX_F64 ? launch_step<NX_F64>(*g, omega0, (hipStream_t)s)
-                                           : launch_step<NX_F32>(*g, omega0, (hipStream_t)s);
-        if (e) return e;
+        if (g->storage == NX_F64)
+            Stepper<NX_F64>{*g, omega0, (hipStream_t)s}.level(g->depth - 1);
+        else
+            Stepper<NX_F32>{*g, omega0, (hipStream_t)s}.level(g->depth - 1);
+        NX_CHECK_LAUNCH("mgrid step");
     }
     return NX_OK;
 }
 
 extern "C" int nx_mgrid_fields(nx_mgrid g, int32_t level, int32_t which, void* dense)
 {
-    if (!g || (level != 0 && level != 1) || (which != 0 && which != 1) || !dense)
-        return set_error(NX_ERR_INVALID, "nx_mgrid_fields: bad arguments");
+    if (int e = check_level(g, level)) return e;
+    if ((which != 0 && which != 1) || !dense) return set_error(NX_ERR_INVALID, "nx_mgrid_fields: bad arguments");
     if (!g->ready) return set_error(NX_ERR_NOT_INIT, "nx_mgrid_fields: call nx_mgrid_cavity first");
     const Level& lv = g->L[level];
     const size_t es = g->storage == NX_F64 ? 8 : 4, n = lv.nvox();
@@ -684,10 +820,11 @@ extern "C" int nx_mgrid_destroy(nx_mgrid g)
     if (!g) return NX_OK;
     for (auto& lv : g->L)
         for (void* p : {(void*)lv.d_state, (void*)lv.d_links, (void*)lv.d_conn, (void*)lv.d_origins,
-                        (void*)lv.d_table, lv.fin, lv.fout})
+                        (void*)lv.d_table, (void*)lv.d_ss, lv.fin, lv.fout})
             if (p) hipFree(p);
-    for (void* p : {(void*)g->d_ss, (void*)g->d_target, (void*)g->d_first, (void*)g->d_src})
-        if (p) hipFree(p);
+    for (auto& sl : g->store)
+        for (void* p : {(void*)sl.d_target, (void*)sl.d_first, (void*)sl.d_src})
+            if (p) hipFree(p);
     delete g;
     return NX_OK;
 }

@@ -1,6 +1,7 @@
-"""Two-level grid refinement (Neon's mGrid with depth 2) driven through the
-C ABI (nx_mgrid_*): the level activation of the reference's lid-driven cavity
-(apps/lbmMultiRes/lidDrivenCavity.h:88-271) and the solver handle."""
+"""Grid refinement (Neon's mGrid, depth 2 to 5) driven through the C ABI
+(nx_mgrid_*): the level activation of the reference's lid-driven cavity
+(apps/lbmMultiRes/lidDrivenCavity.h:88-271), its --scale configurations, and
+the solver handle."""
 from __future__ import annotations
 
 import ctypes as C
@@ -24,12 +25,22 @@ def sdf_cube(ids: np.ndarray, dim) -> np.ndarray:
     return (np.minimum(d.max(-1), np.float32(0.0)) + length).astype(np.float32)
 
 
+def level_dims(dim, depth: int):
+    """Voxels per axis of every level: ceil(dim / 2^l) (a level-l voxel v is
+    in the domain when its level-0 origin v 2^l is)."""
+    out = [tuple(int(v) for v in dim)]
+    for _ in range(1, depth):
+        out.append(tuple((v + 1) // 2 for v in out[-1]))
+    return out
+
+
 def cavity_lambdas(dim, level_sdf=(0.0, -1.0 / 3.0, -1.0)):
-    """Activation of the two levels of the refined cavity: level l holds the
+    """Activation of the levels of the refined cavity: level l holds the
     voxels with level_sdf[l+1] < sdfCube <= level_sdf[l] (lidDrivenCavity.h:
-    229-236; sdfCube of the level-0 position against dim - 1).  Returns
-    (lam0 at every level-0 voxel, lam1 at every level-1 voxel evaluated at
-    its level-0 origin 2v), uint8, x fastest."""
+    229-236; sdfCube of the level-0 position against dim - 1), one level per
+    interval of level_sdf.  Returns a tuple with, for every level l, the
+    lambda at every level-l voxel evaluated at its level-0 origin v 2^l,
+    uint8, x fastest."""
     X, Y, Z = (int(v) for v in dim)
     box = np.array([X - 1, Y - 1, Z - 1])
     lv = [np.float32(v) for v in level_sdf]
@@ -40,39 +51,70 @@ def cavity_lambdas(dim, level_sdf=(0.0, -1.0 / 3.0, -1.0)):
         s = sdf_cube(ids, box)
         return ((s <= lo) & (s > hi)).astype(np.uint8).reshape(-1)
 
-    d1 = ((X + 1) // 2, (Y + 1) // 2, (Z + 1) // 2)
-    return lam((X, Y, Z), 1, lv[0], lv[1]), lam(d1, 2, lv[1], lv[2])
+    depth = len(lv) - 1
+    return tuple(lam(d, 1 << l, lv[l], lv[l + 1]) for l, d in enumerate(level_dims(dim, depth)))
 
 
-def cavity_parameters(dim, re: float, storage: str = "f64", ulb: float = 0.04):
+# The lid-driven cavity configurations of the multi-resolution app
+# (apps/lbmMultiRes/lidDrivenCavity.h:95-214): --scale -> (grid edge,
+# levelSDF); the depth is len(levelSDF) - 1.
+LID_SCALES = {
+    -1: (192, (0.0, -36.0 / 96.0, -72.0 / 96.0, -1.0)),
+    0: (48, (0.0, -8.0 / 24.0, -16.0 / 24.0, -1.0)),
+    1: (160, (0.0, -16.0 / 80.0, -32.0 / 80.0, -1.0)),
+    2: (240, (0.0, -24.0 / 120.0, -80.0 / 120.0, -1.0)),
+    3: (320, (0.0, -32.0 / 160.0, -64.0 / 160.0, -1.0)),
+    4: (480, (0.0, -48.0 / 240.0, -96.0 / 240.0, -1.0)),
+    5: (512, (0.0, -64.0 / 256.0, -112.0 / 256.0, -1.0)),
+    6: (160, (0.0, -32.0 / 160.0, -64.0 / 160.0, -128.0 / 160.0, -1.0)),
+    7: (240, (0.0, -32.0 / 120.0, -80.0 / 120.0, -112.0 / 120.0, -1.0)),
+    8: (320, (0.0, -32.0 / 160.0, -64.0 / 160.0, -112.0 / 160.0, -1.0)),
+    9: (480, (0.0, -48.0 / 240.0, -96.0 / 240.0, -160.0 / 240.0, -1.0)),
+    10: (512, (0.0, -120.0 / 512.0, -200.0 / 512.0, -400.0 / 512.0, -1.0)),
+    11: (512, (0.0, -120.0 / 512.0, -200.0 / 512.0, -400.0 / 512.0, -1.0)),
+}
+
+
+def cavity_parameters(dim, re: float, storage: str = "f64", ulb: float = 0.04, depth: int = 2):
     """(ulb, omega0) of the refined cavity in the field type T
     (lidDrivenCavity.h:246-251): ulb = 0.04 in T, the characteristic length is
-    the top level's dimension (its groups times 2), nu = ulb * length / Re in
-    T, omega0 = 1 / (3 nu + 1/2) evaluated in binary64 and stored in T."""
+    the top level's dimension, grid.getDimension(depth - 1).x = its groups of
+    2 times 2 = 2 ceil(X / 2^depth) (mGrid.cpp:684-687), nu = ulb * length /
+    Re in T, omega0 = 1 / (3 nu + 1/2) evaluated in binary64 and stored in T."""
     T = np.float64 if storage == "f64" else np.float32
     u = T(ulb)
-    length = T(2 * ((int(dim[0]) + 3) // 4))
+    length = T(2 * ((int(dim[0]) + (1 << depth) - 1) >> depth))
     nu = T(u * length / T(re))
     omega = T(1.0 / (3.0 * float(nu) + 0.5))
     return float(u), float(omega)
 
 
 class MGrid:
-    """Two-level refined grid and its LBM fields on the device (nx_mgrid_*).
-    lam0 / lam1: activation of level 0 (every level-0 voxel) and level 1
-    (every level-1 voxel at its level-0 origin), uint8, x fastest."""
+    """Refined grid and its LBM fields on the device (nx_mgrid_*).
+    MGrid(dim, [lam_0, .., lam_{D-1}], storage) or, for two levels,
+    MGrid(dim, lam0, lam1, storage): lam_l is the activation of level l at
+    every level-l voxel (evaluated at its level-0 origin v 2^l), uint8, x
+    fastest."""
 
-    def __init__(self, dim, lam0, lam1, storage="f64"):
+    def __init__(self, dim, lams, *rest):
         from .lattice import init, _storage
+        if isinstance(lams, (list, tuple)):
+            storage = rest[0] if rest else "f64"
+        else:  # (dim, lam0, lam1[, storage])
+            lams = [lams, rest[0]]
+            storage = rest[1] if len(rest) > 1 else "f64"
         self.dim = tuple(int(v) for v in dim)
-        self.dim1 = tuple((v + 1) // 2 for v in self.dim)
+        self.depth = len(lams)
+        self.dims = level_dims(self.dim, self.depth)
+        self.dim1 = self.dims[1]
         self.storage = _storage(storage)
-        lam0 = np.ascontiguousarray(lam0, np.uint8).reshape(-1)
-        lam1 = np.ascontiguousarray(lam1, np.uint8).reshape(-1)
-        assert lam0.size == int(np.prod(self.dim)) and lam1.size == int(np.prod(self.dim1))
+        lams = [np.ascontiguousarray(a, np.uint8).reshape(-1) for a in lams]
+        for a, d in zip(lams, self.dims):
+            if a.size != int(np.prod(d)):
+                raise ValueError(f"activation array of {a.size} voxels for a level of {d}")
         h = C.c_void_p()
-        check(lib.nx_mgrid_create((C.c_int32 * 3)(*self.dim), lam0.ctypes.data, lam1.ctypes.data,
-                                  self.storage, C.byref(h)))
+        ptrs = (C.c_void_p * self.depth)(*[a.ctypes.data for a in lams])
+        check(lib.nx_mgrid_create_levels((C.c_int32 * 3)(*self.dim), self.depth, ptrs, self.storage, C.byref(h)))
         self.h = h.value
         self._init = init
 
@@ -82,17 +124,19 @@ class MGrid:
         except Exception:
             pass
 
-    def state(self):
-        """(state bytes of level 0, of level 1): bit 0 active, bit 1 refined,
-        bits 2..3 the cell class."""
-        s0 = np.zeros(int(np.prod(self.dim)), np.uint8)
-        s1 = np.zeros(int(np.prod(self.dim1)), np.uint8)
-        check(lib.nx_mgrid_state(self.h, s0.ctypes.data, s1.ctypes.data))
-        return s0, s1
+    def state(self, level=None):
+        """State bytes of one level, or of every level when level is None:
+        bit 0 active, bit 1 refined, bits 2..3 the cell class."""
+        if level is None:
+            return tuple(self.state(l) for l in range(self.depth))
+        st = np.zeros(int(np.prod(self.dims[level])), np.uint8)
+        check(lib.nx_mgrid_state_level(self.h, int(level), st.ctypes.data))
+        return st
 
     def info(self) -> dict:
-        nb = (C.c_uint32 * 2)()
-        na, nl = (C.c_uint64 * 2)(), (C.c_uint64 * 2)()
+        D = self.depth
+        nb = (C.c_uint32 * D)()
+        na, nl = (C.c_uint64 * D)(), (C.c_uint64 * D)()
         ns, fused = C.c_uint64(), C.c_int32()
         check(lib.nx_mgrid_info(self.h, nb, na, nl, C.byref(ns), C.byref(fused)))
         return {"blocks": list(nb), "active": list(na), "leaf": list(nl), "store_links": int(ns.value),
@@ -107,7 +151,7 @@ class MGrid:
 
     def fields(self, level: int, which: int = 0) -> np.ndarray:
         """fin (which 0) or fout (which 1) of one level, dense [19 * n]."""
-        n = int(np.prod(self.dim1 if level else self.dim))
+        n = int(np.prod(self.dims[level]))
         dt = np.float64 if self.storage == abi.NX_F64 else np.float32
         out = np.zeros(19 * n, dt)
         check(lib.nx_mgrid_fields(self.h, int(level), int(which), out.ctypes.data))

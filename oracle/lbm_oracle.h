@@ -98,14 +98,15 @@ double or_mass_block(const or_block* d, const void* fin, const uint32_t* flags);
 void or_run_block(const or_block* d, void* pop0, void* pop1, const uint32_t* flags,
                   double omega, const double g[3], int nsteps);
 
-/* two-level grid refinement (mres_oracle.c) */
-void or_mres_dims(const int32_t dim[3], int32_t dim1[3]);
-void or_mres_build(const int32_t dim[3], const uint8_t* lam0, const uint8_t* lam1, uint8_t* st0,
-                   uint8_t* st1);
-void or_mres_setup(const int32_t dim[3], int storage, double ulb, uint8_t* st0, uint8_t* st1,
-                   void* fin0, void* fout0, void* fin1, void* fout1, float* ss1);
+/* grid refinement over `depth` levels (mres_oracle.c); per-level arrays are
+ * indexed by level, dims[3 * l + axis] */
+#define OR_MRES_MAX_DEPTH 5
+void or_mres_dims(const int32_t dim[3], int depth, int32_t* dims);
+int or_mres_build(const int32_t dim[3], int depth, const uint8_t* const* lam, uint8_t* const* st);
+void or_mres_setup(const int32_t dim[3], int depth, int storage, double ulb, uint8_t* const* st,
+                   void* const* fin, void* const* fout, float* const* ss);
 double or_mres_omega(double omega0, int level, int num_levels);
-void or_mres_step(const int32_t dim[3], int s, const uint8_t* st0, const uint8_t* st1, void* fin0,
-                  void* fout0, void* fin1, void* fout1, const float* ss1, double omega0);
+void or_mres_step(const int32_t dim[3], int depth, int s, const uint8_t* const* st, void* const* fin,
+                  void* const* fout, const float* const* ss, double omega0);
 
 #endif

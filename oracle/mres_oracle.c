@@ -1,23 +1,25 @@
-/* mres_oracle.c -- CPU restatement of Neon's two-level grid-refinement LBM
- * (apps/lbmMultiRes, BGK D3Q19 variant) on a stack of two levels.
+/* mres_oracle.c -- CPU restatement of Neon's grid-refinement LBM
+ * (apps/lbmMultiRes, BGK D3Q19 variant) on a stack of `depth` levels
+ * (2 <= depth <= OR_MRES_MAX_DEPTH).
  *
  * TEST INFRASTRUCTURE ONLY: the checker for the library's nx_mgrid_* path.
  * Nothing in the product links or calls this file.
  *
- * Level 0 is the finest level (the domain dim[3]), level 1 has half the
- * resolution (a level-1 voxel v covers the level-0 voxels 2v + {0,1}^3).
- * Storage is dense per level, SoA [q][z][y][x], one byte of state per voxel:
- * active, refined (has children), class.  Every function follows the
- * reference operation by operation, in the reference's loop order where the
- * result depends on it (the fine-to-coarse accumulation).
+ * Level 0 is the finest level (the domain dim[3]); level l has voxels of
+ * edge 2^l and ceil(dim / 2^l) of them per axis; a level-(l+1) voxel v
+ * covers the level-l voxels 2v + {0,1}^3.  Storage is dense per level, SoA
+ * [q][z][y][x], one byte of state per voxel: active, refined (has
+ * children), class.  Every function follows the reference operation by
+ * operation, in the reference's loop order where the result depends on it
+ * (the strong-balance pass, the fine-to-coarse accumulation).
  *
  * Reference files restated:
- *   grid:     libNeonDomain/src/domain/details/mGrid/mGrid.cpp:89-264
- *             (activation per 2^3 group, parent marking, overlap culling;
- *             the strong-balance pass, :266-345, cannot change a two-level
- *             grid) and :466-555 (a voxel has children iff one of its eight
- *             children is active)
- *   neighbours: mPartition_imp.h:186-235 (parent, uncle),
+ *   grid:     libNeonDomain/src/domain/details/mGrid/mGrid.cpp:89-156
+ *             (activation per 2^3 group, parent marking), :159-263 (overlap
+ *             culling, coarsest level first), :266-345 (strong balance) and
+ *             :466-555 (a voxel has children iff one of its eight children
+ *             is active in the final masks)
+ *   neighbours: mPartition_imp.h:155-235 (children, parent, uncle),
  *             apps/lbmMultiRes/util.h:19-37 (uncleOffset),
  *             util.h:40-45 (computeOmega)
  *   set-up:   apps/lbmMultiRes/lidDrivenCavity.h:8-78 (cavity),
@@ -56,96 +58,149 @@ static inline int inside(const int32_t d[3], int x, int y, int z)
 {
     return x >= 0 && y >= 0 && z >= 0 && x < d[0] && y < d[1] && z < d[2];
 }
+static inline size_t nvox(const int32_t d[3]) { return (size_t)d[0] * d[1] * d[2]; }
 
-void or_mres_dims(const int32_t dim[3], int32_t dim1[3])
+/* level l: ceil(dim / 2^l) voxels per axis (a voxel is in the domain when
+ * its level-0 origin v 2^l is, mGrid.cpp:105-108) */
+void or_mres_dims(const int32_t dim[3], int depth, int32_t* dims)
 {
-    for (int a = 0; a < 3; a++) dim1[a] = (dim[a] + 1) / 2;
+    for (int a = 0; a < 3; a++) dims[a] = dim[a];
+    for (int l = 1; l < depth; l++)
+        for (int a = 0; a < 3; a++) dims[3 * l + a] = (dims[3 * (l - 1) + a] + 1) / 2;
 }
 
-/* mGrid constructor for depth 2 (mGrid.cpp:89-264).  lam0: the level-0
- * activation lambda at every level-0 voxel; lam1: the level-1 lambda at
- * every level-1 voxel (evaluated by the caller at the voxel's level-0
- * origin 2v).  Outputs the state byte of every voxel of both levels. */
-void or_mres_build(const int32_t dim[3], const uint8_t* lam0, const uint8_t* lam1, uint8_t* st0,
-                   uint8_t* st1)
+static inline int act(const int32_t d[3], const uint8_t* st, int x, int y, int z)
 {
-    int32_t d1[3];
-    or_mres_dims(dim, d1);
-    const size_t n0 = (size_t)dim[0] * dim[1] * dim[2], n1 = (size_t)d1[0] * d1[1] * d1[2];
-    memset(st0, 0, n0);
-    memset(st1, 0, n1);
-    /* level 0: groups of 2^3 voxels (one level-1 voxel each, :89-156): a
-     * group with an active voxel activates all of its voxels inside the
-     * domain and its parent */
-    for (int gz = 0; gz < d1[2]; gz++)
-        for (int gy = 0; gy < d1[1]; gy++)
-            for (int gx = 0; gx < d1[0]; gx++) {
-                int any = 0;
-                for (int k = 0; k < 8; k++) {
-                    const int x = 2 * gx + (k & 1), y = 2 * gy + ((k >> 1) & 1), z = 2 * gz + (k >> 2);
-                    if (inside(dim, x, y, z) && (st0[lin(dim, x, y, z)] || lam0[lin(dim, x, y, z)])) any = 1;
+    return inside(d, x, y, z) && (st[lin(d, x, y, z)] & MR_ACTIVE);
+}
+
+/* isRefined (mGrid.cpp:167-199): one of the eight children of level-l voxel
+ * (x, y, z) is active on level l - 1 (the voxel itself need not be) */
+static int has_child(const int32_t dc[3], const uint8_t* stc, int x, int y, int z)
+{
+    for (int k = 0; k < 8; k++)
+        if (act(dc, stc, 2 * x + (k & 1), 2 * y + ((k >> 1) & 1), 2 * z + (k >> 2))) return 1;
+    return 0;
+}
+
+/* mGrid constructor (mGrid.cpp:89-345, :466-555).  lam[l]: the level-l
+ * activation lambda at every level-l voxel (evaluated by the caller at the
+ * voxel's level-0 origin v 2^l).  Outputs the state byte of every voxel of
+ * every level; returns the number of voxels the strong-balance pass added. */
+int or_mres_build(const int32_t dim[3], int depth, const uint8_t* const* lam, uint8_t* const* st)
+{
+    int32_t dims[3 * OR_MRES_MAX_DEPTH];
+    or_mres_dims(dim, depth, dims);
+    for (int l = 0; l < depth; l++) memset(st[l], 0, nvox(dims + 3 * l));
+    /* activation (:89-156): level by level from the finest, groups of 2^3
+     * voxels; a group with an active voxel (set earlier by a finer level or
+     * by its lambda) activates all of its voxels inside the domain and its
+     * parent */
+    for (int l = 0; l < depth; l++) {
+        const int32_t* d = dims + 3 * l;
+        const int32_t g[3] = {(d[0] + 1) / 2, (d[1] + 1) / 2, (d[2] + 1) / 2};
+        for (int gz = 0; gz < g[2]; gz++)
+            for (int gy = 0; gy < g[1]; gy++)
+                for (int gx = 0; gx < g[0]; gx++) {
+                    int any = 0;
+                    for (int k = 0; k < 8; k++) {
+                        const int x = 2 * gx + (k & 1), y = 2 * gy + ((k >> 1) & 1), z = 2 * gz + (k >> 2);
+                        if (inside(d, x, y, z) && (st[l][lin(d, x, y, z)] || lam[l][lin(d, x, y, z)])) any = 1;
+                    }
+                    if (!any) continue;
+                    for (int k = 0; k < 8; k++) {
+                        const int x = 2 * gx + (k & 1), y = 2 * gy + ((k >> 1) & 1), z = 2 * gz + (k >> 2);
+                        if (inside(d, x, y, z)) st[l][lin(d, x, y, z)] = MR_ACTIVE;
+                    }
+                    if (l < depth - 1) st[l + 1][lin(dims + 3 * (l + 1), gx, gy, gz)] = MR_ACTIVE;
                 }
-                if (!any) continue;
-                for (int k = 0; k < 8; k++) {
-                    const int x = 2 * gx + (k & 1), y = 2 * gy + ((k >> 1) & 1), z = 2 * gz + (k >> 2);
-                    if (inside(dim, x, y, z)) st0[lin(dim, x, y, z)] = MR_ACTIVE;
+    }
+    /* overlap culling (:159-263), coarsest level first: an active voxel with
+     * an active child whose in-domain neighbours all have an active child
+     * is removed (level l - 1 is still unculled when level l is visited) */
+    for (int l = depth - 1; l > 0; l--) {
+        const int32_t* d = dims + 3 * l;
+        const int32_t* dc = dims + 3 * (l - 1);
+        const size_t n = nvox(d);
+        uint8_t* refined = (uint8_t*)calloc(n, 1);
+        for (int z = 0; z < d[2]; z++)
+            for (int y = 0; y < d[1]; y++)
+                for (int x = 0; x < d[0]; x++) refined[lin(d, x, y, z)] = (uint8_t)has_child(dc, st[l - 1], x, y, z);
+        for (int z = 0; z < d[2]; z++)
+            for (int y = 0; y < d[1]; y++)
+                for (int x = 0; x < d[0]; x++) {
+                    const size_t i = lin(d, x, y, z);
+                    if (!(st[l][i] & MR_ACTIVE) || !refined[i]) continue;
+                    int deactivate = 1;
+                    for (int k = -1; k <= 1; k++)
+                        for (int j = -1; j <= 1; j++)
+                            for (int h = -1; h <= 1; h++) {
+                                if (!h && !j && !k) continue;
+                                if (inside(d, x + h, y + j, z + k) && !refined[lin(d, x + h, y + j, z + k)])
+                                    deactivate = 0;
+                            }
+                    if (deactivate) st[l][i] = 0;
                 }
-                st1[lin(d1, gx, gy, gz)] = MR_ACTIVE;
-            }
-    /* level 1 (the top level): groups of 2^3 level-1 voxels; a voxel is in
-     * the domain when its level-0 origin 2v is */
-    const int32_t d2[3] = {(d1[0] + 1) / 2, (d1[1] + 1) / 2, (d1[2] + 1) / 2};
-    for (int gz = 0; gz < d2[2]; gz++)
-        for (int gy = 0; gy < d2[1]; gy++)
-            for (int gx = 0; gx < d2[0]; gx++) {
-                int any = 0;
-                for (int k = 0; k < 8; k++) {
-                    const int x = 2 * gx + (k & 1), y = 2 * gy + ((k >> 1) & 1), z = 2 * gz + (k >> 2);
-                    if (inside(d1, x, y, z) && (st1[lin(d1, x, y, z)] || lam1[lin(d1, x, y, z)])) any = 1;
-                }
-                if (!any) continue;
-                for (int k = 0; k < 8; k++) {
-                    const int x = 2 * gx + (k & 1), y = 2 * gy + ((k >> 1) & 1), z = 2 * gz + (k >> 2);
-                    if (inside(d1, x, y, z)) st1[lin(d1, x, y, z)] = MR_ACTIVE;
-                }
-            }
-    /* a level-1 voxel is refined when one of its children is active
-     * (mGrid.cpp:176-205; :497-545 for the child link) */
-    uint8_t* refined = (uint8_t*)calloc(n1, 1);
-    for (int z = 0; z < d1[2]; z++)
-        for (int y = 0; y < d1[1]; y++)
-            for (int x = 0; x < d1[0]; x++) {
-                int r = 0;
-                for (int k = 0; k < 8; k++) {
-                    const int cx = 2 * x + (k & 1), cy = 2 * y + ((k >> 1) & 1), cz = 2 * z + (k >> 2);
-                    if (inside(dim, cx, cy, cz) && (st0[lin(dim, cx, cy, cz)] & MR_ACTIVE)) r = 1;
-                }
-                refined[lin(d1, x, y, z)] = (uint8_t)r;
-            }
-    /* overlap culling (:209-263): an active refined voxel whose 26
-     * neighbours inside the domain are all refined is removed */
-    for (int z = 0; z < d1[2]; z++)
-        for (int y = 0; y < d1[1]; y++)
-            for (int x = 0; x < d1[0]; x++) {
-                const size_t i = lin(d1, x, y, z);
-                if (!(st1[i] & MR_ACTIVE) || !refined[i]) continue;
-                int deactivate = 1;
-                for (int k = -1; k <= 1; k++)
-                    for (int j = -1; j <= 1; j++)
-                        for (int l = -1; l <= 1; l++) {
-                            if (!l && !j && !k) continue;
-                            const int nx = x + l, ny = y + j, nz = z + k;
-                            /* inside test on the level-0 origin of the neighbour */
-                            if (2 * nx < 0 || 2 * ny < 0 || 2 * nz < 0 || 2 * nx >= dim[0] ||
-                                2 * ny >= dim[1] || 2 * nz >= dim[2])
-                                continue;
-                            if (!refined[lin(d1, nx, ny, nz)]) deactivate = 0;
+        free(refined);
+    }
+    /* strong balance (:266-345): for every active voxel of level l and each
+     * of its 26 neighbour positions inside the domain, look for the finest
+     * active voxel covering the position from level l upwards; found on
+     * level l or l + 1 it is fine, found on level l_n >= l + 2 the voxel of
+     * level l_n - 1 covering the position is activated (on its own), and the
+     * search goes on upwards.  Repeated until a pass activates nothing; the
+     * passes see their own activations in loop order. */
+    int added = 0, again = 1;
+    while (again) {
+        again = 0;
+        for (int l = 0; l < depth; l++) {
+            const int32_t* d = dims + 3 * l;
+            const int32_t g[3] = {(d[0] + 1) / 2, (d[1] + 1) / 2, (d[2] + 1) / 2};
+            for (int bz = 0; bz < g[2]; bz++)
+                for (int by = 0; by < g[1]; by++)
+                    for (int bx = 0; bx < g[0]; bx++)
+                        for (int k8 = 0; k8 < 8; k8++) {
+                            const int x = 2 * bx + (k8 & 1), y = 2 * by + ((k8 >> 1) & 1), z = 2 * bz + (k8 >> 2);
+                            if (!act(d, st[l], x, y, z)) continue;
+                            for (int k = -1; k <= 1; k++)
+                                for (int j = -1; j <= 1; j++)
+                                    for (int h = -1; h <= 1; h++) {
+                                        if (!h && !j && !k) continue;
+                                        int px = x + h, py = y + j, pz = z + k;
+                                        if (!inside(d, px, py, pz)) continue;
+                                        int qx = 0, qy = 0, qz = 0; /* the previous level's position */
+                                        for (int ln = l; ln < depth; ln++) {
+                                            const int32_t* dn = dims + 3 * ln;
+                                            if (act(dn, st[ln], px, py, pz)) {
+                                                if (ln == l || ln == l + 1) break;
+                                                uint8_t* s = &st[ln - 1][lin(dims + 3 * (ln - 1), qx, qy, qz)];
+                                                if (!(*s & MR_ACTIVE)) added++;
+                                                *s = MR_ACTIVE;
+                                                again = 1;
+                                            }
+                                            qx = px;
+                                            qy = py;
+                                            qz = pz;
+                                            px >>= 1;
+                                            py >>= 1;
+                                            pz >>= 1;
+                                        }
+                                    }
                         }
-                if (deactivate) st1[i] = 0;
-            }
-    for (size_t i = 0; i < n1; i++)
-        if ((st1[i] & MR_ACTIVE) && refined[i]) st1[i] |= MR_REFINED;
-    free(refined);
+        }
+    }
+    /* refined flags from the final masks (:497-545) */
+    for (int l = 1; l < depth; l++) {
+        const int32_t* d = dims + 3 * l;
+        for (int z = 0; z < d[2]; z++)
+            for (int y = 0; y < d[1]; y++)
+                for (int x = 0; x < d[0]; x++) {
+                    const size_t i = lin(d, x, y, z);
+                    if ((st[l][i] & MR_ACTIVE) && has_child(dims + 3 * (l - 1), st[l - 1], x, y, z))
+                        st[l][i] |= MR_REFINED;
+                }
+    }
+    return added;
 }
 
 /* ---- set-up ------------------------------------------------------------- */
@@ -158,27 +213,24 @@ static inline int uoff(int x, int q)
     return s <= 0 ? s : s - 1;
 }
 
-/* level-1 voxel active (getUncle's mask test, mPartition_imp.h:213-226) */
-static inline int act1(const int32_t d1[3], const uint8_t* st1, int x, int y, int z)
+/* the 8^3 memory blocks of a level that hold an active voxel */
+static uint8_t* memory_blocks(const int32_t d[3], const uint8_t* st, int32_t nb[3])
 {
-    return inside(d1, x, y, z) && (st1[lin(d1, x, y, z)] & MR_ACTIVE);
-}
-static inline int act0(const int32_t d0[3], const uint8_t* st0, int x, int y, int z)
-{
-    return inside(d0, x, y, z) && (st0[lin(d0, x, y, z)] & MR_ACTIVE);
+    for (int a = 0; a < 3; a++) nb[a] = (d[a] + 7) / 8;
+    uint8_t* b = (uint8_t*)calloc(nvox(nb), 1);
+    for (int z = 0; z < d[2]; z++)
+        for (int y = 0; y < d[1]; y++)
+            for (int x = 0; x < d[0]; x++)
+                if (st[lin(d, x, y, z)] & MR_ACTIVE) b[lin(nb, x >> 3, y >> 3, z >> 3)] = 1;
+    return b;
 }
 
-/* hasParent (mPartition_imp.h:199-208): the level-1 memory block (8^3
- * level-1 voxels, mPartition.h:13) that holds the parent voxel exists, i.e.
- * has an active voxel -- not that the parent voxel itself is active */
-static int parent_block(const int32_t d1[3], const uint8_t* st1, int px, int py, int pz)
+/* hasParent (mPartition_imp.h:172-208): the memory block (8^3 voxels,
+ * mPartition.h:13) of the next level that holds the parent voxel exists,
+ * i.e. has an active voxel -- not that the parent voxel itself is active */
+static inline int parent_block(const int32_t nbu[3], const uint8_t* blk_up, int px, int py, int pz)
 {
-    const int bx = px >> 3, by = py >> 3, bz = pz >> 3;
-    for (int z = 8 * bz; z < 8 * bz + 8; z++)
-        for (int y = 8 * by; y < 8 * by + 8; y++)
-            for (int x = 8 * bx; x < 8 * bx + 8; x++)
-                if (act1(d1, st1, x, y, z)) return 1;
-    return 0;
+    return inside(nbu, px >> 3, py >> 3, pz >> 3) && blk_up[lin(nbu, px >> 3, py >> 3, pz >> 3)];
 }
 
 static inline void put(void* p, int s, size_t i, double v)
@@ -197,32 +249,29 @@ static inline double get(const void* p, int s, size_t i)
  * with zero populations; leaf voxels of level 0 on the domain faces are
  * bounce-back, those with y = Y-1 the moving lid; leaf voxels hold w_q
  * (bulk), 0 (bounce-back) or -6 w_q (c_q . ulid) (lid), both fields.
- * ulid = (ulb, 0, 0).  sumStore of level 1 after init.h:8-86: every level-0
- * voxel with a parent counts, for each q != 9, one towards the coarse voxel
- * that will later pull its population q, then ss = 1 / (2 count). */
-void or_mres_setup(const int32_t dim[3], int storage, double ulb, uint8_t* st0, uint8_t* st1,
-                   void* fin0, void* fout0, void* fin1, void* fout1, float* ss1)
+ * ulid = (ulb, 0, 0).  sumStore of levels 1.. after init.h:8-86: every
+ * active voxel of level l with a parent counts, for each q != 9, one towards
+ * the level-(l+1) voxel that will later pull its population q, then
+ * ss = 1 / (2 count).  ss[0] is not used. */
+void or_mres_setup(const int32_t dim[3], int depth, int storage, double ulb, uint8_t* const* st,
+                   void* const* fin, void* const* fout, float* const* ss)
 {
-    int32_t d1[3];
-    or_mres_dims(dim, d1);
-    const size_t n0 = (size_t)dim[0] * dim[1] * dim[2], n1 = (size_t)d1[0] * d1[1] * d1[2];
-    for (int lvl = 0; lvl < 2; lvl++) {
-        const int32_t* d = lvl ? d1 : dim;
-        uint8_t* st = lvl ? st1 : st0;
-        void* fi = lvl ? fin1 : fin0;
-        void* fo = lvl ? fout1 : fout0;
-        const size_t n = lvl ? n1 : n0;
+    int32_t dims[3 * OR_MRES_MAX_DEPTH];
+    or_mres_dims(dim, depth, dims);
+    for (int lvl = 0; lvl < depth; lvl++) {
+        const int32_t* d = dims + 3 * lvl;
+        const size_t n = nvox(d);
         for (int z = 0; z < d[2]; z++)
             for (int y = 0; y < d[1]; y++)
                 for (int x = 0; x < d[0]; x++) {
                     const size_t i = lin(d, x, y, z);
-                    if (!(st[i] & MR_ACTIVE)) continue;
+                    if (!(st[lvl][i] & MR_ACTIVE)) continue;
                     int cls = OR_BULK;
                     for (int q = 0; q < 19; q++) {
-                        put(fi, storage, (size_t)q * n + i, 0.);
-                        put(fo, storage, (size_t)q * n + i, 0.);
+                        put(fin[lvl], storage, (size_t)q * n + i, 0.);
+                        put(fout[lvl], storage, (size_t)q * n + i, 0.);
                     }
-                    if (!(st[i] & MR_REFINED)) {
+                    if (!(st[lvl][i] & MR_REFINED)) {
                         if (lvl == 0 && (x == 0 || x == d[0] - 1 || y == 0 || y == d[1] - 1 || z == 0 ||
                                          z == d[2] - 1)) {
                             cls = OR_BOUNCEBACK;
@@ -253,42 +302,52 @@ void or_mres_setup(const int32_t dim[3], int storage, double ulb, uint8_t* st0, 
                                 }
                                 v = (double)p;
                             }
-                            put(fo, storage, (size_t)q * n + i, v);
-                            put(fi, storage, (size_t)q * n + i, v);
+                            put(fout[lvl], storage, (size_t)q * n + i, v);
+                            put(fin[lvl], storage, (size_t)q * n + i, v);
                         }
                     }
-                    st[i] = (uint8_t)((st[i] & (MR_ACTIVE | MR_REFINED)) | (cls << MR_CLS_SHIFT));
+                    st[lvl][i] = (uint8_t)((st[lvl][i] & (MR_ACTIVE | MR_REFINED)) | (cls << MR_CLS_SHIFT));
                 }
     }
-    /* sumStore counts (init.h:8-55) */
-    memset(ss1, 0, 19 * n1 * sizeof(float));
-    for (int z = 0; z < dim[2]; z++)
-        for (int y = 0; y < dim[1]; y++)
-            for (int x = 0; x < dim[0]; x++) {
-                if (!(st0[lin(dim, x, y, z)] & MR_ACTIVE)) continue;
-                const int px = x >> 1, py = y >> 1, pz = z >> 1;
-                if (!parent_block(d1, st1, px, py, pz)) continue; /* hasParent */
-                for (int q = 0; q < 19; q++) {
-                    if (q == 9) continue;
-                    const int ux = uoff(x, MC[q][0]), uy = uoff(y, MC[q][1]), uz = uoff(z, MC[q][2]);
-                    /* the same-level voxel along the uncle offset: a proxy for
-                     * "the uncle's region is not refined" */
-                    if (act0(dim, st0, x + ux, y + uy, z + uz)) continue;
-                    if (!act1(d1, st1, px + ux, py + uy, pz + uz)) continue;
-                    const int cx = px + ux - MC[q][0], cy = py + uy - MC[q][1], cz = pz + uz - MC[q][2];
-                    if (!act1(d1, st1, cx, cy, cz)) continue;
-                    ss1[(size_t)q * n1 + lin(d1, cx, cy, cz)] += 1.f;
+    /* sumStore counts (init.h:8-55), level l -> l + 1: no test of the fine
+     * voxel's children or of the coarse target's child */
+    for (int l = 1; l < depth; l++) memset(ss[l], 0, 19 * nvox(dims + 3 * l) * sizeof(float));
+    for (int l = 0; l + 1 < depth; l++) {
+        const int32_t* d = dims + 3 * l;
+        const int32_t* du = dims + 3 * (l + 1);
+        const size_t nu = nvox(du);
+        int32_t nbu[3];
+        uint8_t* blk = memory_blocks(du, st[l + 1], nbu);
+        for (int z = 0; z < d[2]; z++)
+            for (int y = 0; y < d[1]; y++)
+                for (int x = 0; x < d[0]; x++) {
+                    if (!(st[l][lin(d, x, y, z)] & MR_ACTIVE)) continue;
+                    const int px = x >> 1, py = y >> 1, pz = z >> 1;
+                    if (!parent_block(nbu, blk, px, py, pz)) continue; /* hasParent */
+                    for (int q = 0; q < 19; q++) {
+                        if (q == 9) continue;
+                        const int ux = uoff(x, MC[q][0]), uy = uoff(y, MC[q][1]), uz = uoff(z, MC[q][2]);
+                        /* the same-level voxel along the uncle offset: a proxy
+                         * for "the uncle's region is not refined" */
+                        if (act(d, st[l], x + ux, y + uy, z + uz)) continue;
+                        if (!act(du, st[l + 1], px + ux, py + uy, pz + uz)) continue;
+                        const int cx = px + ux - MC[q][0], cy = py + uy - MC[q][1], cz = pz + uz - MC[q][2];
+                        if (!act(du, st[l + 1], cx, cy, cz)) continue;
+                        ss[l + 1][(size_t)q * nu + lin(du, cx, cy, cz)] += 1.f;
+                    }
                 }
+        free(blk);
+    }
+    /* reciprocal (init.h:58-80): 1.f / (refFactor * ss) with refFactor of
+     * the field's type T */
+    for (int l = 1; l < depth; l++)
+        for (size_t i = 0; i < 19 * nvox(dims + 3 * l); i++)
+            if ((double)ss[l][i] > 0.001) {
+                if (storage == OR_F64)
+                    ss[l][i] = (float)(1.0 / (2.0 * (double)ss[l][i]));
+                else
+                    ss[l][i] = 1.f / (2.0f * ss[l][i]);
             }
-    /* reciprocal (init.h:58-80), level 1: 1.f / (refFactor * ss) with
-     * refFactor of the field's type T */
-    for (size_t i = 0; i < 19 * n1; i++)
-        if ((double)ss1[i] > 0.001) {
-            if (storage == OR_F64)
-                ss1[i] = (float)(1.0 / (2.0 * (double)ss1[i]));
-            else
-                ss1[i] = 1.f / (2.0f * ss1[i]);
-        }
 }
 
 /* computeOmega (util.h:40-45) */
@@ -302,12 +361,12 @@ double or_mres_omega(double omega0, int level, int num_levels)
 
 /* BGK of collide.h:8-149 in T = double / float: the moment sums and the
  * collision in the cavity benchmark's association order, all constants of
- * type T.  Refined voxels of level 1 reset their fout (the accumulator of
- * storeFine) to 0. */
+ * type T.  Refined voxels of levels above 0 reset their fout (the
+ * accumulator of storeFine) to 0. */
 static void collide_level(const int32_t d[3], int lvl, int s, const uint8_t* st, const void* fin,
                           void* fout, double omega)
 {
-    const size_t n = (size_t)d[0] * d[1] * d[2];
+    const size_t n = nvox(d);
 #pragma omp parallel for schedule(static) num_threads(or_get_threads())
     for (int z = 0; z < d[2]; z++)
         for (int y = 0; y < d[1]; y++)
@@ -371,73 +430,79 @@ static void collide_level(const int32_t d[3], int lvl, int s, const uint8_t* st,
             }
 }
 
-/* storeFine (store.h:98-163): every level-0 voxel with a parent adds, for
- * each q != 9, its post-collision population q to the level-1 voxel that
- * the coarse leaf downstream will pull it from.  Serial, voxels in z, y, x
- * order and q ascending: the order of the additions is the order of this
- * loop (the reference adds with atomics in an unspecified order). */
-static void store_fine(const int32_t dim[3], int s, const uint8_t* st0, const uint8_t* st1,
-                       const void* fout0, void* fout1)
+/* storeFine (store.h:98-163) from level l into level l + 1: every active
+ * voxel of level l without children and with a parent adds, for each q != 9,
+ * its post-collision population q to the level-(l+1) voxel that the coarse
+ * leaf downstream will pull it from.  Serial, voxels in z, y, x order and q
+ * ascending: the order of the additions is the order of this loop (the
+ * reference adds with atomics in an unspecified order). */
+static void store_fine(const int32_t* dims, int l, int s, const uint8_t* const* st, const void* fout_l,
+                       void* fout_u)
 {
-    int32_t d1[3];
-    or_mres_dims(dim, d1);
-    const size_t n0 = (size_t)dim[0] * dim[1] * dim[2], n1 = (size_t)d1[0] * d1[1] * d1[2];
-    for (int z = 0; z < dim[2]; z++)
-        for (int y = 0; y < dim[1]; y++)
-            for (int x = 0; x < dim[0]; x++) {
-                const size_t i = lin(dim, x, y, z);
-                if (!(st0[i] & MR_ACTIVE)) continue;
+    const int32_t* d = dims + 3 * l;
+    const int32_t* du = dims + 3 * (l + 1);
+    const size_t n = nvox(d), nu = nvox(du);
+    int32_t nbu[3];
+    uint8_t* blk = memory_blocks(du, st[l + 1], nbu);
+    for (int z = 0; z < d[2]; z++)
+        for (int y = 0; y < d[1]; y++)
+            for (int x = 0; x < d[0]; x++) {
+                const size_t i = lin(d, x, y, z);
+                if (!(st[l][i] & MR_ACTIVE) || (st[l][i] & MR_REFINED)) continue;
                 const int px = x >> 1, py = y >> 1, pz = z >> 1;
-                if (!parent_block(d1, st1, px, py, pz)) continue; /* hasParent */
+                if (!parent_block(nbu, blk, px, py, pz)) continue; /* hasParent */
                 for (int q = 0; q < 19; q++) {
                     if (q == 9) continue;
                     const int ux = uoff(x, MC[q][0]), uy = uoff(y, MC[q][1]), uz = uoff(z, MC[q][2]);
-                    if (act0(dim, st0, x + ux, y + uy, z + uz)) continue;
-                    if (!act1(d1, st1, px + ux, py + uy, pz + uz)) continue; /* uncle */
+                    if (act(d, st[l], x + ux, y + uy, z + uz)) continue;
+                    if (!act(du, st[l + 1], px + ux, py + uy, pz + uz)) continue; /* uncle */
                     const int sx = ux - MC[q][0], sy = uy - MC[q][1], sz = uz - MC[q][2];
-                    if (!act1(d1, st1, px + sx, py + sy, pz + sz)) continue; /* cs */
-                    if (!act0(dim, st0, x + sx, y + sy, z + sz)) continue;   /* csChild */
-                    const size_t t = (size_t)q * n1 + lin(d1, px + sx, py + sy, pz + sz);
-                    const double v = get(fout0, s, (size_t)q * n0 + i);
+                    if (!act(du, st[l + 1], px + sx, py + sy, pz + sz)) continue; /* cs */
+                    if (!act(d, st[l], x + sx, y + sy, z + sz)) continue;         /* csChild */
+                    const size_t t = (size_t)q * nu + lin(du, px + sx, py + sy, pz + sz);
+                    const double v = get(fout_l, s, (size_t)q * n + i);
                     if (s == OR_F64)
-                        ((double*)fout1)[t] += v;
+                        ((double*)fout_u)[t] += v;
                     else
-                        ((float*)fout1)[t] += (float)v;
+                        ((float*)fout_u)[t] += (float)v;
                 }
             }
+    free(blk);
 }
 
 /* Stream fused with coalescence and explosion (stream.h:170-225), pull,
- * for the bulk leaf voxels of one level: from a same-level neighbour that
- * is not refined (bounce-back with the own opposite population when it is
- * not bulk); from the parent level's neighbour (explosion) when the
- * neighbour is absent and the voxel has a parent; from a refined neighbour
+ * for the bulk leaf voxels of level l: from a same-level neighbour without
+ * children (bounce-back with the own opposite population when it is not
+ * bulk); from the next level's neighbour (explosion) when the neighbour is
+ * absent and the voxel has a parent; from a neighbour with children
  * (coalescence) scaled by its sumStore weight. */
-static void stream_level(const int32_t dim[3], int lvl, int s, const uint8_t* st0, const uint8_t* st1,
+static void stream_level(const int32_t* dims, int depth, int l, int s, const uint8_t* const* st,
                          const void* fout_l, const void* fout_up, const float* ss, void* fin_l)
 {
-    int32_t d1[3];
-    or_mres_dims(dim, d1);
-    const int32_t* d = lvl ? d1 : dim;
-    const uint8_t* st = lvl ? st1 : st0;
-    const size_t n = (size_t)d[0] * d[1] * d[2], n1 = (size_t)d1[0] * d1[1] * d1[2];
+    const int32_t* d = dims + 3 * l;
+    const size_t n = nvox(d);
+    const int top = l == depth - 1;
+    const int32_t* du = top ? NULL : dims + 3 * (l + 1);
+    const size_t nu = top ? 0 : nvox(du);
+    int32_t nbu[3] = {0, 0, 0};
+    uint8_t* blk = top ? NULL : memory_blocks(du, st[l + 1], nbu);
 #pragma omp parallel for schedule(static) num_threads(or_get_threads())
     for (int z = 0; z < d[2]; z++)
         for (int y = 0; y < d[1]; y++)
             for (int x = 0; x < d[0]; x++) {
                 const size_t i = lin(d, x, y, z);
-                const uint8_t sv = st[i];
+                const uint8_t sv = st[l][i];
                 if (!(sv & MR_ACTIVE) || ((sv >> MR_CLS_SHIFT) & 3) != OR_BULK || (sv & MR_REFINED)) continue;
-                const int has_parent = lvl == 0 && parent_block(d1, st1, x >> 1, y >> 1, z >> 1);
+                const int has_parent = !top && parent_block(nbu, blk, x >> 1, y >> 1, z >> 1);
                 for (int q = 0; q < 19; q++) {
                     const int dx = -MC[q][0], dy = -MC[q][1], dz = -MC[q][2];
                     const int nx = x + dx, ny = y + dy, nz = z + dz;
-                    const int valid = inside(d, nx, ny, nz) && (st[lin(d, nx, ny, nz)] & MR_ACTIVE);
+                    const int valid = act(d, st[l], nx, ny, nz);
                     const size_t j = valid ? lin(d, nx, ny, nz) : 0;
-                    const int ngh_refined = valid && (st[j] & MR_REFINED);
+                    const int ngh_refined = valid && (st[l][j] & MR_REFINED);
                     if (!ngh_refined) {
                         if (valid) {
-                            if (((st[j] >> MR_CLS_SHIFT) & 3) == OR_BULK) {
+                            if (((st[l][j] >> MR_CLS_SHIFT) & 3) == OR_BULK) {
                                 put(fin_l, s, (size_t)q * n + i, get(fout_l, s, (size_t)q * n + j));
                             } else {
                                 const int o = MOPP[q];
@@ -453,9 +518,9 @@ static void stream_level(const int32_t dim[3], int lvl, int s, const uint8_t* st
                         } else if (has_parent && q != 9) {
                             const int ux = (x >> 1) + uoff(x, dx), uy = (y >> 1) + uoff(y, dy),
                                       uz = (z >> 1) + uoff(z, dz);
-                            if (act1(d1, st1, ux, uy, uz))
+                            if (act(du, st[l + 1], ux, uy, uz))
                                 put(fin_l, s, (size_t)q * n + i,
-                                    get(fout_up, s, (size_t)q * n1 + lin(d1, ux, uy, uz)));
+                                    get(fout_up, s, (size_t)q * nu + lin(du, ux, uy, uz)));
                         }
                     } else if (q != 9) {
                         const float w = ss[(size_t)q * n + j];
@@ -467,23 +532,51 @@ static void stream_level(const int32_t dim[3], int lvl, int s, const uint8_t* st
                     }
                 }
             }
+    free(blk);
 }
 
-/* One step of the coarse level (nonUniformTimestepRecursive for two
- * levels, lbmMultiRes.h:141-263, baseline operation order with the fine
- * store): collide level 1; twice (collide level 0, store level 0, stream
- * level 0); stream level 1.  omega0 is the coarse level's relaxation. */
-void or_mres_step(const int32_t dim[3], int s, const uint8_t* st0, const uint8_t* st1, void* fin0,
-                  void* fout0, void* fin1, void* fout1, const float* ss1, double omega0)
+typedef struct {
+    const int32_t* dims;
+    int depth, s;
+    const uint8_t* const* st;
+    void* const* fin;
+    void* const* fout;
+    const float* const* ss;
+    double omega0;
+} mr_ctx;
+
+static void collide_store(const mr_ctx* c, int l)
 {
-    int32_t d1[3];
-    or_mres_dims(dim, d1);
-    const double om1 = or_mres_omega(omega0, 1, 2), om0 = or_mres_omega(omega0, 0, 2);
-    collide_level(d1, 1, s, st1, fin1, fout1, om1);
-    for (int sub = 0; sub < 2; sub++) {
-        collide_level(dim, 0, s, st0, fin0, fout0, om0);
-        store_fine(dim, s, st0, st1, fout0, fout1);
-        stream_level(dim, 0, s, st0, st1, fout0, fout1, NULL, fin0);
-    }
-    stream_level(dim, 1, s, st0, st1, fout1, NULL, ss1, fin1);
+    collide_level(c->dims + 3 * l, l, c->s, c->st[l], c->fin[l], c->fout[l],
+                  or_mres_omega(c->omega0, l, c->depth));
+    if (l != c->depth - 1) store_fine(c->dims, l, c->s, c->st, c->fout[l], c->fout[l + 1]);
+}
+
+static void stream_of(const mr_ctx* c, int l)
+{
+    stream_level(c->dims, c->depth, l, c->s, c->st, c->fout[l], l + 1 < c->depth ? c->fout[l + 1] : NULL,
+                 c->ss[l], c->fin[l]);
+}
+
+/* nonUniformTimestepRecursive (lbmMultiRes.h:141-263) with the fine store
+ * and the stream fused with coalescence and explosion, unfused collision */
+static void recurse(const mr_ctx* c, int l)
+{
+    collide_store(c, l);
+    if (l != 0) recurse(c, l - 1);
+    stream_of(c, l);
+    if (l == c->depth - 1) return;
+    collide_store(c, l);
+    if (l != 0) recurse(c, l - 1);
+    stream_of(c, l);
+}
+
+/* One step of the coarsest level; omega0 is its relaxation. */
+void or_mres_step(const int32_t dim[3], int depth, int s, const uint8_t* const* st, void* const* fin,
+                  void* const* fout, const float* const* ss, double omega0)
+{
+    int32_t dims[3 * OR_MRES_MAX_DEPTH];
+    or_mres_dims(dim, depth, dims);
+    const mr_ctx c = {dims, depth, s, st, fin, fout, ss, omega0};
+    recurse(&c, depth - 1);
 }

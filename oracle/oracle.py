@@ -69,11 +69,11 @@ _SIG = {
     "or_moments_block": (None, [_P, _P, _P, _P, _P]),
     "or_mass_block": (_D, [_P, _P, _P]),
     "or_run_block": (None, [_P, _P, _P, _P, _D, _P, C.c_int]),
-    "or_mres_dims": (None, [_P, _P]),
-    "or_mres_build": (None, [_P, _P, _P, _P, _P]),
-    "or_mres_setup": (None, [_P, C.c_int, _D, _P, _P, _P, _P, _P, _P, _P]),
+    "or_mres_dims": (None, [_P, C.c_int, _P]),
+    "or_mres_build": (C.c_int, [_P, C.c_int, _P, _P]),
+    "or_mres_setup": (None, [_P, C.c_int, C.c_int, _D, _P, _P, _P, _P]),
     "or_mres_omega": (_D, [_D, C.c_int, C.c_int]),
-    "or_mres_step": (None, [_P, C.c_int, _P, _P, _P, _P, _P, _P, _P, _D]),
+    "or_mres_step": (None, [_P, C.c_int, C.c_int, _P, _P, _P, _P, _D]),
 }
 
 
@@ -285,52 +285,71 @@ MR_ACTIVE, MR_REFINED, MR_CLS_SHIFT = 1, 2, 2
 
 
 class MresOracle:
-    """Two-level grid refinement LBM (mres_oracle.c, restating
-    apps/lbmMultiRes for depth 2, BGK D3Q19).  lam0 / lam1: the activation
-    lambdas of level 0 (every level-0 voxel) and level 1 (every level-1
-    voxel, evaluated at its level-0 origin), uint8, x fastest.  Fields are
-    dense per level, [19][z][y][x]; st0 / st1 the per-voxel state bytes."""
+    """Grid refinement LBM over `depth` levels (mres_oracle.c, restating
+    apps/lbmMultiRes, BGK D3Q19).  MresOracle(dim, [lam_0, .., lam_{D-1}],
+    storage) or, for two levels, MresOracle(dim, lam0, lam1, storage): lam_l
+    is the activation lambda of level l at every level-l voxel (evaluated at
+    its level-0 origin v 2^l), uint8, x fastest.  Fields are dense per level,
+    [19][z][y][x]; st[l] the per-voxel state bytes (st0 / st1, dim1 name
+    levels 0 and 1)."""
 
-    def __init__(self, dim, lam0, lam1, storage=F64):
+    def __init__(self, dim, lams, *rest):
+        if isinstance(lams, (list, tuple)):
+            storage = rest[0] if rest else F64
+        else:  # (dim, lam0, lam1[, storage])
+            lams = [lams, rest[0]]
+            storage = rest[1] if len(rest) > 1 else F64
+        self.depth = len(lams)
+        assert 2 <= self.depth <= 5
         self.dim = tuple(int(v) for v in dim)
         self._d0 = (C.c_int32 * 3)(*self.dim)
-        d1 = (C.c_int32 * 3)()
-        lib.or_mres_dims(self._d0, d1)
-        self.dim1 = tuple(int(v) for v in d1)
-        self._d1 = d1
-        self.n0 = int(np.prod(self.dim))
-        self.n1 = int(np.prod(self.dim1))
+        dims = (C.c_int32 * (3 * self.depth))()
+        lib.or_mres_dims(self._d0, self.depth, dims)
+        self.dims = [tuple(int(v) for v in dims[3 * l:3 * l + 3]) for l in range(self.depth)]
+        self.dim1 = self.dims[1]
+        self.n = [int(np.prod(d)) for d in self.dims]
+        self.n0, self.n1 = self.n[0], self.n[1]
         self.storage = storage
-        lam0 = np.ascontiguousarray(lam0, np.uint8).reshape(-1)
-        lam1 = np.ascontiguousarray(lam1, np.uint8).reshape(-1)
-        assert lam0.size == self.n0 and lam1.size == self.n1
-        self.st0 = np.zeros(self.n0, np.uint8)
-        self.st1 = np.zeros(self.n1, np.uint8)
-        lib.or_mres_build(self._d0, _ptr(lam0), _ptr(lam1), _ptr(self.st0), _ptr(self.st1))
+        lams = [np.ascontiguousarray(a, np.uint8).reshape(-1) for a in lams]
+        assert all(a.size == n for a, n in zip(lams, self.n))
+        self.st = [np.zeros(n, np.uint8) for n in self.n]
+        self._lams = lams
+        self.balanced = int(lib.or_mres_build(self._d0, self.depth, self._ptrs(lams), self._ptrs(self.st)))
+
+    @staticmethod
+    def _ptrs(arrays):
+        return (C.c_void_p * len(arrays))(*[None if a is None else a.ctypes.data for a in arrays])
+
+    @property
+    def st0(self):
+        return self.st[0]
+
+    @property
+    def st1(self):
+        return self.st[1]
 
     def active(self, level):
-        return ((self.st1 if level else self.st0) & MR_ACTIVE) != 0
+        return (self.st[level] & MR_ACTIVE) != 0
 
-    def refined(self):
-        return (self.st1 & MR_REFINED) != 0
+    def refined(self, level=1):
+        return (self.st[level] & MR_REFINED) != 0
 
     def setup_cavity(self, ulb):
         dt = NP_STORAGE[self.storage]
-        self.fin = [np.zeros(19 * self.n0, dt), np.zeros(19 * self.n1, dt)]
-        self.fout = [np.zeros(19 * self.n0, dt), np.zeros(19 * self.n1, dt)]
-        self.ss1 = np.zeros(19 * self.n1, np.float32)
-        lib.or_mres_setup(self._d0, self.storage, float(ulb), _ptr(self.st0), _ptr(self.st1),
-                          _ptr(self.fin[0]), _ptr(self.fout[0]), _ptr(self.fin[1]), _ptr(self.fout[1]),
-                          _ptr(self.ss1))
+        self.fin = [np.zeros(19 * n, dt) for n in self.n]
+        self.fout = [np.zeros(19 * n, dt) for n in self.n]
+        self.ss = [None] + [np.zeros(19 * n, np.float32) for n in self.n[1:]]
+        self.ss1 = self.ss[1]
+        lib.or_mres_setup(self._d0, self.depth, self.storage, float(ulb), self._ptrs(self.st),
+                          self._ptrs(self.fin), self._ptrs(self.fout), self._ptrs(self.ss))
 
     def classes(self, level):
-        return ((self.st1 if level else self.st0) >> MR_CLS_SHIFT) & 3
+        return (self.st[level] >> MR_CLS_SHIFT) & 3
 
     def run(self, n, omega0):
+        st, fi, fo, ss = self._ptrs(self.st), self._ptrs(self.fin), self._ptrs(self.fout), self._ptrs(self.ss)
         for _ in range(int(n)):
-            lib.or_mres_step(self._d0, self.storage, _ptr(self.st0), _ptr(self.st1),
-                             _ptr(self.fin[0]), _ptr(self.fout[0]), _ptr(self.fin[1]),
-                             _ptr(self.fout[1]), _ptr(self.ss1), float(omega0))
+            lib.or_mres_step(self._d0, self.depth, self.storage, st, fi, fo, ss, float(omega0))
 
     @staticmethod
     def omega(omega0, level, num_levels=2):

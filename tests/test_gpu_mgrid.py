@@ -1,8 +1,10 @@
-"""Two-level grid refinement on the GPU (nx_mgrid_*, the lbmMultiRes BGK
-path for depth 2) against the CPU oracle: every population of both fields of
-both levels bit for bit after several coarse steps, for the reference's two
-field types; and at 256^3 the properties the cavity keeps (mass, z-mirror
-symmetry)."""
+"""Grid refinement on the GPU (nx_mgrid_*, the lbmMultiRes BGK path for
+depth 2 to 4) against the CPU oracle: every population of both fields of
+every level bit for bit after several coarse steps, for the reference's two
+field types; at 256^3 the properties the cavity keeps (mass, z-mirror
+symmetry); and the centre line of the 160^3 cavity, two levels and the
+multi-resolution app's own three-level --scale 1 grid, against the
+reference's uniform run."""
 import numpy as np
 import pytest
 
@@ -134,3 +136,92 @@ def test_mgrid_160_centerline_close_to_reference_uniform(nx):
     print(f"two-level 160^3 centre line vs the reference's uniform run: max |du|/U {dev.max():.4f} "
           f"at y = {int(dev.argmax())}, mean {dev.mean():.4f}")
     assert dev.max() < 0.02
+
+
+def _strong_balance_lambdas():
+    dim = (24, 24, 24)
+    lam = [np.zeros(int(np.prod(d)), np.uint8) for d in mg.level_dims(dim, 3)]
+    for x, y, z in ((6, 6, 6), (13, 9, 11)):
+        lam[0][x + 24 * (y + 24 * z)] = 1
+    lam[2][:] = 1
+    return dim, lam
+
+
+DEEP = [((48, 48, 48), mg.LID_SCALES[0][1], 100.0, 8),   # the app's --scale 0 grid
+        ((37, 30, 41), (0.0, -0.25, -0.5, -1.0), 30.0, 8),
+        ((50, 44, 40), (0.0, -0.2, -0.4, -0.7, -1.0), 50.0, 4),
+        ((64, 64, 64), mg.LID_SCALES[6][1], 400.0, 3),
+        ((24, 24, 24), None, 20.0, 6)]                     # strong balance adds level-1 voxels
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("dim,sdf,re,steps", DEEP)
+def test_mgrid_deep_equals_oracle(nx, orc, storage, dim, sdf, re, steps):
+    """Three and four levels: every population of both fields of every level
+    bit for bit with the oracle after `steps` coarse steps (4 or 8 level-0
+    steps each)."""
+    if sdf is None:
+        dim, lams = _strong_balance_lambdas()
+    else:
+        lams = list(mg.cavity_lambdas(dim, sdf))
+    depth = len(lams)
+    ulb, om0 = mg.cavity_parameters(dim, re, storage, depth=depth)
+    print(f"mgrid {dim} depth {depth} {storage} Re={re} omega0={om0:.6f} steps={steps}", flush=True)
+    g = mg.MGrid(dim, lams, storage)
+    g.setup_cavity(ulb)
+    m = orc.MresOracle(dim, lams, orc.F64 if storage == "f64" else orc.F32)
+    m.setup_cavity(ulb)
+    for l, st in enumerate(g.state()):
+        assert np.array_equal(st, m.st[l]), l
+    assert g.info()["store_links"] > 0
+    g.run(steps, om0)
+    m.run(steps, om0)
+    for lvl in range(depth):
+        act = np.tile(m.active(lvl), 19)
+        for which, ref in ((0, m.fin[lvl]), (1, m.fout[lvl])):
+            got = g.fields(lvl, which)
+            n_diff = int(np.count_nonzero(got[act].view(np.uint8) != ref[act].view(np.uint8)))
+            assert n_diff == 0, (lvl, which, n_diff, np.abs(got[act] - ref[act]).max())
+
+
+def test_mgrid_160_three_levels_centerline(nx):
+    """The multi-resolution app's --scale 1 grid (160^3, three levels, levelSDF
+    0 / -0.2 / -0.4 / -1; lidDrivenCavity.h:119-126) at Re 100 with the level-0
+    relaxation of the uniform run, 5000 coarse steps (20000 level-0 steps):
+    the centre line u_x(y) at x = z = 80, read from the leaf covering each
+    level-0 position, stays within 2.5% of the lid speed of the reference's
+    uniform dGrid output (tests/golden/NeonUniformLBM_20000_Y.dat); measured
+    1.79% at y = 127 (read from level 2), mean 0.50%."""
+    import pathlib
+    N, RE, ULB, STEPS = 160, 100.0, 0.04, 5000
+    dat = np.loadtxt(pathlib.Path(__file__).resolve().parent / "golden" / "NeonUniformLBM_20000_Y.dat")
+    omega_f = 1.0 / (3.0 * ULB * (N - 2) / RE + 0.5)
+    omega0 = 8.0 * omega_f / (2.0 + 3.0 * omega_f)  # computeOmega(omega0, 0, 3) == omega_f
+    dim, sdf = (N, N, N), mg.LID_SCALES[1][1]
+    g = mg.MGrid(dim, list(mg.cavity_lambdas(dim, sdf)), "f64")
+    g.setup_cavity(ULB)
+    g.run(STEPS, omega0)
+    st = g.state()
+    c = np.array([[-1, 0, 0], [0, -1, 0], [0, 0, -1], [-1, -1, 0], [-1, 1, 0], [-1, 0, -1], [-1, 0, 1],
+                  [0, -1, -1], [0, -1, 1], [0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1], [1, 1, 0], [1, -1, 0],
+                  [1, 0, 1], [1, 0, -1], [0, 1, 1], [0, 1, -1]], float)
+    f = [g.fields(l).reshape(19, *g.dims[l][::-1]) for l in range(3)]
+    a = [((st[l] & 3) == 1).reshape(g.dims[l][::-1]) for l in range(3)]  # active leaves
+    ux = np.zeros(N)
+    used = [0, 0, 0]
+    for y in range(1, N - 1):
+        for l in range(3):
+            p = (N // 2 >> l, y >> l, N // 2 >> l)
+            if a[l][p[2], p[1], p[0]]:
+                v = f[l][:, p[2], p[1], p[0]]
+                ux[y] = (c[:, 0] @ v) / v.sum()
+                used[l] += 1
+                break
+        else:
+            raise AssertionError(f"no leaf covers y = {y}")
+    ux[0], ux[-1] = dat[0, 1], dat[-1, 1]
+    dev = np.abs(ux - dat[:, 1]) / ULB
+    print(f"three-level 160^3 centre line vs the reference's uniform run: max |du|/U {dev.max():.4f} "
+          f"at y = {int(dev.argmax())}, mean {dev.mean():.4f}; positions per level {used}")
+    assert all(used)
+    assert dev.max() < 0.025

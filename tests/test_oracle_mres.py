@@ -161,3 +161,162 @@ def test_two_levels_close_to_uniform(orc):
     mass_m = (m.fin[0].reshape(19, -1)[:, leaf0.reshape(-1)].sum()
               + 8 * m.fin[1].reshape(19, -1)[:, leaf1.reshape(-1)].sum())
     assert abs(mass_m - mass_u) / mass_u < 1e-4
+
+
+# ---- more than two levels ------------------------------------------------------
+
+DEEP = [((48, 48, 48), mg.LID_SCALES[0][1]),              # the app's --scale 0: 48^3, depth 3
+        ((64, 64, 64), mg.LID_SCALES[6][1]),              # --scale 6's level bands at 64^3, depth 4
+        ((37, 30, 41), (0.0, -0.25, -0.5, -1.0)),
+        ((50, 44, 40), (0.0, -0.2, -0.4, -0.7, -1.0))]
+
+
+def _cover_levels(m):
+    """Per level l, for every level-l position: the lowest level >= l whose
+    voxel covering it is active (depth when none is)."""
+    out = [None] * m.depth
+    for l in range(m.depth - 1, -1, -1):
+        d = m.dims[l]
+        a = m.active(l).reshape(d[2], d[1], d[0])
+        up = np.full(a.shape, m.depth) if l == m.depth - 1 else \
+            np.repeat(np.repeat(np.repeat(out[l + 1], 2, 0), 2, 1), 2, 2)[:d[2], :d[1], :d[0]]
+        out[l] = np.where(a, l, up)
+    return out
+
+
+def _balance_violations(m):
+    """Active voxels of level l with an in-domain neighbour position whose
+    finest active cover lies two or more levels up."""
+    cover = _cover_levels(m)
+    bad = 0
+    for l in range(m.depth):
+        d = m.dims[l]
+        a = m.active(l).reshape(d[2], d[1], d[0])
+        c = np.pad(cover[l], 1, constant_values=-1)  # outside the domain: never a violation
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    if dx or dy or dz:
+                        n = c[1 + dz:1 + dz + d[2], 1 + dy:1 + dy + d[1], 1 + dx:1 + dx + d[0]]
+                        bad += int((a & (n > l + 1) & (n < m.depth)).sum())
+    return bad
+
+
+@pytest.mark.parametrize("dim,sdf", DEEP)
+def test_deep_builder_matches_oracle(orc, dim, sdf):
+    lams = mg.cavity_lambdas(dim, sdf)
+    m = orc.MresOracle(dim, list(lams), orc.F64)
+    g = mg.MGrid(dim, list(lams), "f64")
+    st = g.state()
+    for l in range(m.depth):
+        assert np.array_equal(st[l] & 3, m.st[l] & 3), l
+    assert g.info()["active"] == [int(m.active(l).sum()) for l in range(m.depth)]
+
+
+@pytest.mark.parametrize("dim,sdf", DEEP)
+def test_deep_grid_structure(orc, dim, sdf):
+    """Every level-0 position is covered by exactly one leaf, refined <=> an
+    active child, kept refined voxels have an unrefined in-domain neighbour,
+    and every neighbour position of an active voxel of level l is covered on
+    level l or l + 1 (the strong-balance condition)."""
+    m = orc.MresOracle(dim, list(mg.cavity_lambdas(dim, sdf)), orc.F64)
+    X, Y, Z = dim
+    cover = np.zeros((Z, Y, X), int)
+    for l in range(m.depth):
+        d = m.dims[l]
+        a = m.active(l).reshape(d[2], d[1], d[0])
+        assert a.any(), l
+        leaf = a & ~m.refined(l).reshape(a.shape) if l else a
+        s = 1 << l
+        cover += np.repeat(np.repeat(np.repeat(leaf, s, 0), s, 1), s, 2)[:Z, :Y, :X]
+        if l:
+            f = m.dims[l - 1]
+            pad = np.zeros((2 * d[2], 2 * d[1], 2 * d[0]), bool)
+            pad[:f[2], :f[1], :f[0]] = m.active(l - 1).reshape(f[2], f[1], f[0])
+            child = pad.reshape(d[2], 2, d[1], 2, d[0], 2).any(axis=(1, 3, 5))
+            assert np.array_equal(m.refined(l).reshape(a.shape), a & child), l
+    assert (cover == 1).all()
+    assert _balance_violations(m) == 0 and m.balanced == 0
+
+
+def test_strong_balance_pass(orc):
+    """Two isolated fine groups inside a coarse level-2 region with no level-1
+    band between them: the pass adds level-1 voxels (38 here) until every
+    neighbour of an active voxel is covered one level up at most; the
+    library's builder agrees voxel for voxel."""
+    dim = (24, 24, 24)
+    dims = mg.level_dims(dim, 3)
+    lam = [np.zeros(int(np.prod(d)), np.uint8) for d in dims]
+    for x, y, z in ((6, 6, 6), (13, 9, 11)):
+        lam[0][x + 24 * (y + 24 * z)] = 1
+    lam[2][:] = 1
+    m = orc.MresOracle(dim, lam, orc.F64)
+    assert m.balanced == 38
+    assert _balance_violations(m) == 0
+    g = mg.MGrid(dim, lam, "f64")
+    st = g.state()
+    for l in range(3):
+        assert np.array_equal(st[l] & 3, m.st[l] & 3), l
+
+
+@pytest.mark.parametrize("depth", [3, 4])
+def test_deep_rest_state_is_kept(orc, depth):
+    dim = (40, 36, 44)
+    sdf = (0.0, -0.2, -0.45, -1.0) if depth == 3 else (0.0, -0.2, -0.35, -0.6, -1.0)
+    m = orc.MresOracle(dim, list(mg.cavity_lambdas(dim, sdf)), orc.F64)
+    m.setup_cavity(0.0)
+    m.run(6, 1.6)
+    _, w, _ = orc.lattice()
+    for lvl in range(depth):
+        act = m.active(lvl) & (m.classes(lvl) == orc.BULK)
+        if lvl:
+            act &= ~m.refined(lvl)
+        f = m.fin[lvl].reshape(19, -1)[:, act].astype(np.float64)
+        assert act.sum() > 0
+        assert np.abs(f - w[:, None]).max() <= 2e-8 * w.max(), lvl
+
+
+def test_deep_all_fine_equals_uniform(orc):
+    """Depth 3 with every voxel on level 0: one coarse step is four uniform
+    steps with computeOmega(omega0, 0, 3)."""
+    N, T, re, ulb = 24, 30, 20.0, 0.04
+    m = orc.MresOracle((N, N, N), list(mg.cavity_lambdas((N, N, N), (0.0, -5.0, -6.0, -7.0))), orc.F64)
+    assert m.active(0).all() and not m.active(1).any() and not m.active(2).any()
+    m.setup_cavity(ulb)
+    om0 = 1.0 / (3.0 * ulb * m.dims[2][0] / re + 0.5)
+    m.run(T, om0)
+    u = orc.DenseOracle((N, N, N), orc.F64, orc.COMPUTE_F64)
+    u.setup_cavity(ulb)
+    u.run(4 * T, orc.MresOracle.omega(om0, 0, 3))
+    bulk = ((u.flags >> 24) & 3).reshape(N, N, N) == orc.BULK
+    d = np.abs(_uniform_velocity(orc, m.fin[0], (N, N, N)) - _uniform_velocity(orc, u.fin, (N, N, N)))[:, bulk]
+    assert d.max() < 1e-14, d.max()
+
+
+def test_three_levels_close_to_uniform(orc):
+    """The app's --scale 0 grid (48^3, three levels), Re 20, 100 coarse steps
+    against 400 uniform steps at level-0 resolution: largest velocity
+    difference on the leaves of levels 0 / 1 / 2 measured 1.9% / 1.8% / 2.0%
+    of the lid speed, RMS 0.23% / 0.63% / 1.1%.  Two level-0 sub-steps per
+    level-1 step instead of four, or level 1 relaxing with level 0's omega,
+    move these by tens of percent."""
+    N, T, re, ulb = 48, 100, 20.0, 0.04
+    m = orc.MresOracle((N, N, N), list(mg.cavity_lambdas((N, N, N), mg.LID_SCALES[0][1])), orc.F64)
+    m.setup_cavity(ulb)
+    om0 = 1.0 / (3.0 * ulb * m.dims[2][0] / re + 0.5)
+    m.run(T, om0)
+    u = orc.DenseOracle((N, N, N), orc.F64, orc.COMPUTE_F64)
+    u.setup_cavity(ulb)
+    u.run(4 * T, orc.MresOracle.omega(om0, 0, 3))
+    uu = _uniform_velocity(orc, u.fin, (N, N, N))
+    bounds = [(0.03, 0.005), (0.03, 0.01), (0.03, 0.02)]
+    for l in range(3):
+        d, s = m.dims[l], 1 << l
+        ul = _uniform_velocity(orc, m.fin[l], d)
+        leaf = (m.active(l) & ((m.st[l] & 2) == 0) & (m.classes(l) == orc.BULK)).reshape(d[2], d[1], d[0])
+        pad = np.zeros((3, d[2] * s, d[1] * s, d[0] * s))
+        pad[:, :N, :N, :N] = uu
+        uc = pad.reshape(3, d[2], s, d[1], s, d[0], s).mean(axis=(2, 4, 6))
+        dd = np.sqrt(((ul - uc) ** 2).sum(0))[leaf] / ulb
+        print(f"level {l}: {int(leaf.sum())} leaves, max {dd.max():.4f}, rms {np.sqrt((dd ** 2).mean()):.4f}")
+        assert dd.max() < bounds[l][0] and np.sqrt((dd ** 2).mean()) < bounds[l][1], l
