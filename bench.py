@@ -77,15 +77,22 @@ CONVERGE = {
 }
 
 
-# Two-level refined lid-driven cavity (the lbmMultiRes lid problem at depth
-# 2, BGK D3Q19): 256^3 domain, level 0 = the shell within a quarter of the
-# half width of the faces (sdfCube > -1/4), level 1 the interior; ulb 0.04,
-# Re 1000 on the top level's length.  A step is one coarse step: collide,
-# store, stream at both levels, level 0 twice.
+# Refined lid-driven cavity (the lbmMultiRes lid problem, BGK D3Q19).
+# mres256_*: 256^3 domain, two levels, level 0 = the shell within a quarter
+# of the half width of the faces (sdfCube > -1/4), level 1 the interior, Re
+# 1000.  mres_scale*: the app's own --scale configurations
+# (apps/lbmMultiRes/lidDrivenCavity.h:95-214; grid edge and levelSDF, depth
+# 3 or 4) at Re 100, as its run.sh sweeps them.  ulb 0.04, Re on the top
+# level's length.  A step is one coarse step: the level recursion, level l
+# updated 2^(depth-1-l) times.
 MRES = {
-    # name: (N, level-0 sdf threshold, storage, default steps, default warmup)
-    "mres256_f32": (256, -0.25, "f32", 100, 10),
-    "mres256_f64": (256, -0.25, "f64", 50, 5),
+    # name: (N, levelSDF, storage, default steps, default warmup, Re)
+    "mres256_f32": (256, (0.0, -0.25, -1.0), "f32", 100, 10, 1000.0),
+    "mres256_f64": (256, (0.0, -0.25, -1.0), "f64", 50, 5, 1000.0),
+    "mres_scale2_f32": (240, (0.0, -24.0 / 120.0, -80.0 / 120.0, -1.0), "f32", 50, 5, 100.0),
+    "mres_scale2_f64": (240, (0.0, -24.0 / 120.0, -80.0 / 120.0, -1.0), "f64", 25, 3, 100.0),
+    "mres_scale5_f32": (512, (0.0, -64.0 / 256.0, -112.0 / 256.0, -1.0), "f32", 20, 2, 100.0),
+    "mres_scale9_f32": (480, (0.0, -48.0 / 240.0, -96.0 / 240.0, -160.0 / 240.0, -1.0), "f32", 10, 1, 100.0),
 }
 
 
@@ -524,17 +531,17 @@ def batch_converge(nx, a):
 
 
 def mres_cavity(nx, a):
-    N, thr, storage, dsteps, dwarm = MRES[a.workload]
+    N, sdf, storage, dsteps, dwarm, re = MRES[a.workload]
     steps = dsteps if a.steps is None else a.steps
     warmup = dwarm if a.warmup is None else a.warmup
     from neon_amd import mgrid as mg
     nx.init(0)
     dim = (N, N, N)
+    depth = len(sdf) - 1
     t0 = time.perf_counter()
-    lam0, lam1 = mg.cavity_lambdas(dim, (0.0, thr, -1.0))
-    re = 1000.0
-    ulb, om0 = mg.cavity_parameters(dim, re, storage)
-    g = mg.MGrid(dim, lam0, lam1, storage)
+    lams = mg.cavity_lambdas(dim, sdf)
+    ulb, om0 = mg.cavity_parameters(dim, re, storage, depth=depth)
+    g = mg.MGrid(dim, list(lams), storage)
     g.setup_cavity(ulb)
     t_setup = time.perf_counter() - t0
     info = g.info()
@@ -551,38 +558,36 @@ def mres_cavity(nx, a):
     wall = time.perf_counter() - tw
     ms = e0.elapsed_ms(e1)
     ms_step = ms / steps
-    leaf0, leaf1 = info["leaf"]
+    leaf = info["leaf"]
     # the reference's count (lbmMultiRes.h:385-391): level l's leaves updated
     # 2^(depth-1-l) times per coarse step
-    updates = 2 * leaf0 + leaf1
+    updates = sum(n << (depth - 1 - l) for l, n in enumerate(leaf))
     mlups = updates * steps / (ms * 1e-3) / 1e6
-    emlups = 2 * N ** 3 * steps / (ms * 1e-3) / 1e6
+    emlups = (1 << (depth - 1)) * N ** 3 * steps / (ms * 1e-3) / 1e6
     s_b = STORAGE_BYTES[storage]
     # bytes the kernels must move per coarse step: every pass reads and
-    # writes the 19 populations of a leaf; level 1 collides and streams
-    # (2 passes), level 0 collides, streams fused with the next collide, and
-    # streams (3 passes for its 2 sub-steps; 4 when not fused)
-    fine_passes = 3 if info["fine_fused"] else 4
-    b_step = (leaf0 * fine_passes + leaf1 * 2) * 2 * 19 * s_b
+    # writes the 19 populations of a leaf; a sub-step of level l is one pass
+    # when the level's stream runs fused with its next collide (steady
+    # state inside a call), else two (collide, stream)
+    passes = [(1 if fused else 2) << (depth - 1 - l) for l, fused in enumerate(info["fused_levels"])]
+    b_step = sum(n * p for n, p in zip(leaf, passes)) * 2 * 19 * s_b
     achieved = b_step / (ms_step * 1e-3) / 1e9
     out = {
         "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
         "steps": steps, "warmup": warmup, "ms_per_step": round(ms_step, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": storage,
-        "data": "synthetic (two-level refined lid-driven cavity started at rest)",
-        "config": {"workload": f"two-level refined lid-driven cavity {N}^3 (level 0: sdfCube > {thr}, level 1 "
-                               f"inside), D3Q19 BGK, {storage}, ulb {ulb:.4g}, Re {re:g}, {steps} timed coarse "
-                               f"steps after {warmup}",
-                   "leaves": info["leaf"], "active": info["active"], "blocks": info["blocks"],
+        "data": f"synthetic ({depth}-level refined lid-driven cavity started at rest)",
+        "config": {"workload": f"{depth}-level refined lid-driven cavity {N}^3 (levelSDF {list(sdf)}), D3Q19 BGK, "
+                               f"{storage}, ulb {ulb:.4g}, Re {re:g}, {steps} timed coarse steps after {warmup}",
+                   "leaves": leaf, "active": info["active"], "blocks": info["blocks"],
                    "store_links": info["store_links"], "setup_s": round(t_setup, 2), "omega0": om0,
                    "device": nx.device_name()},
         "mlups_effective": round(emlups, 1),
         "wall_s": round(wall, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": f"coarse step: k_mr_collide + k_mr_stream<{storage}> x {fine_passes} passes over "
-                               f"level 0 and 2 over level 1, 2*19*{s_b} B per leaf per pass, "
-                               f"{b_step / 1e9:.3f} GB"},
+                     "kernel": f"coarse step: k_mr_collide + k_mr_stream<{storage}>, passes per level {passes}, "
+                               f"2*19*{s_b} B per leaf per pass, {b_step / 1e9:.3f} GB"},
         "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)

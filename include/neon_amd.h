@@ -432,11 +432,13 @@ int nx_batch_destroy(nx_batch b);
  *   fine-to-coarse sums are added in the fine voxels' z, y, x order (the
  *   reference adds with atomics in no fixed order), so results are
  *   deterministic.
- * Level 0's stream is fused with its next collide (one pass instead of two,
- * same results) when no bulk leaf of level 0 has a link without a source;
- * nx_mgrid_info reports it (fine_fused, after nx_mgrid_cavity) with the
- * block, active and leaf voxel counts per level (arrays of depth entries)
- * and the number of fine sources of all stores.
+ * A level's stream runs fused with its next collide (one pass instead of
+ * two, same results) when no bulk leaf of the level has a link without a
+ * source; the last stream of each level in a call is not fused, so the
+ * fields between calls are those of the unfused sequence.  nx_mgrid_info
+ * reports the fused levels (fine_fused, bit l for level l, after
+ * nx_mgrid_cavity) with the block, active and leaf voxel counts per level
+ * (arrays of depth entries) and the number of fine sources of all stores.
  * nx_mgrid_fields copies fin (which 0) or fout (which 1) of one level into
  *   a host array [19][Z][Y][X] of that level's dimensions; entries of
  *   inactive voxels are not written.  Synchronises the device.

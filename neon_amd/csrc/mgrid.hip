@@ -122,10 +122,11 @@ struct UpLevel {  // the next coarser level, seen from this one (explosion)
 // accumulated fine populations times its weight); links in state "none"
 // keep the value in `out`.
 // COLLIDE: the pulled populations are collided at once with omega and the
-// result written to `out` -- stream followed by the next collide of the
-// same level (collide.h:8-149) in one pass; the results are those of the two
-// kernels in sequence, bit for bit.  KEEP = false: the level has no link in
-// state "none" (checked on the host), every population is stored.
+// result written to `out`, and refined voxels reset their accumulator in
+// `out` -- stream followed by the next collide of the same level
+// (collide.h:8-149) in one pass; the results are those of the two kernels in
+// sequence, bit for bit.  KEEP = false: the level has no link in state
+// "none" (checked on the host), every population is stored.
 template <int S, bool COLLIDE, bool KEEP>
 __global__ void __launch_bounds__(kBV)
     k_mr_stream(const typename MS<S>::T* __restrict__ fout, typename MS<S>::T* __restrict__ out,
@@ -137,6 +138,16 @@ __global__ void __launch_bounds__(kBV)
     const uint32_t b = blockIdx.x, v = threadIdx.x;
     const Conn cb = load_conn(conn, b);
     const uint32_t st = state[(uint64_t)b * kBV + v];
+    if ((st & kActive) && (st & kRefined) && cls_bits(st) == NX_CELL_BULK) {
+        // a refined voxel: with COLLIDE the collide's reset of its
+        // accumulator (collide.h:131-141); otherwise its fin, which the
+        // reference never writes after the set-up's 0, back to 0 (a fused
+        // pass earlier in the call left an accumulator in this buffer)
+        const uint64_t own = (uint64_t)b * (Q * kBV) + v;
+#pragma unroll
+        for (int q = 0; q < Q; q++) __builtin_nontemporal_store(T(0), &out[own + (uint64_t)q * kBV]);
+        return;
+    }
     if (!bulk_leaf(st)) return;
     const uint64_t lk = links[(uint64_t)b * kBV + v];
     const int x = (int)(v & 7u), y = (int)((v >> 3) & 7u), z = (int)(v >> 6);
@@ -262,7 +273,6 @@ struct nx_mgrid_s {
     int depth = 2;
     nx::mres::Level L[nx::mres::kMaxDepth];
     nx::mres::Store store[nx::mres::kMaxDepth];
-    bool fuse_fine = false;  // level 0 without "none" links: stream fused with the next collide
     bool ready = false;
 };
 
@@ -565,8 +575,12 @@ extern "C" int nx_mgrid_depth(nx_mgrid g, int32_t* depth, int32_t* dims)
 extern "C" int nx_mgrid_info(nx_mgrid g, uint32_t* n_blocks, uint64_t* n_active, uint64_t* n_leaf,
                              uint64_t* n_store, int32_t* fine_fused)
 {
-    if (g && fine_fused) *fine_fused = g->fuse_fine ? 1 : 0;
     if (!g) return set_error(NX_ERR_INVALID, "null mgrid");
+    if (fine_fused) {  // bit l: level l's stream runs fused with its next collide
+        *fine_fused = 0;
+        for (int l = 0; l < g->depth; l++)
+            if (g->ready && !g->L[l].keep) *fine_fused |= 1 << l;
+    }
     uint64_t ns = 0;
     for (int l = 0; l < g->depth; l++) {
         uint64_t a = 0, lf = 0;
@@ -631,7 +645,6 @@ extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
         if (int e = upload((void**)&lv.d_state, st.data(), st.size() * 4)) return e;
         std::vector<uint64_t> lk;
         const uint64_t n_skip = links_of(*g, l, lk);
-        if (l == 0) g->fuse_fine = n_skip == 0;
         lv.keep = n_skip != 0;
         if (int e = upload((void**)&lv.d_links, lk.data(), lk.size() * 8)) return e;
     }
@@ -693,6 +706,27 @@ extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
 
 namespace {
 
+// One coarse step as the sequence of the reference's recursion
+// (lbmMultiRes.h:141-263): per level l from the top, collide l and store
+// l -> l+1, the step of level l-1, stream l, and below the top level the
+// same once more.
+struct Op {
+    uint8_t kind;   // 0 collide, 1 store, 2 stream
+    uint8_t level;
+    bool last;      // the level's last stream of the coarse step
+};
+
+void expand(int l, int depth, std::vector<Op>& ops)
+{
+    for (int half = 0; half < 2; half++) {
+        ops.push_back({0, (uint8_t)l, false});
+        if (l + 1 < depth) ops.push_back({1, (uint8_t)l, false});
+        if (l != 0) expand(l - 1, depth, ops);
+        ops.push_back({2, (uint8_t)l, false});
+        if (l == depth - 1) break;
+    }
+}
+
 template <int S>
 struct Stepper {
     using T = typename MS<S>::T;
@@ -719,59 +753,55 @@ struct Stepper {
             hipLaunchKernelGGL((k_mr_collide<S>), dim3(lv.n_blocks), dim3(kBV), 0, st, (const T*)lv.fin,
                                (T*)lv.fout, lv.d_state, omega(l));
     }
-    void store(int l, const void* fine) const
+    void store(int l) const
     {
         const Store& sl = g.store[l];
         if (l + 1 < g.depth && sl.n)
-            hipLaunchKernelGGL((k_mr_store<S>), dim3((sl.n + 255) / 256), dim3(256), 0, st, (const T*)fine,
+            hipLaunchKernelGGL((k_mr_store<S>), dim3((sl.n + 255) / 256), dim3(256), 0, st, (const T*)g.L[l].fout,
                                (T*)g.L[l + 1].fout, sl.d_target, sl.d_first, sl.d_src, sl.n);
     }
     template <bool COLLIDE, bool KEEP>
-    void stream(int l, const void* src, void* dst) const
-    {
-        const Level& lv = g.L[l];
-        if (lv.n_blocks)
-            hipLaunchKernelGGL((k_mr_stream<S, COLLIDE, KEEP>), dim3(lv.n_blocks), dim3(kBV), 0, st, (const T*)src,
-                               (T*)dst, lv.d_state, lv.d_links, lv.d_conn, lv.d_origins, (const float*)lv.d_ss,
-                               up(l), COLLIDE ? omega(l) : 0.0);
-    }
     void stream(int l) const
     {
         const Level& lv = g.L[l];
-        if (lv.keep)
-            stream<false, true>(l, lv.fout, lv.fin);
-        else
-            stream<false, false>(l, lv.fout, lv.fin);
+        if (lv.n_blocks)
+            hipLaunchKernelGGL((k_mr_stream<S, COLLIDE, KEEP>), dim3(lv.n_blocks), dim3(kBV), 0, st,
+                               (const T*)lv.fout, (T*)lv.fin, lv.d_state, lv.d_links, lv.d_conn, lv.d_origins,
+                               (const float*)lv.d_ss, up(l), COLLIDE ? omega(l) : 0.0);
     }
-    // nonUniformTimestepRecursive (lbmMultiRes.h:141-263)
-    void level(int l) const
+
+    // n coarse steps.  A level's stream and its next collide touch no other
+    // level and nothing of the level runs between them (the operations in
+    // between are the coarser levels' stream, collide and store), so on a
+    // level without links in state "none" they run as one pass (fout -> fin
+    // buffer, then the two buffers trade roles) and the collide is skipped
+    // when its turn comes; the store after it then reads the fused pass's
+    // output.  The last stream of each level in the call is not fused, so
+    // the fields between calls are those of the unfused sequence.
+    void run(const std::vector<Op>& ops, int n) const
     {
-        Level& lv = g.L[l];
-        if (l == 0 && g.fuse_fine) {
-            // level 0, two sub-steps: collide (fin -> fout), store, then
-            // stream fused with the second collide (fout -> fin buffer, which
-            // now holds post-collision values), store, stream (-> the old
-            // fout buffer); afterwards the two buffers have traded roles.
-            // Only when no bulk leaf of level 0 has a link that keeps its
-            // previous value (state "none"): that value is gone once the
-            // fused pass has written.
-            collide(0);
-            store(0, lv.fout);
-            stream<true, false>(0, lv.fout, lv.fin);
-            store(0, lv.fin);
-            stream<false, false>(0, lv.fin, lv.fout);
-            std::swap(lv.fin, lv.fout);
-            return;
-        }
-        collide(l);
-        store(l, lv.fout);
-        if (l != 0) level(l - 1);
-        stream(l);
-        if (l == g.depth - 1) return;
-        collide(l);
-        store(l, lv.fout);
-        if (l != 0) level(l - 1);
-        stream(l);
+        bool done[kMaxDepth] = {};  // the level's next collide already ran in a fused pass
+        for (int k = 0; k < n; k++)
+            for (const Op& op : ops) {
+                const int l = op.level;
+                Level& lv = g.L[l];
+                if (op.kind == 0) {
+                    if (done[l])
+                        done[l] = false;
+                    else
+                        collide(l);
+                } else if (op.kind == 1) {
+                    store(l);
+                } else if (!lv.keep && !(op.last && k == n - 1)) {
+                    stream<true, false>(l);
+                    std::swap(lv.fin, lv.fout);
+                    done[l] = true;
+                } else if (lv.keep) {
+                    stream<false, true>(l);
+                } else {
+                    stream<false, false>(l);
+                }
+            }
     }
 };
 
@@ -782,13 +812,19 @@ extern "C" int nx_mgrid_step(nx_mgrid g, double omega0, int32_t n_steps, nx_stre
     if (!g || n_steps < 0) return set_error(NX_ERR_INVALID, "nx_mgrid_step: bad arguments");
     if (!g->ready) return set_error(NX_ERR_NOT_INIT, "nx_mgrid_step: call nx_mgrid_cavity first");
     if (!(omega0 > 0.0 && omega0 < 2.0)) return set_error(NX_ERR_INVALID, "nx_mgrid_step: omega0 in (0, 2)");
-    for (int k = 0; k < n_steps; k++) {
-        if (g->storage == NX_F64)
-            Stepper<NX_F64>{*g, omega0, (hipStream_t)s}.level(g->depth - 1);
-        else
-            Stepper<NX_F32>{*g, omega0, (hipStream_t)s}.level(g->depth - 1);
-        NX_CHECK_LAUNCH("mgrid step");
-    }
+    std::vector<Op> ops;
+    expand(g->depth - 1, g->depth, ops);
+    for (int l = 0; l < g->depth; l++)
+        for (size_t i = ops.size(); i-- > 0;)
+            if (ops[i].kind == 2 && ops[i].level == l) {
+                ops[i].last = true;
+                break;
+            }
+    if (g->storage == NX_F64)
+        Stepper<NX_F64>{*g, omega0, (hipStream_t)s}.run(ops, n_steps);
+    else
+        Stepper<NX_F32>{*g, omega0, (hipStream_t)s}.run(ops, n_steps);
+    NX_CHECK_LAUNCH("mgrid step");
     return NX_OK;
 }
 

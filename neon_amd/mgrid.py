@@ -140,7 +140,8 @@ class MGrid:
         ns, fused = C.c_uint64(), C.c_int32()
         check(lib.nx_mgrid_info(self.h, nb, na, nl, C.byref(ns), C.byref(fused)))
         return {"blocks": list(nb), "active": list(na), "leaf": list(nl), "store_links": int(ns.value),
-                "fine_fused": bool(fused.value)}
+                "fine_fused": bool(fused.value & 1),
+                "fused_levels": [bool((fused.value >> l) & 1) for l in range(D)]}
 
     def setup_cavity(self, ulb: float) -> None:
         self._init()

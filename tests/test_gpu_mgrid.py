@@ -225,3 +225,27 @@ def test_mgrid_160_three_levels_centerline(nx):
           f"at y = {int(dev.argmax())}, mean {dev.mean():.4f}; positions per level {used}")
     assert all(used)
     assert dev.max() < 0.025
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_mgrid_split_calls_equal_one_call(nx, orc, storage):
+    """Inside a call each level's stream runs fused with its next collide; the
+    last stream of each level in a call is not, so runs split over several
+    calls (1 + 3 + 4 coarse steps) give the oracle's fields bit for bit at
+    every boundary, as one call of 8 does."""
+    dim, sdf, re = (37, 30, 41), (0.0, -0.25, -0.5, -1.0), 30.0
+    lams = list(mg.cavity_lambdas(dim, sdf))
+    ulb, om0 = mg.cavity_parameters(dim, re, storage, depth=3)
+    g = mg.MGrid(dim, lams, storage)
+    g.setup_cavity(ulb)
+    assert g.info()["fused_levels"] == [True, True, True]
+    m = orc.MresOracle(dim, lams, orc.F64 if storage == "f64" else orc.F32)
+    m.setup_cavity(ulb)
+    for n in (1, 3, 4):
+        g.run(n, om0)
+        m.run(n, om0)
+        for lvl in range(3):
+            act = np.tile(m.active(lvl), 19)
+            for which, ref in ((0, m.fin[lvl]), (1, m.fout[lvl])):
+                got = g.fields(lvl, which)
+                assert np.array_equal(got[act].view(np.uint8), ref[act].view(np.uint8)), (n, lvl, which)
