@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(kBV)
                  const uint32_t* __restrict__ state, double omega)
 {
     using T = typename MS<S>::T;
-    const uint32_t b = blockIdx.x, v = threadIdx.x;
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x), v = threadIdx.x;  // neighbour blocks share an XCD's L2
     const uint32_t st = state[(uint64_t)b * kBV + v];
     if (!(st & kActive) || cls_bits(st) != NX_CELL_BULK) return;
     const uint64_t own = (uint64_t)b * (Q * kBV) + v;
@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(kBV)
                 const float* __restrict__ ss, UpLevel up, double omega)
 {
     using T = typename MS<S>::T;
-    const uint32_t b = blockIdx.x, v = threadIdx.x;
+    const uint32_t b = xcd_remap(blockIdx.x, gridDim.x), v = threadIdx.x;  // neighbour blocks share an XCD's L2
     const Conn cb = load_conn(conn, b);
     const uint32_t st = state[(uint64_t)b * kBV + v];
     if ((st & kActive) && (st & kRefined) && cls_bits(st) == NX_CELL_BULK) {
