@@ -101,12 +101,15 @@ void or_run_block(const or_block* d, void* pop0, void* pop1, const uint32_t* fla
 /* grid refinement over `depth` levels (mres_oracle.c); per-level arrays are
  * indexed by level, dims[3 * l + axis] */
 #define OR_MRES_MAX_DEPTH 5
+#define OR_MRES_BGK19 0 /* lattice and collision: BGK D3Q19 */
+#define OR_MRES_KBC27 1 /* KBC D3Q27 */
 void or_mres_dims(const int32_t dim[3], int depth, int32_t* dims);
 int or_mres_build(const int32_t dim[3], int depth, const uint8_t* const* lam, uint8_t* const* st);
-void or_mres_setup(const int32_t dim[3], int depth, int storage, double ulb, uint8_t* const* st,
+void or_mres_setup(const int32_t dim[3], int depth, int lat, int storage, double ulb, uint8_t* const* st,
                    void* const* fin, void* const* fout, float* const* ss);
 double or_mres_omega(double omega0, int level, int num_levels);
-void or_mres_step(const int32_t dim[3], int depth, int s, const uint8_t* const* st, void* const* fin,
+void or_mres_step(const int32_t dim[3], int depth, int lat, int s, const uint8_t* const* st, void* const* fin,
                   void* const* fout, const float* const* ss, double omega0);
+void or_kbc27_collide(int storage, const void* fin, void* fout, double omega);
 
 #endif

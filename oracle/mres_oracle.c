@@ -5,6 +5,10 @@
  * TEST INFRASTRUCTURE ONLY: the checker for the library's nx_mgrid_* path.
  * Nothing in the product links or calls this file.
  *
+ * Two lattices: BGK D3Q19 (the app's BGK variant, the cavity benchmark's
+ * lattice and arithmetic) and KBC D3Q27 (its default; collide.h:148-284,
+ * lattice.h:14-106), selected by `lat` (OR_MRES_BGK19 / OR_MRES_KBC27).
+ *
  * Level 0 is the finest level (the domain dim[3]); level l has voxels of
  * edge 2^l and ceil(dim / 2^l) of them per axis; a level-(l+1) voxel v
  * covers the level-l voxels 2v + {0,1}^3.  Storage is dense per level, SoA
@@ -44,6 +48,34 @@ static const int MC[19][3] = {{-1, 0, 0}, {0, -1, 0}, {0, 0, -1}, {-1, -1, 0}, {
                               {1, 0, 1},   {1, 0, -1},  {0, 1, 1},   {0, 1, -1}};
 static const int MOPP[19] = {10, 11, 12, 13, 14, 15, 16, 17, 18, 9, 0, 1, 2, 3, 4, 5, 6, 7, 8};
 static double mw(int q) { return q == 9 ? 1. / 3. : ((q % 10) < 3 ? 1. / 18. : 1. / 36.); }
+
+/* D3Q27 of apps/lbmMultiRes/lattice.h:14-75 (the KBC lattice): z fastest,
+ * then y, then x, each over 0, -1, +1 */
+static const int KC[27][3] = {
+    {0, 0, 0},    {0, 0, -1},  {0, 0, 1},   {0, -1, 0},  {0, -1, -1}, {0, -1, 1},  {0, 1, 0},
+    {0, 1, -1},   {0, 1, 1},   {-1, 0, 0},  {-1, 0, -1}, {-1, 0, 1},  {-1, -1, 0}, {-1, -1, -1},
+    {-1, -1, 1},  {-1, 1, 0},  {-1, 1, -1}, {-1, 1, 1},  {1, 0, 0},   {1, 0, -1},  {1, 0, 1},
+    {1, -1, 0},   {1, -1, -1}, {1, -1, 1},  {1, 1, 0},   {1, 1, -1},  {1, 1, 1}};
+static const int KOPP[27] = {0,  2,  1,  6,  8,  7,  3,  5,  4,  18, 20, 19, 24, 26,
+                             25, 21, 23, 22, 9,  11, 10, 15, 17, 16, 12, 14, 13};
+/* weights by the number of nonzero velocity components (lattice.h:47-75) */
+static double kw(int q)
+{
+    const int n = (KC[q][0] != 0) + (KC[q][1] != 0) + (KC[q][2] != 0);
+    return n == 0 ? 8.0 / 27.0 : n == 1 ? 2.0 / 27.0 : n == 2 ? 1.0 / 54.0 : 1.0 / 216.0;
+}
+
+typedef struct {
+    int Q;
+    int rest; /* index of the rest population */
+    const int (*c)[3];
+    const int* opp;
+    double (*w)(int);
+} mr_lat;
+
+static const mr_lat LAT19 = {19, 9, MC, MOPP, mw};
+static const mr_lat LAT27 = {27, 0, KC, KOPP, kw};
+static const mr_lat* lat_of(int lat) { return lat == OR_MRES_KBC27 ? &LAT27 : &LAT19; }
 
 /* per-voxel state bits */
 #define MR_ACTIVE 1u
@@ -250,12 +282,15 @@ static inline double get(const void* p, int s, size_t i)
  * bounce-back, those with y = Y-1 the moving lid; leaf voxels hold w_q
  * (bulk), 0 (bounce-back) or -6 w_q (c_q . ulid) (lid), both fields.
  * ulid = (ulb, 0, 0).  sumStore of levels 1.. after init.h:8-86: every
- * active voxel of level l with a parent counts, for each q != 9, one towards
+ * active voxel of level l with a parent counts, for each moving q, one towards
  * the level-(l+1) voxel that will later pull its population q, then
- * ss = 1 / (2 count).  ss[0] is not used. */
-void or_mres_setup(const int32_t dim[3], int depth, int storage, double ulb, uint8_t* const* st,
+ * ss = 1 / (2 count).  ss[0] is not used.  Arrays hold Q populations per
+ * voxel, Q = 19 or 27 after the lattice. */
+void or_mres_setup(const int32_t dim[3], int depth, int lat, int storage, double ulb, uint8_t* const* st,
                    void* const* fin, void* const* fout, float* const* ss)
 {
+    const mr_lat* L = lat_of(lat);
+    const int Q = L->Q;
     int32_t dims[3 * OR_MRES_MAX_DEPTH];
     or_mres_dims(dim, depth, dims);
     for (int lvl = 0; lvl < depth; lvl++) {
@@ -267,7 +302,7 @@ void or_mres_setup(const int32_t dim[3], int depth, int storage, double ulb, uin
                     const size_t i = lin(d, x, y, z);
                     if (!(st[lvl][i] & MR_ACTIVE)) continue;
                     int cls = OR_BULK;
-                    for (int q = 0; q < 19; q++) {
+                    for (int q = 0; q < Q; q++) {
                         put(fin[lvl], storage, (size_t)q * n + i, 0.);
                         put(fout[lvl], storage, (size_t)q * n + i, 0.);
                     }
@@ -277,28 +312,28 @@ void or_mres_setup(const int32_t dim[3], int depth, int storage, double ulb, uin
                             cls = OR_BOUNCEBACK;
                             if (y == d[1] - 1) cls = OR_MOVINGWALL;
                         }
-                        for (int q = 0; q < 19; q++) {
+                        for (int q = 0; q < Q; q++) {
                             double v;
                             if (storage == OR_F64) {
-                                double p = mw(q);
+                                double p = L->w(q);
                                 if (cls == OR_BOUNCEBACK) p = 0;
                                 if (cls == OR_MOVINGWALL) {
                                     p = 0;
-                                    p += (double)MC[q][0] * ulb;
-                                    p += (double)MC[q][1] * 0.;
-                                    p += (double)MC[q][2] * 0.;
-                                    p *= -6. * mw(q);
+                                    p += (double)L->c[q][0] * ulb;
+                                    p += (double)L->c[q][1] * 0.;
+                                    p += (double)L->c[q][2] * 0.;
+                                    p *= -6. * L->w(q);
                                 }
                                 v = p;
                             } else {
-                                float p = (float)mw(q);
+                                float p = (float)L->w(q);
                                 if (cls == OR_BOUNCEBACK) p = 0;
                                 if (cls == OR_MOVINGWALL) {
                                     p = 0;
-                                    p = (float)((double)p + (double)MC[q][0] * ulb);
-                                    p = (float)((double)p + (double)MC[q][1] * 0.);
-                                    p = (float)((double)p + (double)MC[q][2] * 0.);
-                                    p = (float)((double)p * (-6. * mw(q)));
+                                    p = (float)((double)p + (double)L->c[q][0] * ulb);
+                                    p = (float)((double)p + (double)L->c[q][1] * 0.);
+                                    p = (float)((double)p + (double)L->c[q][2] * 0.);
+                                    p = (float)((double)p * (-6. * L->w(q)));
                                 }
                                 v = (double)p;
                             }
@@ -311,7 +346,7 @@ void or_mres_setup(const int32_t dim[3], int depth, int storage, double ulb, uin
     }
     /* sumStore counts (init.h:8-55), level l -> l + 1: no test of the fine
      * voxel's children or of the coarse target's child */
-    for (int l = 1; l < depth; l++) memset(ss[l], 0, 19 * nvox(dims + 3 * l) * sizeof(float));
+    for (int l = 1; l < depth; l++) memset(ss[l], 0, (size_t)Q * nvox(dims + 3 * l) * sizeof(float));
     for (int l = 0; l + 1 < depth; l++) {
         const int32_t* d = dims + 3 * l;
         const int32_t* du = dims + 3 * (l + 1);
@@ -324,14 +359,14 @@ void or_mres_setup(const int32_t dim[3], int depth, int storage, double ulb, uin
                     if (!(st[l][lin(d, x, y, z)] & MR_ACTIVE)) continue;
                     const int px = x >> 1, py = y >> 1, pz = z >> 1;
                     if (!parent_block(nbu, blk, px, py, pz)) continue; /* hasParent */
-                    for (int q = 0; q < 19; q++) {
-                        if (q == 9) continue;
-                        const int ux = uoff(x, MC[q][0]), uy = uoff(y, MC[q][1]), uz = uoff(z, MC[q][2]);
+                    for (int q = 0; q < Q; q++) {
+                        if (q == L->rest) continue;
+                        const int ux = uoff(x, L->c[q][0]), uy = uoff(y, L->c[q][1]), uz = uoff(z, L->c[q][2]);
                         /* the same-level voxel along the uncle offset: a proxy
                          * for "the uncle's region is not refined" */
                         if (act(d, st[l], x + ux, y + uy, z + uz)) continue;
                         if (!act(du, st[l + 1], px + ux, py + uy, pz + uz)) continue;
-                        const int cx = px + ux - MC[q][0], cy = py + uy - MC[q][1], cz = pz + uz - MC[q][2];
+                        const int cx = px + ux - L->c[q][0], cy = py + uy - L->c[q][1], cz = pz + uz - L->c[q][2];
                         if (!act(du, st[l + 1], cx, cy, cz)) continue;
                         ss[l + 1][(size_t)q * nu + lin(du, cx, cy, cz)] += 1.f;
                     }
@@ -341,7 +376,7 @@ void or_mres_setup(const int32_t dim[3], int depth, int storage, double ulb, uin
     /* reciprocal (init.h:58-80): 1.f / (refFactor * ss) with refFactor of
      * the field's type T */
     for (int l = 1; l < depth; l++)
-        for (size_t i = 0; i < 19 * nvox(dims + 3 * l); i++)
+        for (size_t i = 0; i < (size_t)Q * nvox(dims + 3 * l); i++)
             if ((double)ss[l][i] > 0.001) {
                 if (storage == OR_F64)
                     ss[l][i] = (float)(1.0 / (2.0 * (double)ss[l][i]));
@@ -359,11 +394,142 @@ double or_mres_omega(double omega0, int level, int num_levels)
 
 /* ---- step --------------------------------------------------------------- */
 
-/* BGK of collide.h:8-149 in T = double / float: the moment sums and the
- * collision in the cavity benchmark's association order, all constants of
- * type T.  Refined voxels of levels above 0 reset their fout (the
- * accumulator of storeFine) to 0. */
-static void collide_level(const int32_t d[3], int lvl, int s, const uint8_t* st, const void* fin,
+/* BGK D3Q19 of collide.h:8-149 on voxel i (n voxels per population) in
+ * T = double / float: the moment sums and the collision in the cavity
+ * benchmark's association order, all constants of type T. */
+static void bgk19_voxel(int s, size_t n, size_t i, const void* fin, void* fout, double omega)
+{
+    if (s == OR_F64) {
+        double f[19], o[19];
+        for (int q = 0; q < 19; q++) f[q] = ((const double*)fin)[(size_t)q * n + i];
+        const double XM1 = f[0] + f[3] + f[4] + f[5] + f[6];
+        const double XP1 = f[10] + f[13] + f[14] + f[15] + f[16];
+        const double X0 = f[9] + f[1] + f[2] + f[7] + f[8] + f[11] + f[12] + f[17] + f[18];
+        const double YM1 = f[1] + f[3] + f[7] + f[8] + f[14];
+        const double YP1 = f[4] + f[11] + f[13] + f[17] + f[18];
+        const double ZM1 = f[2] + f[5] + f[7] + f[16] + f[18];
+        const double ZP1 = f[6] + f[8] + f[12] + f[15] + f[17];
+        const double rho = XM1 + XP1 + X0;
+        const double u0 = (XP1 - XM1) / rho, u1 = (YP1 - YM1) / rho, u2 = (ZP1 - ZM1) / rho;
+        const double usqr = 1.5 * (u0 * u0 + u1 * u1 + u2 * u2);
+        const double ck[9] = {u0, u1, u2, u0 + u1, u0 - u1, u0 + u2, u0 - u2, u1 + u2, u1 - u2};
+        for (int k = 0; k < 9; k++) {
+            const double w = k < 3 ? 1. / 18. : 1. / 36.;
+            const double eq = rho * w * (1. - 3. * ck[k] + 4.5 * ck[k] * ck[k] - usqr);
+            const double eqo = eq + rho * w * 6. * ck[k];
+            o[k] = (1. - omega) * f[k] + omega * eq;
+            o[k + 10] = (1. - omega) * f[k + 10] + omega * eqo;
+        }
+        o[9] = (1. - omega) * f[9] + omega * (rho * (1. / 3.) * (1. - usqr));
+        for (int q = 0; q < 19; q++) ((double*)fout)[(size_t)q * n + i] = o[q];
+    } else {
+        const float om = (float)omega;
+        float f[19], o[19];
+        for (int q = 0; q < 19; q++) f[q] = ((const float*)fin)[(size_t)q * n + i];
+        const float XM1 = f[0] + f[3] + f[4] + f[5] + f[6];
+        const float XP1 = f[10] + f[13] + f[14] + f[15] + f[16];
+        const float X0 = f[9] + f[1] + f[2] + f[7] + f[8] + f[11] + f[12] + f[17] + f[18];
+        const float YM1 = f[1] + f[3] + f[7] + f[8] + f[14];
+        const float YP1 = f[4] + f[11] + f[13] + f[17] + f[18];
+        const float ZM1 = f[2] + f[5] + f[7] + f[16] + f[18];
+        const float ZP1 = f[6] + f[8] + f[12] + f[15] + f[17];
+        const float rho = XM1 + XP1 + X0;
+        const float u0 = (XP1 - XM1) / rho, u1 = (YP1 - YM1) / rho, u2 = (ZP1 - ZM1) / rho;
+        const float usqr = 1.5f * (u0 * u0 + u1 * u1 + u2 * u2);
+        const float ck[9] = {u0, u1, u2, u0 + u1, u0 - u1, u0 + u2, u0 - u2, u1 + u2, u1 - u2};
+        for (int k = 0; k < 9; k++) {
+            const float w = k < 3 ? (float)(1. / 18.) : (float)(1. / 36.);
+            const float eq = rho * w * (1.f - 3.f * ck[k] + 4.5f * ck[k] * ck[k] - usqr);
+            const float eqo = eq + rho * w * 6.f * ck[k];
+            o[k] = (1.f - om) * f[k] + om * eq;
+            o[k + 10] = (1.f - om) * f[k + 10] + om * eqo;
+        }
+        o[9] = (1.f - om) * f[9] + om * (rho * (float)(1. / 3.) * (1.f - usqr));
+        for (int q = 0; q < 19; q++) ((float*)fout)[(size_t)q * n + i] = o[q];
+    }
+}
+
+/* KBC D3Q27 of collide.h:148-284 on voxel i, for T = double and float with
+ * the reference's type promotions: velocity() sums f c over q in order
+ * (util.h:47-62), the equilibrium and the shear split evaluate their double
+ * literals in binary64 and store in T, e0 / e1 / gamma / out follow the
+ * source's expressions term by term. */
+#define KBC_BODY(T)                                                                                      \
+    do {                                                                                                 \
+        const T* fi = (const T*)fin;                                                                     \
+        T* fo = (T*)fout;                                                                                \
+        const T om = (T)omega;                                                                           \
+        const T beta = (T)(om * 0.5);                                                                    \
+        const T invBeta = (T)(1.0 / beta);                                                               \
+        const T tiny = (T)1e-7;                                                                          \
+        T ins[27], feq[27], fneq[27], dS[27];                                                            \
+        for (int q = 0; q < 27; q++) ins[q] = fi[(size_t)q * n + i];                                     \
+        T rho = 0;                                                                                       \
+        for (int q = 0; q < 27; q++) rho += ins[q];                                                      \
+        T v[3] = {0, 0, 0};                                                                              \
+        for (int q = 0; q < 27; q++)                                                                     \
+            for (int d = 0; d < 3; d++) v[d] += ins[q] * (T)KC[q][d];                                    \
+        for (int d = 0; d < 3; d++) v[d] /= rho;                                                         \
+        const T usqr = (T)((3.0 / 2.0) * (double)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));            \
+        for (int q = 0; q < 27; q++) {                                                                   \
+            T cu = 0;                                                                                    \
+            for (int d = 0; d < 3; d++) cu += (T)KC[q][d] * v[d];                                        \
+            cu = (T)((double)cu * 3.0);                                                                  \
+            feq[q] = (T)((double)rho * kw(q) *                                                           \
+                         (1. + (double)cu + 0.5 * (double)cu * (double)cu - (double)usqr));              \
+            fneq[q] = ins[q] - feq[q];                                                                   \
+        }                                                                                                \
+        T Pi[6] = {0, 0, 0, 0, 0, 0};                                                                    \
+        for (int q = 0; q < 27; q++) {                                                                   \
+            const int m[6] = {KC[q][0] * KC[q][0], KC[q][0] * KC[q][1], KC[q][0] * KC[q][2],             \
+                              KC[q][1] * KC[q][1], KC[q][1] * KC[q][2], KC[q][2] * KC[q][2]};            \
+            for (int k = 0; k < 6; k++) Pi[k] += fneq[q] * (T)m[k];                                      \
+        }                                                                                                \
+        const T Nxz = Pi[0] - Pi[5], Nyz = Pi[3] - Pi[5];                                                \
+        T e0 = 0, e1 = 0;                                                                                \
+        for (int q = 0; q < 27; q++) {                                                                   \
+            T sh;                                                                                        \
+            const int cx = KC[q][0], cy = KC[q][1], cz = KC[q][2];                                       \
+            const int nz = (cx != 0) + (cy != 0) + (cz != 0);                                            \
+            if (nz == 1 && cx)                                                                           \
+                sh = (T)((2.0 * (double)Nxz - (double)Nyz) / 6.0);                                       \
+            else if (nz == 1 && cy)                                                                      \
+                sh = (T)((-(double)Nxz + 2.0 * (double)Nyz) / 6.0);                                      \
+            else if (nz == 1 && cz)                                                                      \
+                sh = (T)((double)(-Nxz - Nyz) / 6.0); /* a T subtraction */                                \
+            else if (nz == 2 && !cz)                                                                     \
+                sh = (T)((double)(cx * cy > 0 ? Pi[1] : -Pi[1]) / 4.0);                                  \
+            else if (nz == 2 && !cy)                                                                     \
+                sh = (T)((double)(cx * cz > 0 ? Pi[2] : -Pi[2]) / 4.0);                                  \
+            else if (nz == 2 && !cx)                                                                     \
+                sh = (T)((double)(cy * cz > 0 ? Pi[4] : -Pi[4]) / 4.0);                                  \
+            else                                                                                         \
+                sh = (T)0;                                                                               \
+            dS[q] = rho * sh;                                                                            \
+            const T dH = fneq[q] - dS[q];                                                                \
+            e0 += (dS[q] * dH / feq[q]);                                                                 \
+            e1 += (dH * dH / feq[q]);                                                                    \
+        }                                                                                                \
+        const T gamma = (T)((double)invBeta - (2.0 - (double)invBeta) * (double)e0 / (double)(tiny + e1)); \
+        for (int q = 0; q < 27; q++) {                                                                   \
+            const T dH = fneq[q] - dS[q];                                                                \
+            fo[(size_t)q * n + i] =                                                                      \
+                (T)((double)ins[q] - (double)beta * (2.0 * (double)dS[q] + (double)(gamma * dH)));      \
+        }                                                                                                \
+    } while (0)
+
+static void kbc27_voxel(int s, size_t n, size_t i, const void* fin, void* fout, double omega)
+{
+    if (s == OR_F64)
+        KBC_BODY(double);
+    else
+        KBC_BODY(float);
+}
+
+/* collide.h:8-284 for every bulk active voxel of a level: the lattice's
+ * collision on leaves; refined voxels of levels above 0 reset their fout
+ * (the accumulator of storeFine) to 0. */
+static void collide_level(const mr_lat* L, const int32_t d[3], int lvl, int s, const uint8_t* st, const void* fin,
                           void* fout, double omega)
 {
     const size_t n = nvox(d);
@@ -376,68 +542,24 @@ static void collide_level(const int32_t d[3], int lvl, int s, const uint8_t* st,
                 if (!(sv & MR_ACTIVE) || ((sv >> MR_CLS_SHIFT) & 3) != OR_BULK) continue;
                 if (sv & MR_REFINED) {
                     if (lvl != 0)
-                        for (int q = 0; q < 19; q++) put(fout, s, (size_t)q * n + i, 0.);
+                        for (int q = 0; q < L->Q; q++) put(fout, s, (size_t)q * n + i, 0.);
                     continue;
                 }
-                if (s == OR_F64) {
-                    double f[19], o[19];
-                    for (int q = 0; q < 19; q++) f[q] = ((const double*)fin)[(size_t)q * n + i];
-                    const double XM1 = f[0] + f[3] + f[4] + f[5] + f[6];
-                    const double XP1 = f[10] + f[13] + f[14] + f[15] + f[16];
-                    const double X0 = f[9] + f[1] + f[2] + f[7] + f[8] + f[11] + f[12] + f[17] + f[18];
-                    const double YM1 = f[1] + f[3] + f[7] + f[8] + f[14];
-                    const double YP1 = f[4] + f[11] + f[13] + f[17] + f[18];
-                    const double ZM1 = f[2] + f[5] + f[7] + f[16] + f[18];
-                    const double ZP1 = f[6] + f[8] + f[12] + f[15] + f[17];
-                    const double rho = XM1 + XP1 + X0;
-                    const double u0 = (XP1 - XM1) / rho, u1 = (YP1 - YM1) / rho, u2 = (ZP1 - ZM1) / rho;
-                    const double usqr = 1.5 * (u0 * u0 + u1 * u1 + u2 * u2);
-                    const double ck[9] = {u0, u1, u2, u0 + u1, u0 - u1, u0 + u2, u0 - u2, u1 + u2, u1 - u2};
-                    for (int k = 0; k < 9; k++) {
-                        const double w = k < 3 ? 1. / 18. : 1. / 36.;
-                        const double eq = rho * w * (1. - 3. * ck[k] + 4.5 * ck[k] * ck[k] - usqr);
-                        const double eqo = eq + rho * w * 6. * ck[k];
-                        o[k] = (1. - omega) * f[k] + omega * eq;
-                        o[k + 10] = (1. - omega) * f[k + 10] + omega * eqo;
-                    }
-                    o[9] = (1. - omega) * f[9] + omega * (rho * (1. / 3.) * (1. - usqr));
-                    for (int q = 0; q < 19; q++) ((double*)fout)[(size_t)q * n + i] = o[q];
-                } else {
-                    const float om = (float)omega;
-                    float f[19], o[19];
-                    for (int q = 0; q < 19; q++) f[q] = ((const float*)fin)[(size_t)q * n + i];
-                    const float XM1 = f[0] + f[3] + f[4] + f[5] + f[6];
-                    const float XP1 = f[10] + f[13] + f[14] + f[15] + f[16];
-                    const float X0 = f[9] + f[1] + f[2] + f[7] + f[8] + f[11] + f[12] + f[17] + f[18];
-                    const float YM1 = f[1] + f[3] + f[7] + f[8] + f[14];
-                    const float YP1 = f[4] + f[11] + f[13] + f[17] + f[18];
-                    const float ZM1 = f[2] + f[5] + f[7] + f[16] + f[18];
-                    const float ZP1 = f[6] + f[8] + f[12] + f[15] + f[17];
-                    const float rho = XM1 + XP1 + X0;
-                    const float u0 = (XP1 - XM1) / rho, u1 = (YP1 - YM1) / rho, u2 = (ZP1 - ZM1) / rho;
-                    const float usqr = 1.5f * (u0 * u0 + u1 * u1 + u2 * u2);
-                    const float ck[9] = {u0, u1, u2, u0 + u1, u0 - u1, u0 + u2, u0 - u2, u1 + u2, u1 - u2};
-                    for (int k = 0; k < 9; k++) {
-                        const float w = k < 3 ? (float)(1. / 18.) : (float)(1. / 36.);
-                        const float eq = rho * w * (1.f - 3.f * ck[k] + 4.5f * ck[k] * ck[k] - usqr);
-                        const float eqo = eq + rho * w * 6.f * ck[k];
-                        o[k] = (1.f - om) * f[k] + om * eq;
-                        o[k + 10] = (1.f - om) * f[k + 10] + om * eqo;
-                    }
-                    o[9] = (1.f - om) * f[9] + om * (rho * (float)(1. / 3.) * (1.f - usqr));
-                    for (int q = 0; q < 19; q++) ((float*)fout)[(size_t)q * n + i] = o[q];
-                }
+                if (L->Q == 27)
+                    kbc27_voxel(s, n, i, fin, fout, omega);
+                else
+                    bgk19_voxel(s, n, i, fin, fout, omega);
             }
 }
 
 /* storeFine (store.h:98-163) from level l into level l + 1: every active
- * voxel of level l without children and with a parent adds, for each q != 9,
- * its post-collision population q to the level-(l+1) voxel that the coarse
- * leaf downstream will pull it from.  Serial, voxels in z, y, x order and q
- * ascending: the order of the additions is the order of this loop (the
- * reference adds with atomics in an unspecified order). */
-static void store_fine(const int32_t* dims, int l, int s, const uint8_t* const* st, const void* fout_l,
-                       void* fout_u)
+ * voxel of level l without children and with a parent adds, for each moving
+ * q, its post-collision population q to the level-(l+1) voxel that the
+ * coarse leaf downstream will pull it from.  Serial, voxels in z, y, x order
+ * and q ascending: the order of the additions is the order of this loop
+ * (the reference adds with atomics in an unspecified order). */
+static void store_fine(const mr_lat* L, const int32_t* dims, int l, int s, const uint8_t* const* st,
+                       const void* fout_l, void* fout_u)
 {
     const int32_t* d = dims + 3 * l;
     const int32_t* du = dims + 3 * (l + 1);
@@ -451,12 +573,13 @@ static void store_fine(const int32_t* dims, int l, int s, const uint8_t* const* 
                 if (!(st[l][i] & MR_ACTIVE) || (st[l][i] & MR_REFINED)) continue;
                 const int px = x >> 1, py = y >> 1, pz = z >> 1;
                 if (!parent_block(nbu, blk, px, py, pz)) continue; /* hasParent */
-                for (int q = 0; q < 19; q++) {
-                    if (q == 9) continue;
-                    const int ux = uoff(x, MC[q][0]), uy = uoff(y, MC[q][1]), uz = uoff(z, MC[q][2]);
+                for (int q = 0; q < L->Q; q++) {
+                    if (q == L->rest) continue;
+                    const int* c = L->c[q];
+                    const int ux = uoff(x, c[0]), uy = uoff(y, c[1]), uz = uoff(z, c[2]);
                     if (act(d, st[l], x + ux, y + uy, z + uz)) continue;
                     if (!act(du, st[l + 1], px + ux, py + uy, pz + uz)) continue; /* uncle */
-                    const int sx = ux - MC[q][0], sy = uy - MC[q][1], sz = uz - MC[q][2];
+                    const int sx = ux - c[0], sy = uy - c[1], sz = uz - c[2];
                     if (!act(du, st[l + 1], px + sx, py + sy, pz + sz)) continue; /* cs */
                     if (!act(d, st[l], x + sx, y + sy, z + sz)) continue;         /* csChild */
                     const size_t t = (size_t)q * nu + lin(du, px + sx, py + sy, pz + sz);
@@ -476,7 +599,7 @@ static void store_fine(const int32_t* dims, int l, int s, const uint8_t* const* 
  * bulk); from the next level's neighbour (explosion) when the neighbour is
  * absent and the voxel has a parent; from a neighbour with children
  * (coalescence) scaled by its sumStore weight. */
-static void stream_level(const int32_t* dims, int depth, int l, int s, const uint8_t* const* st,
+static void stream_level(const mr_lat* L, const int32_t* dims, int depth, int l, int s, const uint8_t* const* st,
                          const void* fout_l, const void* fout_up, const float* ss, void* fin_l)
 {
     const int32_t* d = dims + 3 * l;
@@ -494,8 +617,8 @@ static void stream_level(const int32_t* dims, int depth, int l, int s, const uin
                 const uint8_t sv = st[l][i];
                 if (!(sv & MR_ACTIVE) || ((sv >> MR_CLS_SHIFT) & 3) != OR_BULK || (sv & MR_REFINED)) continue;
                 const int has_parent = !top && parent_block(nbu, blk, x >> 1, y >> 1, z >> 1);
-                for (int q = 0; q < 19; q++) {
-                    const int dx = -MC[q][0], dy = -MC[q][1], dz = -MC[q][2];
+                for (int q = 0; q < L->Q; q++) {
+                    const int dx = -L->c[q][0], dy = -L->c[q][1], dz = -L->c[q][2];
                     const int nx = x + dx, ny = y + dy, nz = z + dz;
                     const int valid = act(d, st[l], nx, ny, nz);
                     const size_t j = valid ? lin(d, nx, ny, nz) : 0;
@@ -505,7 +628,7 @@ static void stream_level(const int32_t* dims, int depth, int l, int s, const uin
                             if (((st[l][j] >> MR_CLS_SHIFT) & 3) == OR_BULK) {
                                 put(fin_l, s, (size_t)q * n + i, get(fout_l, s, (size_t)q * n + j));
                             } else {
-                                const int o = MOPP[q];
+                                const int o = L->opp[q];
                                 if (s == OR_F64)
                                     ((double*)fin_l)[(size_t)q * n + i] =
                                         ((const double*)fout_l)[(size_t)o * n + i] +
@@ -515,14 +638,14 @@ static void stream_level(const int32_t* dims, int depth, int l, int s, const uin
                                         ((const float*)fout_l)[(size_t)o * n + i] +
                                         ((const float*)fout_l)[(size_t)o * n + j];
                             }
-                        } else if (has_parent && q != 9) {
+                        } else if (has_parent && q != L->rest) {
                             const int ux = (x >> 1) + uoff(x, dx), uy = (y >> 1) + uoff(y, dy),
                                       uz = (z >> 1) + uoff(z, dz);
                             if (act(du, st[l + 1], ux, uy, uz))
                                 put(fin_l, s, (size_t)q * n + i,
                                     get(fout_up, s, (size_t)q * nu + lin(du, ux, uy, uz)));
                         }
-                    } else if (q != 9) {
+                    } else if (q != L->rest) {
                         const float w = ss[(size_t)q * n + j];
                         if (s == OR_F64)
                             ((double*)fin_l)[(size_t)q * n + i] =
@@ -536,6 +659,7 @@ static void stream_level(const int32_t* dims, int depth, int l, int s, const uin
 }
 
 typedef struct {
+    const mr_lat* L;
     const int32_t* dims;
     int depth, s;
     const uint8_t* const* st;
@@ -547,14 +671,14 @@ typedef struct {
 
 static void collide_store(const mr_ctx* c, int l)
 {
-    collide_level(c->dims + 3 * l, l, c->s, c->st[l], c->fin[l], c->fout[l],
+    collide_level(c->L, c->dims + 3 * l, l, c->s, c->st[l], c->fin[l], c->fout[l],
                   or_mres_omega(c->omega0, l, c->depth));
-    if (l != c->depth - 1) store_fine(c->dims, l, c->s, c->st, c->fout[l], c->fout[l + 1]);
+    if (l != c->depth - 1) store_fine(c->L, c->dims, l, c->s, c->st, c->fout[l], c->fout[l + 1]);
 }
 
 static void stream_of(const mr_ctx* c, int l)
 {
-    stream_level(c->dims, c->depth, l, c->s, c->st, c->fout[l], l + 1 < c->depth ? c->fout[l + 1] : NULL,
+    stream_level(c->L, c->dims, c->depth, l, c->s, c->st, c->fout[l], l + 1 < c->depth ? c->fout[l + 1] : NULL,
                  c->ss[l], c->fin[l]);
 }
 
@@ -572,11 +696,18 @@ static void recurse(const mr_ctx* c, int l)
 }
 
 /* One step of the coarsest level; omega0 is its relaxation. */
-void or_mres_step(const int32_t dim[3], int depth, int s, const uint8_t* const* st, void* const* fin,
+void or_mres_step(const int32_t dim[3], int depth, int lat, int s, const uint8_t* const* st, void* const* fin,
                   void* const* fout, const float* const* ss, double omega0)
 {
     int32_t dims[3 * OR_MRES_MAX_DEPTH];
     or_mres_dims(dim, depth, dims);
-    const mr_ctx c = {dims, depth, s, st, fin, fout, ss, omega0};
+    const mr_ctx c = {lat_of(lat), dims, depth, s, st, fin, fout, ss, omega0};
     recurse(&c, depth - 1);
+}
+
+/* The KBC D3Q27 collision of one voxel alone (for its closed-form checks):
+ * fin, fout hold 27 values of the storage type. */
+void or_kbc27_collide(int storage, const void* fin, void* fout, double omega)
+{
+    kbc27_voxel(storage, 1, 0, fin, fout, omega);
 }

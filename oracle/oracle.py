@@ -71,9 +71,10 @@ _SIG = {
     "or_run_block": (None, [_P, _P, _P, _P, _D, _P, C.c_int]),
     "or_mres_dims": (None, [_P, C.c_int, _P]),
     "or_mres_build": (C.c_int, [_P, C.c_int, _P, _P]),
-    "or_mres_setup": (None, [_P, C.c_int, C.c_int, _D, _P, _P, _P, _P]),
+    "or_mres_setup": (None, [_P, C.c_int, C.c_int, C.c_int, _D, _P, _P, _P, _P]),
     "or_mres_omega": (_D, [_D, C.c_int, C.c_int]),
-    "or_mres_step": (None, [_P, C.c_int, C.c_int, _P, _P, _P, _P, _D]),
+    "or_mres_step": (None, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _D]),
+    "or_kbc27_collide": (None, [C.c_int, _P, _P, _D]),
 }
 
 
@@ -282,6 +283,22 @@ class BlockOracle:
 
 
 MR_ACTIVE, MR_REFINED, MR_CLS_SHIFT = 1, 2, 2
+BGK19, KBC27 = 0, 1  # lattice and collision of the refinement oracle
+
+# D3Q27 of apps/lbmMultiRes/lattice.h:14-75 (z fastest, then y, then x, each
+# over 0, -1, +1)
+D3Q27_C = np.array([(x, y, z) for x in (0, -1, 1) for y in (0, -1, 1) for z in (0, -1, 1)], np.int64)
+
+
+def kbc27_collide(f, omega, storage=F64):
+    """The oracle's KBC D3Q27 collision of single voxels: f [..., 27]."""
+    dt = NP_STORAGE[storage]
+    f = np.ascontiguousarray(f, dt)
+    out = np.empty_like(f)
+    flat_in, flat_out = f.reshape(-1, 27), out.reshape(-1, 27)
+    for k in range(flat_in.shape[0]):
+        lib.or_kbc27_collide(storage, _ptr(flat_in[k]), _ptr(flat_out[k]), float(omega))
+    return out
 
 
 class MresOracle:
@@ -293,12 +310,14 @@ class MresOracle:
     [19][z][y][x]; st[l] the per-voxel state bytes (st0 / st1, dim1 name
     levels 0 and 1)."""
 
-    def __init__(self, dim, lams, *rest):
+    def __init__(self, dim, lams, *rest, lattice=BGK19):
         if isinstance(lams, (list, tuple)):
             storage = rest[0] if rest else F64
         else:  # (dim, lam0, lam1[, storage])
             lams = [lams, rest[0]]
             storage = rest[1] if len(rest) > 1 else F64
+        self.lattice = lattice
+        self.Q = 27 if lattice == KBC27 else 19
         self.depth = len(lams)
         assert 2 <= self.depth <= 5
         self.dim = tuple(int(v) for v in dim)
@@ -336,11 +355,11 @@ class MresOracle:
 
     def setup_cavity(self, ulb):
         dt = NP_STORAGE[self.storage]
-        self.fin = [np.zeros(19 * n, dt) for n in self.n]
-        self.fout = [np.zeros(19 * n, dt) for n in self.n]
-        self.ss = [None] + [np.zeros(19 * n, np.float32) for n in self.n[1:]]
+        self.fin = [np.zeros(self.Q * n, dt) for n in self.n]
+        self.fout = [np.zeros(self.Q * n, dt) for n in self.n]
+        self.ss = [None] + [np.zeros(self.Q * n, np.float32) for n in self.n[1:]]
         self.ss1 = self.ss[1]
-        lib.or_mres_setup(self._d0, self.depth, self.storage, float(ulb), self._ptrs(self.st),
+        lib.or_mres_setup(self._d0, self.depth, self.lattice, self.storage, float(ulb), self._ptrs(self.st),
                           self._ptrs(self.fin), self._ptrs(self.fout), self._ptrs(self.ss))
 
     def classes(self, level):
@@ -349,7 +368,7 @@ class MresOracle:
     def run(self, n, omega0):
         st, fi, fo, ss = self._ptrs(self.st), self._ptrs(self.fin), self._ptrs(self.fout), self._ptrs(self.ss)
         for _ in range(int(n)):
-            lib.or_mres_step(self._d0, self.depth, self.storage, st, fi, fo, ss, float(omega0))
+            lib.or_mres_step(self._d0, self.depth, self.lattice, self.storage, st, fi, fo, ss, float(omega0))
 
     @staticmethod
     def omega(omega0, level, num_levels=2):
