@@ -395,7 +395,8 @@ int nx_batch_autotune(nx_batch b, nx_stream s, int32_t* best_wg, int32_t* best_w
                       float* best_ms);
 int nx_batch_destroy(nx_batch b);
 
-/* ---- grid refinement (Neon's mGrid with depth 2 to 5, BGK D3Q19) -------
+/* ---- grid refinement (Neon's mGrid with depth 2 to 5, BGK D3Q19 or KBC
+ * D3Q27) ----------------------------------------------------------------------
  * Level 0 is the finest level (the domain dim[3]); level l has voxels of
  * edge 2^l, ceil(dim / 2^l) of them per axis, and a level-(l+1) voxel v
  * covers the level-l voxels 2v + {0,1}^3.  Each level is stored as a bGrid of
@@ -409,7 +410,11 @@ int nx_batch_destroy(nx_batch b);
  *   neighbours are all refined are removed; the strong-balance pass then
  *   activates the voxels that keep every neighbour of an active voxel of
  *   level l covered on level l or l + 1.  storage NX_F64 or NX_F32.
- * nx_mgrid_create: the same for depth 2 (lam0, lam1).
+ *   lattice NX_MGRID_BGK19 (collide.h:8-149 on the D3Q19 of
+ *   lattice.h:108-173) or NX_MGRID_KBC27 (collide.h:148-284 on the D3Q27 of
+ *   lattice.h:14-106); fields then hold 19 or 27 populations per voxel
+ *   (nx_mgrid_populations).
+ * nx_mgrid_create: the same for depth 2 (lam0, lam1) and BGK D3Q19.
  * nx_mgrid_depth: the depth and, if dims is not NULL, the voxels per axis of
  *   every level, dims[3 * l + axis].
  * nx_mgrid_state_level: the state byte of every voxel of one level (dense,
@@ -444,11 +449,14 @@ int nx_batch_destroy(nx_batch b);
  *   inactive voxels are not written.  Synchronises the device.
  * A handle is not for concurrent use from several host threads (a step
  * swaps level 0's two buffers when it is queued). */
+#define NX_MGRID_BGK19 0 /* BGK collision on D3Q19 (the app's BGK build) */
+#define NX_MGRID_KBC27 1 /* KBC collision on D3Q27 (the app's default build) */
 int nx_mgrid_create_levels(const int32_t dim[3], int32_t depth, const uint8_t* const* lams, int32_t storage,
-                           nx_mgrid* g);
+                           int32_t lattice, nx_mgrid* g);
 int nx_mgrid_create(const int32_t dim[3], const uint8_t* lam0, const uint8_t* lam1, int32_t storage,
                     nx_mgrid* g);
 int nx_mgrid_depth(nx_mgrid g, int32_t* depth, int32_t* dims);
+int nx_mgrid_populations(nx_mgrid g, int32_t* q);
 int nx_mgrid_info(nx_mgrid g, uint32_t* n_blocks, uint64_t* n_active, uint64_t* n_leaf, uint64_t* n_store,
                   int32_t* fine_fused);
 int nx_mgrid_state_level(nx_mgrid g, int32_t level, uint8_t* st);

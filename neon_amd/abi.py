@@ -145,7 +145,8 @@ SIGNATURES = {
     "nx_batch_get_launch": (_I, [_P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32), C.POINTER(_I32)]),
     "nx_batch_autotune": (_I, [_P, _P, C.POINTER(_I32), C.POINTER(_I32), C.POINTER(C.c_float)]),
     "nx_batch_destroy": (_I, [_P]),
-    "nx_mgrid_create_levels": (_I, [_P, _I32, _P, _I32, C.POINTER(_P)]),
+    "nx_mgrid_create_levels": (_I, [_P, _I32, _P, _I32, _I32, C.POINTER(_P)]),
+    "nx_mgrid_populations": (_I, [_P, _P]),
     "nx_mgrid_create": (_I, [_P, _P, _P, _I32, C.POINTER(_P)]),
     "nx_mgrid_depth": (_I, [_P, _P, _P]),
     "nx_mgrid_state_level": (_I, [_P, _I32, _P]),

@@ -46,12 +46,13 @@ __device__ __forceinline__ Conn load_conn(const uint32_t* __restrict__ conn, uin
 }
 
 // Neighbour voxel (block t, in-block index nv) of voxel (x,y,z) of block b at
-// offset -c_q (the upstream voxel of population q).
-template <int q>
-__device__ __forceinline__ void upstream(uint32_t b, const Conn& cn, int x, int y, int z,
-                                         uint32_t& t, uint32_t& nv)
+// offset (s0, s1, s2), each in {-1, 0, 1}: the block is b unless the offset
+// leaves it across a face, an edge or a corner, then the connectivity row's
+// entry 13 + s0 + 3 s1 + 9 s2 restricted to the axes crossed.
+template <int s0, int s1, int s2>
+__device__ __forceinline__ void upstream_off(uint32_t b, const Conn& cn, int x, int y, int z, uint32_t& t,
+                                             uint32_t& nv)
 {
-    constexpr int s0 = -CX[q], s1 = -CY[q], s2 = -CZ[q];
     const bool c0 = s0 != 0 && x == (s0 < 0 ? 0 : 7);
     const bool c1 = s1 != 0 && y == (s1 < 0 ? 0 : 7);
     const bool c2 = s2 != 0 && z == (s2 < 0 ? 0 : 7);
@@ -62,7 +63,15 @@ __device__ __forceinline__ void upstream(uint32_t b, const Conn& cn, int x, int 
     if constexpr (s0 != 0 && s1 != 0) t = (c0 && c1) ? cn.c[13 + s0 + 3 * s1] : t;
     if constexpr (s0 != 0 && s2 != 0) t = (c0 && c2) ? cn.c[13 + s0 + 9 * s2] : t;
     if constexpr (s1 != 0 && s2 != 0) t = (c1 && c2) ? cn.c[13 + 3 * s1 + 9 * s2] : t;
+    if constexpr (s0 != 0 && s1 != 0 && s2 != 0) t = (c0 && c1 && c2) ? cn.c[13 + s0 + 3 * s1 + 9 * s2] : t;
     nv = (uint32_t)(((x + s0) & 7) + 8 * ((y + s1) & 7) + 64 * ((z + s2) & 7));
+}
+
+// the upstream voxel of D3Q19 population q (offset -c_q)
+template <int q>
+__device__ __forceinline__ void upstream(uint32_t b, const Conn& cn, int x, int y, int z, uint32_t& t, uint32_t& nv)
+{
+    upstream_off<-CX[q], -CY[q], -CZ[q]>(b, cn, x, y, z, t, nv);
 }
 
 template <int q>

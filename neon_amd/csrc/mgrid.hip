@@ -1,12 +1,12 @@
 // mgrid.hip -- grid-refinement LBM (Neon's mGrid, depth 2 to 5, and the
-// lbmMultiRes application's BGK D3Q19 scheme) on a stack of block-sparse
-// levels.
+// lbmMultiRes application's scheme with its BGK D3Q19 or KBC D3Q27
+// collision) on a stack of block-sparse levels.
 //
 // Level 0 is the finest level (the domain); level l has voxels of edge 2^l,
 // ceil(dim / 2^l) per axis, and a level-(l+1) voxel v covers the level-l
 // voxels 2v + {0,1}^3.  Each level is a bGrid of 8^3 blocks (the library's
 // builder: blocks ordered bz, by, bx, 27-entry connectivity) with AoSoA
-// populations (b*19 + q)*512 + x + 8y + 64z, and a dense table from block
+// populations (b*Q + q)*512 + x + 8y + 64z (Q = 19 or 27), and a dense table from block
 // coordinates to block index for the cross-level lookups.
 //
 // Everything that depends only on the geometry is resolved on the host when
@@ -22,7 +22,8 @@
 // Reference semantics restated (file:line in each kernel):
 //   grid      libNeonDomain/src/domain/details/mGrid/mGrid.cpp:89-345, 466-555
 //   set-up    apps/lbmMultiRes/lidDrivenCavity.h:8-78, init.h:8-86
-//   collide   apps/lbmMultiRes/collide.h:8-149
+//   collide   apps/lbmMultiRes/collide.h:8-149 (BGK), :148-284 (KBC)
+//   lattice   apps/lbmMultiRes/lattice.h:14-106 (D3Q27), :108-173 (D3Q19)
 //   store     apps/lbmMultiRes/store.h:98-163
 //   stream    apps/lbmMultiRes/stream.h:170-225
 //   helpers   apps/lbmMultiRes/util.h:19-45, mPartition_imp.h:155-235
@@ -62,32 +63,153 @@ struct MS<NX_F32> {
     using T = float;
 };
 
+// ---- lattices ------------------------------------------------------------------
+
+// D3Q27 of apps/lbmMultiRes/lattice.h:14-75 (z fastest, then y, then x, each
+// over 0, -1, +1), the lattice of its default KBC collision
+constexpr int K27X[27] = {0, 0, 0, 0, 0, 0, 0, 0, 0, -1, -1, -1, -1, -1, -1, -1, -1, -1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+constexpr int K27Y[27] = {0, 0, 0, -1, -1, -1, 1, 1, 1, 0, 0, 0, -1, -1, -1, 1, 1, 1, 0, 0, 0, -1, -1, -1, 1, 1, 1};
+constexpr int K27Z[27] = {0, -1, 1, 0, -1, 1, 0, -1, 1, 0, -1, 1, 0, -1, 1, 0, -1, 1, 0, -1, 1, 0, -1, 1, 0, -1, 1};
+constexpr int K27OPP[27] = {0,  2,  1,  6,  8,  7,  3,  5,  4,  18, 20, 19, 24, 26,
+                            25, 21, 23, 22, 9,  11, 10, 15, 17, 16, 12, 14, 13};
+__host__ __device__ constexpr double w27(int q)
+{
+    return (K27X[q] != 0) + (K27Y[q] != 0) + (K27Z[q] != 0) == 0   ? 8.0 / 27.0
+           : (K27X[q] != 0) + (K27Y[q] != 0) + (K27Z[q] != 0) == 1 ? 2.0 / 27.0
+           : (K27X[q] != 0) + (K27Y[q] != 0) + (K27Z[q] != 0) == 2 ? 1.0 / 54.0
+                                                                    : 1.0 / 216.0;
+}
+
+// KBC D3Q27 collision (collide.h:148-284) with the reference's type
+// promotions: the density and velocity sums over q in order (util.h:47-62),
+// the double literals of the equilibrium, the shear split and gamma
+// evaluated in binary64 and stored in T, tiny = 1e-7 in gamma's denominator.
+template <typename T>
+__device__ __forceinline__ void kbc27(const T f[27], T om, T out[27])
+{
+    const T beta = (T)((double)om * 0.5);
+    const T invBeta = (T)(1.0 / (double)beta);
+    const T tiny = (T)1e-7;
+    T rho = 0;
+#pragma unroll
+    for (int q = 0; q < 27; q++) rho += f[q];
+    T v0 = 0, v1 = 0, v2 = 0;
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        v0 += f[q] * (T)K27X[q];
+        v1 += f[q] * (T)K27Y[q];
+        v2 += f[q] * (T)K27Z[q];
+    }
+    v0 /= rho;
+    v1 /= rho;
+    v2 /= rho;
+    const T usqr = (T)((3.0 / 2.0) * (double)(v0 * v0 + v1 * v1 + v2 * v2));
+    T feq[27], fneq[27];
+    T p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;  // Pi: xx, xy, xz, yy, yz, zz
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        T cu = 0;
+        cu += (T)K27X[q] * v0;
+        cu += (T)K27Y[q] * v1;
+        cu += (T)K27Z[q] * v2;
+        cu = (T)((double)cu * 3.0);
+        feq[q] = (T)((double)rho * w27(q) * (1. + (double)cu + 0.5 * (double)cu * (double)cu - (double)usqr));
+        fneq[q] = f[q] - feq[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        p0 += fneq[q] * (T)(K27X[q] * K27X[q]);
+        p1 += fneq[q] * (T)(K27X[q] * K27Y[q]);
+        p2 += fneq[q] * (T)(K27X[q] * K27Z[q]);
+        p3 += fneq[q] * (T)(K27Y[q] * K27Y[q]);
+        p4 += fneq[q] * (T)(K27Y[q] * K27Z[q]);
+        p5 += fneq[q] * (T)(K27Z[q] * K27Z[q]);
+    }
+    const T Nxz = p0 - p5, Nyz = p3 - p5;
+    const T shx = (T)((2.0 * (double)Nxz - (double)Nyz) / 6.0);
+    const T shy = (T)((-(double)Nxz + 2.0 * (double)Nyz) / 6.0);
+    const T shz = (T)((double)(-Nxz - Nyz) / 6.0);
+    const T sxy = (T)((double)p1 / 4.0), sxz = (T)((double)p2 / 4.0), syz = (T)((double)p4 / 4.0);
+    T dS[27];
+    T e0 = 0, e1 = 0;
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        const int cx = K27X[q], cy = K27Y[q], cz = K27Z[q];
+        const int nz = (cx != 0) + (cy != 0) + (cz != 0);
+        T sh = (T)0;
+        if (nz == 1) sh = cx ? shx : (cy ? shy : shz);
+        if (nz == 2) sh = !cz ? (cx * cy > 0 ? sxy : -sxy) : (!cy ? (cx * cz > 0 ? sxz : -sxz) : (cy * cz > 0 ? syz : -syz));
+        dS[q] = rho * sh;
+        const T dH = fneq[q] - dS[q];
+        e0 += (dS[q] * dH / feq[q]);
+        e1 += (dH * dH / feq[q]);
+    }
+    const T gamma = (T)((double)invBeta - (2.0 - (double)invBeta) * (double)e0 / (double)(tiny + e1));
+#pragma unroll
+    for (int q = 0; q < 27; q++) {
+        const T dH = fneq[q] - dS[q];
+        out[q] = (T)((double)f[q] - (double)beta * (2.0 * (double)dS[q] + (double)(gamma * dH)));
+    }
+}
+
+// lattice traits: populations, velocity table, rest population, collision
+struct L19 {
+    static constexpr int Q = 19, REST = 9, W = 1;  // W: 64-bit link words per voxel
+    static constexpr __host__ __device__ int cx(int q) { return CX[q]; }
+    static constexpr __host__ __device__ int cy(int q) { return CY[q]; }
+    static constexpr __host__ __device__ int cz(int q) { return CZ[q]; }
+    static constexpr __host__ __device__ int opp(int q) { return OPP[q]; }
+    static constexpr __host__ __device__ double w(int q) { return wq(q); }
+    template <typename T>
+    static __device__ __forceinline__ void collide(const T f[19], T om, T out[19])
+    {
+        bgk<T>(f, om, out);
+    }
+};
+struct L27 {
+    static constexpr int Q = 27, REST = 0, W = 2;
+    static constexpr __host__ __device__ int cx(int q) { return K27X[q]; }
+    static constexpr __host__ __device__ int cy(int q) { return K27Y[q]; }
+    static constexpr __host__ __device__ int cz(int q) { return K27Z[q]; }
+    static constexpr __host__ __device__ int opp(int q) { return K27OPP[q]; }
+    static constexpr __host__ __device__ double w(int q) { return w27(q); }
+    template <typename T>
+    static __device__ __forceinline__ void collide(const T f[27], T om, T out[27])
+    {
+        kbc27<T>(f, om, out);
+    }
+};
+
+// link state of population q: 3 bits, 21 populations per 64-bit word
+constexpr int kLinksPerWord = 21;
+
 // ---- kernels ----------------------------------------------------------------
 
-// collide.h:8-149: BGK on every bulk leaf voxel (fin -> fout, same voxel);
-// a refined voxel (levels above 0) resets its fout, the accumulator of the
-// fine store, to 0.
-template <int S>
+// collide.h:8-284: the lattice's collision on every bulk leaf voxel (fin ->
+// fout, same voxel); a refined voxel (levels above 0) resets its fout, the
+// accumulator of the fine store, to 0.
+template <int S, class LT>
 __global__ void __launch_bounds__(kBV)
     k_mr_collide(const typename MS<S>::T* __restrict__ fin, typename MS<S>::T* __restrict__ fout,
                  const uint32_t* __restrict__ state, double omega)
 {
     using T = typename MS<S>::T;
+    constexpr int NQ = LT::Q;
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x), v = threadIdx.x;  // neighbour blocks share an XCD's L2
     const uint32_t st = state[(uint64_t)b * kBV + v];
     if (!(st & kActive) || cls_bits(st) != NX_CELL_BULK) return;
-    const uint64_t own = (uint64_t)b * (Q * kBV) + v;
+    const uint64_t own = (uint64_t)b * (NQ * kBV) + v;
     if (st & kRefined) {
 #pragma unroll
-        for (int q = 0; q < Q; q++) __builtin_nontemporal_store(T(0), &fout[own + (uint64_t)q * kBV]);
+        for (int q = 0; q < NQ; q++) __builtin_nontemporal_store(T(0), &fout[own + (uint64_t)q * kBV]);
         return;
     }
-    T f[Q], o[Q];
+    T f[NQ], o[NQ];
 #pragma unroll
-    for (int q = 0; q < Q; q++) f[q] = fin[own + (uint64_t)q * kBV];
-    bgk<T>(f, (T)omega, o);
+    for (int q = 0; q < NQ; q++) f[q] = fin[own + (uint64_t)q * kBV];
+    LT::template collide<T>(f, (T)omega, o);
 #pragma unroll
-    for (int q = 0; q < Q; q++) __builtin_nontemporal_store(o[q], &fout[own + (uint64_t)q * kBV]);
+    for (int q = 0; q < NQ; q++) __builtin_nontemporal_store(o[q], &fout[own + (uint64_t)q * kBV]);
 }
 
 // store.h:98-163 as a gather: entry e adds its fine sources, in order, to
@@ -113,8 +235,8 @@ struct UpLevel {  // the next coarser level, seen from this one (explosion)
 };
 
 // stream.h:170-225 (stream fused with coalescence and explosion), pull, for
-// the bulk leaf voxels of one level.  The plain pull of all 19 populations
-// is issued first (upstream voxel through the connectivity row in scalar
+// the bulk leaf voxels of one level.  The plain pull of all populations is
+// issued first (upstream voxel through the connectivity row in scalar
 // registers); voxels with another link state then fix those links up:
 // bounce-back (own opposite + the wall's opposite), explosion (the coarse
 // post-collision population of the uncle, the next level's voxel holding
@@ -124,17 +246,19 @@ struct UpLevel {  // the next coarser level, seen from this one (explosion)
 // COLLIDE: the pulled populations are collided at once with omega and the
 // result written to `out`, and refined voxels reset their accumulator in
 // `out` -- stream followed by the next collide of the same level
-// (collide.h:8-149) in one pass; the results are those of the two kernels in
+// (collide.h:8-284) in one pass; the results are those of the two kernels in
 // sequence, bit for bit.  KEEP = false: the level has no link in state
-// "none" (checked on the host), every population is stored.
-template <int S, bool COLLIDE, bool KEEP>
+// "none" (checked on the host), every population is stored.  `links` holds
+// LT::W words per voxel slot, word k at k * n_slots.
+template <int S, class LT, bool COLLIDE, bool KEEP>
 __global__ void __launch_bounds__(kBV)
     k_mr_stream(const typename MS<S>::T* __restrict__ fout, typename MS<S>::T* __restrict__ out,
-                const uint32_t* __restrict__ state, const uint64_t* __restrict__ links,
+                const uint32_t* __restrict__ state, const uint64_t* __restrict__ links, uint64_t n_slots,
                 const uint32_t* __restrict__ conn, const int32_t* __restrict__ origins,
                 const float* __restrict__ ss, UpLevel up, double omega)
 {
     using T = typename MS<S>::T;
+    constexpr int NQ = LT::Q;
     const uint32_t b = xcd_remap(blockIdx.x, gridDim.x), v = threadIdx.x;  // neighbour blocks share an XCD's L2
     const Conn cb = load_conn(conn, b);
     const uint32_t st = state[(uint64_t)b * kBV + v];
@@ -143,46 +267,51 @@ __global__ void __launch_bounds__(kBV)
         // accumulator (collide.h:131-141); otherwise its fin, which the
         // reference never writes after the set-up's 0, back to 0 (a fused
         // pass earlier in the call left an accumulator in this buffer)
-        const uint64_t own = (uint64_t)b * (Q * kBV) + v;
+        const uint64_t own = (uint64_t)b * (NQ * kBV) + v;
 #pragma unroll
-        for (int q = 0; q < Q; q++) __builtin_nontemporal_store(T(0), &out[own + (uint64_t)q * kBV]);
+        for (int q = 0; q < NQ; q++) __builtin_nontemporal_store(T(0), &out[own + (uint64_t)q * kBV]);
         return;
     }
     if (!bulk_leaf(st)) return;
-    const uint64_t lk = links[(uint64_t)b * kBV + v];
+    uint64_t lk[LT::W];
+#pragma unroll
+    for (int k = 0; k < LT::W; k++) lk[k] = links[(uint64_t)k * n_slots + (uint64_t)b * kBV + v];
     const int x = (int)(v & 7u), y = (int)((v >> 3) & 7u), z = (int)(v >> 6);
-    const uint64_t own = (uint64_t)b * (Q * kBV) + v;
-    T f[Q];
-    for_q(AllQ{}, [&](auto qc) {
+    const uint64_t own = (uint64_t)b * (NQ * kBV) + v;
+    T f[NQ];
+    for_q(std::make_integer_sequence<int, NQ>{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        if constexpr (q == 9) {
-            f[9] = fout[own + 9 * kBV];
+        if constexpr (q == LT::REST) {
+            f[q] = fout[own + (uint64_t)q * kBV];
         } else {
             uint32_t t, nv;
-            upstream<q>(b, cb, x, y, z, t, nv);
-            f[q] = fout[t != kInvalidBlock ? ((uint64_t)t * Q + q) * kBV + nv : own + (uint64_t)q * kBV];
+            upstream_off<-LT::cx(q), -LT::cy(q), -LT::cz(q)>(b, cb, x, y, z, t, nv);
+            f[q] = fout[t != kInvalidBlock ? ((uint64_t)t * NQ + q) * kBV + nv : own + (uint64_t)q * kBV];
         }
     });
 #pragma unroll
-    for (int q = 0; q < Q; q++) nx_pin(f[q]);
+    for (int q = 0; q < NQ; q++) nx_pin(f[q]);
     uint32_t keep = 0;  // links whose value stays as it is in `out`
-    if (lk != 0) {
-        for_q(AllQ{}, [&](auto qc) {
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < LT::W; k++) any |= lk[k] != 0;
+    if (any) {
+        for_q(std::make_integer_sequence<int, NQ>{}, [&](auto qc) {
             constexpr int q = decltype(qc)::value;
-            if constexpr (q != 9) {
-                const uint64_t kind = (lk >> (3 * q)) & 7u;
+            if constexpr (q != LT::REST) {
+                const uint64_t kind = (lk[q / kLinksPerWord] >> (3 * (q % kLinksPerWord))) & 7u;
                 if (kind == kPlain) return;
                 uint32_t t, nv;
-                upstream<q>(b, cb, x, y, z, t, nv);
+                upstream_off<-LT::cx(q), -LT::cy(q), -LT::cz(q)>(b, cb, x, y, z, t, nv);
                 bool ok = true;
                 if (kind == kWall) {
-                    constexpr int o = OPP[q];
+                    constexpr int o = LT::opp(q);
                     ok = t != kInvalidBlock;
-                    if (ok) f[q] = fout[own + (uint64_t)o * kBV] + fout[((uint64_t)t * Q + o) * kBV + nv];
+                    if (ok) f[q] = fout[own + (uint64_t)o * kBV] + fout[((uint64_t)t * NQ + o) * kBV + nv];
                 } else if (kind == kExpl) {
-                    const int ux = (origins[3 * (uint64_t)b] + x - CX[q]) >> 1;
-                    const int uy = (origins[3 * (uint64_t)b + 1] + y - CY[q]) >> 1;
-                    const int uz = (origins[3 * (uint64_t)b + 2] + z - CZ[q]) >> 1;
+                    const int ux = (origins[3 * (uint64_t)b] + x - LT::cx(q)) >> 1;
+                    const int uy = (origins[3 * (uint64_t)b + 1] + y - LT::cy(q)) >> 1;
+                    const int uz = (origins[3 * (uint64_t)b + 2] + z - LT::cz(q)) >> 1;
                     const int bx = ux >> 3, by = uy >> 3, bz = uz >> 3;
                     ok = up.table && !(bx < 0 || by < 0 || bz < 0 || bx >= up.nb[0] || by >= up.nb[1] ||
                                        bz >= up.nb[2]);
@@ -190,10 +319,10 @@ __global__ void __launch_bounds__(kBV)
                                            : kInvalidBlock;
                     ok = tb != kInvalidBlock;
                     const uint32_t slot = (uint32_t)((ux & 7) + 8 * (uy & 7) + 64 * (uz & 7));
-                    if (ok) f[q] = ((const T*)up.fout)[((uint64_t)tb * Q + q) * kBV + slot];
+                    if (ok) f[q] = ((const T*)up.fout)[((uint64_t)tb * NQ + q) * kBV + slot];
                 } else if (kind == kCoal) {
                     ok = t != kInvalidBlock && ss;
-                    if (ok) f[q] = f[q] * (T)ss[((uint64_t)t * Q + q) * kBV + nv];
+                    if (ok) f[q] = f[q] * (T)ss[((uint64_t)t * NQ + q) * kBV + nv];
                 } else {
                     ok = false;
                 }
@@ -205,17 +334,17 @@ __global__ void __launch_bounds__(kBV)
         });
     }
     if constexpr (COLLIDE) {
-        T o[Q];
-        bgk<T>(f, (T)omega, o);
+        T o[NQ];
+        LT::template collide<T>(f, (T)omega, o);
 #pragma unroll
-        for (int q = 0; q < Q; q++) __builtin_nontemporal_store(o[q], &out[own + (uint64_t)q * kBV]);
+        for (int q = 0; q < NQ; q++) __builtin_nontemporal_store(o[q], &out[own + (uint64_t)q * kBV]);
     } else if constexpr (KEEP) {
 #pragma unroll
-        for (int q = 0; q < Q; q++)
+        for (int q = 0; q < NQ; q++)
             if (!((keep >> q) & 1u)) __builtin_nontemporal_store(f[q], &out[own + (uint64_t)q * kBV]);
     } else {
 #pragma unroll
-        for (int q = 0; q < Q; q++) __builtin_nontemporal_store(f[q], &out[own + (uint64_t)q * kBV]);
+        for (int q = 0; q < NQ; q++) __builtin_nontemporal_store(f[q], &out[own + (uint64_t)q * kBV]);
     }
 }
 
@@ -237,6 +366,7 @@ struct Level {
     void* fin = nullptr;
     void* fout = nullptr;
     bool keep = true;  // the level has links in state "none"
+    int nq = 19;       // populations per voxel
 
     size_t nvox() const { return (size_t)dim[0] * dim[1] * dim[2]; }
     size_t lin(int x, int y, int z) const { return (size_t)x + (size_t)dim[0] * ((size_t)y + (size_t)dim[1] * z); }
@@ -253,8 +383,34 @@ struct Level {
         return (int64_t)b * kBV + ((x & 7) + 8 * (y & 7) + 64 * (z & 7));
     }
     // population element of an active voxel's slot
-    static uint64_t elem(int64_t slot, int q) { return ((uint64_t)(slot / kBV) * Q + q) * kBV + slot % kBV; }
+    uint64_t elem(int64_t slot, int q) const { return ((uint64_t)(slot / kBV) * nq + q) * kBV + slot % kBV; }
+    size_t slots() const { return (size_t)kBV * n_blocks; }
 };
+
+// the lattice on the host: velocities, opposites, weights, link words
+struct HostLat {
+    int Q = 19, rest = 9, W = 1;
+    int c[27][3] = {};
+    int opp[27] = {};
+    double w[27] = {};
+};
+
+template <class LT>
+HostLat host_lat_of()
+{
+    HostLat h;
+    h.Q = LT::Q;
+    h.rest = LT::REST;
+    h.W = LT::W;
+    for (int q = 0; q < LT::Q; q++) {
+        h.c[q][0] = LT::cx(q);
+        h.c[q][1] = LT::cy(q);
+        h.c[q][2] = LT::cz(q);
+        h.opp[q] = LT::opp(q);
+        h.w[q] = LT::w(q);
+    }
+    return h;
+}
 
 struct Store {  // fine-to-coarse gather list of one level pair (l -> l + 1)
     uint32_t n = 0;
@@ -270,6 +426,8 @@ struct Store {  // fine-to-coarse gather list of one level pair (l -> l + 1)
 
 struct nx_mgrid_s {
     int storage = NX_F64;
+    int lattice = NX_MGRID_BGK19;
+    nx::mres::HostLat lat;
     int depth = 2;
     nx::mres::Level L[nx::mres::kMaxDepth];
     nx::mres::Store store[nx::mres::kMaxDepth];
@@ -437,9 +595,10 @@ void fill_cavity(nx_mgrid_s& g, double ulb, std::vector<T>* pop)
     // bounce-back, those with y = Y-1 the lid; populations w_q / 0 /
     // -6 w_q (c_q . ulid), ulid = (ulb, 0, 0), evaluated as the reference
     // does in T with binary64 constants
+    const HostLat& L = g.lat;
     for (int l = 0; l < g.depth; l++) {
         Level& lv = g.L[l];
-        pop[l].assign(19 * (size_t)kBV * lv.n_blocks, T(0));
+        pop[l].assign((size_t)L.Q * kBV * lv.n_blocks, T(0));
         for (int z = 0; z < lv.dim[2]; z++)
             for (int y = 0; y < lv.dim[1]; y++)
                 for (int x = 0; x < lv.dim[0]; x++) {
@@ -451,16 +610,16 @@ void fill_cavity(nx_mgrid_s& g, double ulb, std::vector<T>* pop)
                         if (l == 0 && (x == 0 || y == 0 || z == 0 || x == lv.dim[0] - 1 || y == lv.dim[1] - 1 ||
                                        z == lv.dim[2] - 1))
                             cls = y == lv.dim[1] - 1 ? NX_CELL_MOVINGWALL : NX_CELL_BOUNCEBACK;
-                        for (int q = 0; q < Q; q++) {
-                            T p = (T)wq(q);
+                        for (int q = 0; q < L.Q; q++) {
+                            T p = (T)L.w[q];
                             if (cls != NX_CELL_BULK) p = 0;
                             if (cls == NX_CELL_MOVINGWALL) {
-                                p = (T)((double)p + (double)CX[q] * ulb);
-                                p = (T)((double)p + (double)CY[q] * 0.);
-                                p = (T)((double)p + (double)CZ[q] * 0.);
-                                p = (T)((double)p * (-6. * wq(q)));
+                                p = (T)((double)p + (double)L.c[q][0] * ulb);
+                                p = (T)((double)p + (double)L.c[q][1] * 0.);
+                                p = (T)((double)p + (double)L.c[q][2] * 0.);
+                                p = (T)((double)p * (-6. * L.w[q]));
                             }
-                            pop[l][Level::elem(s, q)] = p;
+                            pop[l][lv.elem(s, q)] = p;
                         }
                     }
                     lv.st[i] = (uint8_t)((lv.st[i] & (kActive | kRefined)) | (cls << kClsShift));
@@ -480,22 +639,24 @@ bool has_parent(const nx_mgrid_s& g, int l, int x, int y, int z)
 }
 
 // Link states of every bulk leaf voxel of level l (stream.h:170-225 decided
-// per link); returns the number of links in state "none".
+// per link), 3 bits per population, W words per voxel slot (word k at
+// k * slots); returns the number of links in state "none".
 uint64_t links_of(const nx_mgrid_s& g, int l, std::vector<uint64_t>& out)
 {
     uint64_t n_skip = 0;
     const Level& lv = g.L[l];
-    out.assign((size_t)kBV * lv.n_blocks, 0);
+    const HostLat& L = g.lat;
+    out.assign((size_t)L.W * lv.slots(), 0);
     for (int z = 0; z < lv.dim[2]; z++)
         for (int y = 0; y < lv.dim[1]; y++)
             for (int x = 0; x < lv.dim[0]; x++) {
                 const uint8_t st = lv.st[lv.lin(x, y, z)];
                 if (!bulk_leaf(st)) continue;
                 const bool parent = has_parent(g, l, x, y, z);
-                uint64_t lk = 0;
-                for (int q = 0; q < Q; q++) {
-                    if (q == 9) continue;  // own population: plain
-                    const int nx_ = x - CX[q], ny = y - CY[q], nz = z - CZ[q];
+                uint64_t lk[2] = {0, 0};
+                for (int q = 0; q < L.Q; q++) {
+                    if (q == L.rest) continue;  // own population: plain
+                    const int nx_ = x - L.c[q][0], ny = y - L.c[q][1], nz = z - L.c[q][2];
                     uint64_t kind = kSkip;
                     if (lv.active(nx_, ny, nz)) {
                         const uint8_t ns = lv.st[lv.lin(nx_, ny, nz)];
@@ -505,14 +666,14 @@ uint64_t links_of(const nx_mgrid_s& g, int l, std::vector<uint64_t>& out)
                             kind = cls_bits(ns) == NX_CELL_BULK ? kPlain : kWall;
                     } else if (parent) {
                         const Level& c = g.L[l + 1];
-                        const int ux = (x >> 1) + uoff(x, -CX[q]), uy = (y >> 1) + uoff(y, -CY[q]),
-                                  uz = (z >> 1) + uoff(z, -CZ[q]);
+                        const int ux = (x >> 1) + uoff(x, -L.c[q][0]), uy = (y >> 1) + uoff(y, -L.c[q][1]),
+                                  uz = (z >> 1) + uoff(z, -L.c[q][2]);
                         if (c.active(ux, uy, uz)) kind = kExpl;
                     }
-                    lk |= kind << (3 * q);
+                    lk[q / kLinksPerWord] |= kind << (3 * (q % kLinksPerWord));
                     n_skip += kind == kSkip;
                 }
-                out[lv.slot(x, y, z)] = lk;
+                for (int k = 0; k < L.W; k++) out[(size_t)k * lv.slots() + lv.slot(x, y, z)] = lk[k];
             }
     return n_skip;
 }
@@ -529,7 +690,7 @@ int check_level(const nx_mgrid_s* g, int32_t level)
 }  // namespace
 
 extern "C" int nx_mgrid_create_levels(const int32_t dim[3], int32_t depth, const uint8_t* const* lams,
-                                      int32_t storage, nx_mgrid* out)
+                                      int32_t storage, int32_t lattice, nx_mgrid* out)
 {
     if (!dim || !lams || !out) return set_error(NX_ERR_INVALID, "nx_mgrid_create_levels: null argument");
     if (depth < 2 || depth > kMaxDepth)
@@ -541,9 +702,14 @@ extern "C" int nx_mgrid_create_levels(const int32_t dim[3], int32_t depth, const
             return set_error(NX_ERR_INVALID, "nx_mgrid_create_levels: dimensions in [2, 16384]");
     if (storage != NX_F64 && storage != NX_F32)
         return set_error(NX_ERR_UNSUPPORTED, "nx_mgrid_create: storage NX_F64 or NX_F32 (the reference's types)");
+    if (lattice != NX_MGRID_BGK19 && lattice != NX_MGRID_KBC27)
+        return set_error(NX_ERR_INVALID, "nx_mgrid_create_levels: lattice NX_MGRID_BGK19 or NX_MGRID_KBC27");
     auto* g = new nx_mgrid_s;
     g->storage = storage;
+    g->lattice = lattice;
+    g->lat = lattice == NX_MGRID_KBC27 ? host_lat_of<L27>() : host_lat_of<L19>();
     g->depth = depth;
+    for (int l = 0; l < depth; l++) g->L[l].nq = g->lat.Q;
     activate(dim, depth, lams, g->L);
     for (int l = 0; l < depth; l++)
         if (int e = build_level(g->L[l])) {
@@ -559,7 +725,7 @@ extern "C" int nx_mgrid_create(const int32_t dim[3], const uint8_t* lam0, const 
 {
     if (!lam0 || !lam1) return set_error(NX_ERR_INVALID, "nx_mgrid_create: null argument");
     const uint8_t* lams[2] = {lam0, lam1};
-    return nx_mgrid_create_levels(dim, 2, lams, storage, out);
+    return nx_mgrid_create_levels(dim, 2, lams, storage, NX_MGRID_BGK19, out);
 }
 
 extern "C" int nx_mgrid_depth(nx_mgrid g, int32_t* depth, int32_t* dims)
@@ -569,6 +735,13 @@ extern "C" int nx_mgrid_depth(nx_mgrid g, int32_t* depth, int32_t* dims)
     if (dims)
         for (int l = 0; l < g->depth; l++)
             for (int a = 0; a < 3; a++) dims[3 * l + a] = g->L[l].dim[a];
+    return NX_OK;
+}
+
+extern "C" int nx_mgrid_populations(nx_mgrid g, int32_t* q)
+{
+    if (!g || !q) return set_error(NX_ERR_INVALID, "nx_mgrid_populations: null argument");
+    *q = g->lat.Q;
     return NX_OK;
 }
 
@@ -628,7 +801,7 @@ extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
         fill_cavity<float>(*g, ulb, p32);
     for (int l = 0; l < D; l++) {
         Level& lv = g->L[l];
-        const size_t elems = 19 * (size_t)kBV * lv.n_blocks;
+        const size_t elems = (size_t)g->lat.Q * kBV * lv.n_blocks;
         if (int e = upload((void**)&lv.d_conn, lv.conn.data(), lv.conn.size() * 4)) return e;
         if (int e = upload((void**)&lv.d_origins, lv.origins.data(), lv.origins.size() * 4)) return e;
         if (int e = upload((void**)&lv.d_table, lv.table.data(), lv.table.size() * 4)) return e;
@@ -655,7 +828,7 @@ extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
     for (int l = 0; l + 1 < D; l++) {
         const Level& f = g->L[l];
         const Level& c = g->L[l + 1];
-        std::vector<float> cnt(19 * (size_t)kBV * c.n_blocks, 0.f);
+        std::vector<float> cnt((size_t)g->lat.Q * kBV * c.n_blocks, 0.f);
         std::vector<std::pair<uint64_t, uint64_t>> tl;  // (target element, source element)
         for (int z = 0; z < f.dim[2]; z++)
             for (int y = 0; y < f.dim[1]; y++)
@@ -663,17 +836,18 @@ extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
                     if (!f.active(x, y, z) || !has_parent(*g, l, x, y, z)) continue;
                     const bool leaf = !(f.st[f.lin(x, y, z)] & kRefined);
                     const int px = x >> 1, py = y >> 1, pz = z >> 1;
-                    for (int q = 0; q < Q; q++) {
-                        if (q == 9) continue;
-                        const int ux = uoff(x, CX[q]), uy = uoff(y, CY[q]), uz = uoff(z, CZ[q]);
+                    for (int q = 0; q < g->lat.Q; q++) {
+                        if (q == g->lat.rest) continue;
+                        const int* cq = g->lat.c[q];
+                        const int ux = uoff(x, cq[0]), uy = uoff(y, cq[1]), uz = uoff(z, cq[2]);
                         if (f.active(x + ux, y + uy, z + uz)) continue;
                         if (!c.active(px + ux, py + uy, pz + uz)) continue;
-                        const int sx = ux - CX[q], sy = uy - CY[q], sz = uz - CZ[q];
+                        const int sx = ux - cq[0], sy = uy - cq[1], sz = uz - cq[2];
                         const int64_t cs = c.slot(px + sx, py + sy, pz + sz);
                         if (cs < 0) continue;
-                        cnt[Level::elem(cs, q)] += 1.f;
+                        cnt[c.elem(cs, q)] += 1.f;
                         if (!leaf || !f.active(x + sx, y + sy, z + sz)) continue;
-                        tl.emplace_back(Level::elem(cs, q), Level::elem(f.slot(x, y, z), q));
+                        tl.emplace_back(c.elem(cs, q), f.elem(f.slot(x, y, z), q));
                     }
                 }
         for (float& v : cnt)
@@ -727,7 +901,7 @@ void expand(int l, int depth, std::vector<Op>& ops)
     }
 }
 
-template <int S>
+template <int S, class LT>
 struct Stepper {
     using T = typename MS<S>::T;
     nx_mgrid_s& g;
@@ -750,7 +924,7 @@ struct Stepper {
     {
         const Level& lv = g.L[l];
         if (lv.n_blocks)
-            hipLaunchKernelGGL((k_mr_collide<S>), dim3(lv.n_blocks), dim3(kBV), 0, st, (const T*)lv.fin,
+            hipLaunchKernelGGL((k_mr_collide<S, LT>), dim3(lv.n_blocks), dim3(kBV), 0, st, (const T*)lv.fin,
                                (T*)lv.fout, lv.d_state, omega(l));
     }
     void store(int l) const
@@ -765,9 +939,9 @@ struct Stepper {
     {
         const Level& lv = g.L[l];
         if (lv.n_blocks)
-            hipLaunchKernelGGL((k_mr_stream<S, COLLIDE, KEEP>), dim3(lv.n_blocks), dim3(kBV), 0, st,
-                               (const T*)lv.fout, (T*)lv.fin, lv.d_state, lv.d_links, lv.d_conn, lv.d_origins,
-                               (const float*)lv.d_ss, up(l), COLLIDE ? omega(l) : 0.0);
+            hipLaunchKernelGGL((k_mr_stream<S, LT, COLLIDE, KEEP>), dim3(lv.n_blocks), dim3(kBV), 0, st,
+                               (const T*)lv.fout, (T*)lv.fin, lv.d_state, lv.d_links, (uint64_t)lv.slots(),
+                               lv.d_conn, lv.d_origins, (const float*)lv.d_ss, up(l), COLLIDE ? omega(l) : 0.0);
     }
 
     // n coarse steps.  A level's stream and its next collide touch no other
@@ -820,10 +994,18 @@ extern "C" int nx_mgrid_step(nx_mgrid g, double omega0, int32_t n_steps, nx_stre
                 ops[i].last = true;
                 break;
             }
-    if (g->storage == NX_F64)
-        Stepper<NX_F64>{*g, omega0, (hipStream_t)s}.run(ops, n_steps);
-    else
-        Stepper<NX_F32>{*g, omega0, (hipStream_t)s}.run(ops, n_steps);
+    const hipStream_t st = (hipStream_t)s;
+    if (g->lattice == NX_MGRID_KBC27) {
+        if (g->storage == NX_F64)
+            Stepper<NX_F64, L27>{*g, omega0, st}.run(ops, n_steps);
+        else
+            Stepper<NX_F32, L27>{*g, omega0, st}.run(ops, n_steps);
+    } else {
+        if (g->storage == NX_F64)
+            Stepper<NX_F64, L19>{*g, omega0, st}.run(ops, n_steps);
+        else
+            Stepper<NX_F32, L19>{*g, omega0, st}.run(ops, n_steps);
+    }
     NX_CHECK_LAUNCH("mgrid step");
     return NX_OK;
 }
@@ -835,7 +1017,7 @@ extern "C" int nx_mgrid_fields(nx_mgrid g, int32_t level, int32_t which, void* d
     if (!g->ready) return set_error(NX_ERR_NOT_INIT, "nx_mgrid_fields: call nx_mgrid_cavity first");
     const Level& lv = g->L[level];
     const size_t es = g->storage == NX_F64 ? 8 : 4, n = lv.nvox();
-    std::vector<uint8_t> buf(19 * (size_t)kBV * lv.n_blocks * es);
+    std::vector<uint8_t> buf((size_t)lv.nq * kBV * lv.n_blocks * es);
     NX_CHECK_HIP(hipDeviceSynchronize());
     if (!buf.empty())
         NX_CHECK_HIP(hipMemcpy(buf.data(), which ? lv.fout : lv.fin, buf.size(), hipMemcpyDeviceToHost));
@@ -844,9 +1026,9 @@ extern "C" int nx_mgrid_fields(nx_mgrid g, int32_t level, int32_t which, void* d
             for (int x = 0; x < lv.dim[0]; x++) {
                 const int64_t s = lv.slot(x, y, z);
                 if (s < 0) continue;
-                for (int q = 0; q < Q; q++)
+                for (int q = 0; q < lv.nq; q++)
                     std::memcpy((uint8_t*)dense + ((size_t)q * n + lv.lin(x, y, z)) * es,
-                                buf.data() + Level::elem(s, q) * es, es);
+                                buf.data() + lv.elem(s, q) * es, es);
             }
     return NX_OK;
 }

@@ -94,10 +94,17 @@ class MGrid:
     MGrid(dim, [lam_0, .., lam_{D-1}], storage) or, for two levels,
     MGrid(dim, lam0, lam1, storage): lam_l is the activation of level l at
     every level-l voxel (evaluated at its level-0 origin v 2^l), uint8, x
-    fastest."""
+    fastest.  lattice "bgk19" (BGK D3Q19) or "kbc27" (KBC D3Q27, the
+    multi-resolution app's default)."""
 
-    def __init__(self, dim, lams, *rest):
+    LATTICES = {"bgk19": 0, "kbc27": 1}
+
+    def __init__(self, dim, lams, *rest, lattice="bgk19"):
         from .lattice import init, _storage
+        if lattice not in self.LATTICES:
+            raise ValueError(f"lattice {lattice!r}: one of {sorted(self.LATTICES)}")
+        self.lattice = lattice
+        self.Q = 27 if lattice == "kbc27" else 19
         if isinstance(lams, (list, tuple)):
             storage = rest[0] if rest else "f64"
         else:  # (dim, lam0, lam1[, storage])
@@ -114,7 +121,8 @@ class MGrid:
                 raise ValueError(f"activation array of {a.size} voxels for a level of {d}")
         h = C.c_void_p()
         ptrs = (C.c_void_p * self.depth)(*[a.ctypes.data for a in lams])
-        check(lib.nx_mgrid_create_levels((C.c_int32 * 3)(*self.dim), self.depth, ptrs, self.storage, C.byref(h)))
+        check(lib.nx_mgrid_create_levels((C.c_int32 * 3)(*self.dim), self.depth, ptrs, self.storage,
+                                         self.LATTICES[lattice], C.byref(h)))
         self.h = h.value
         self._init = init
 
@@ -151,9 +159,9 @@ class MGrid:
         check(lib.nx_mgrid_step(self.h, float(omega0), int(n), stream))
 
     def fields(self, level: int, which: int = 0) -> np.ndarray:
-        """fin (which 0) or fout (which 1) of one level, dense [19 * n]."""
+        """fin (which 0) or fout (which 1) of one level, dense [Q * n]."""
         n = int(np.prod(self.dims[level]))
         dt = np.float64 if self.storage == abi.NX_F64 else np.float32
-        out = np.zeros(19 * n, dt)
+        out = np.zeros(self.Q * n, dt)
         check(lib.nx_mgrid_fields(self.h, int(level), int(which), out.ctypes.data))
         return out

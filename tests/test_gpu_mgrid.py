@@ -249,3 +249,70 @@ def test_mgrid_split_calls_equal_one_call(nx, orc, storage):
             for which, ref in ((0, m.fin[lvl]), (1, m.fout[lvl])):
                 got = g.fields(lvl, which)
                 assert np.array_equal(got[act].view(np.uint8), ref[act].view(np.uint8)), (n, lvl, which)
+
+
+KBC = [((32, 28, 36), (0.0, -0.3, -1.0), 30.0, 12),
+       ((48, 48, 48), mg.LID_SCALES[0][1], 100.0, 6),    # the app's --scale 0 grid
+       ((50, 44, 40), (0.0, -0.2, -0.4, -0.7, -1.0), 50.0, 3)]
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+@pytest.mark.parametrize("dim,sdf,re,steps", KBC)
+def test_mgrid_kbc27_equals_oracle(nx, orc, storage, dim, sdf, re, steps):
+    """The app's default collision, KBC on D3Q27 (collide.h:148-284), at depth
+    2, 3 and 4: every population of both fields of every level bit for bit
+    with the oracle."""
+    lams = list(mg.cavity_lambdas(dim, sdf))
+    depth = len(lams)
+    ulb, om0 = mg.cavity_parameters(dim, re, storage, depth=depth)
+    print(f"mgrid KBC27 {dim} depth {depth} {storage} Re={re} omega0={om0:.6f} steps={steps}", flush=True)
+    g = mg.MGrid(dim, lams, storage, lattice="kbc27")
+    g.setup_cavity(ulb)
+    m = orc.MresOracle(dim, lams, orc.F64 if storage == "f64" else orc.F32, lattice=orc.KBC27)
+    m.setup_cavity(ulb)
+    for l, st in enumerate(g.state()):
+        assert np.array_equal(st, m.st[l]), l
+    g.run(steps, om0)
+    m.run(steps, om0)
+    for lvl in range(depth):
+        act = np.tile(m.active(lvl), 27)
+        for which, ref in ((0, m.fin[lvl]), (1, m.fout[lvl])):
+            got = g.fields(lvl, which)
+            n_diff = int(np.count_nonzero(got[act].view(np.uint8) != ref[act].view(np.uint8)))
+            assert n_diff == 0, (lvl, which, n_diff, np.abs(got[act] - ref[act]).max())
+
+
+def test_mgrid_160_kbc27_centerline(nx):
+    """The app as it ships: KBC D3Q27 on its three-level --scale 1 grid (160^3)
+    at Re 100 with the level-0 relaxation of the uniform run, 5000 coarse
+    steps: the centre line u_x(y) stays within 3% of the lid speed of the
+    reference's uniform BGK D3Q19 output (measured 1.86% at y = 124, mean
+    0.50%)."""
+    import pathlib
+    N, RE, ULB, STEPS = 160, 100.0, 0.04, 5000
+    dat = np.loadtxt(pathlib.Path(__file__).resolve().parent / "golden" / "NeonUniformLBM_20000_Y.dat")
+    omega_f = 1.0 / (3.0 * ULB * (N - 2) / RE + 0.5)
+    omega0 = 8.0 * omega_f / (2.0 + 3.0 * omega_f)
+    dim = (N, N, N)
+    g = mg.MGrid(dim, list(mg.cavity_lambdas(dim, mg.LID_SCALES[1][1])), "f64", lattice="kbc27")
+    g.setup_cavity(ULB)
+    g.run(STEPS, omega0)
+    st = g.state()
+    from oracle.oracle import D3Q27_C
+    cx = D3Q27_C[:, 0].astype(float)
+    f = [g.fields(l).reshape(27, *g.dims[l][::-1]) for l in range(3)]
+    a = [((st[l] & 3) == 1).reshape(g.dims[l][::-1]) for l in range(3)]
+    ux = np.zeros(N)
+    for y in range(1, N - 1):
+        for l in range(3):
+            p = (N // 2 >> l, y >> l, N // 2 >> l)
+            if a[l][p[2], p[1], p[0]]:
+                v = f[l][:, p[2], p[1], p[0]]
+                ux[y] = (cx @ v) / v.sum()
+                break
+    ux[0], ux[-1] = dat[0, 1], dat[-1, 1]
+    dev = np.abs(ux - dat[:, 1]) / ULB
+    print(f"KBC three-level 160^3 centre line vs the reference's uniform BGK run: max |du|/U {dev.max():.4f} "
+          f"at y = {int(dev.argmax())}, mean {dev.mean():.4f}")
+    assert np.isfinite(ux).all()
+    assert dev.max() < 0.03
