@@ -909,6 +909,119 @@ struct LbmIterationD3Q19
 };
 
 }  // namespace lbm
+
+// ---- multi-resolution grid with its LBM ---------------------------------------
+
+// Neon's mGrid (libNeonDomain/include/Neon/domain/details/mGrid/mGrid.h:23-60)
+// together with the lbmMultiRes application's LBM on it: one activation
+// lambda per level (evaluated at each level-l voxel's level-0 origin
+// v 2^l, as the reference's constructor does, mGrid.cpp:89-156), depth =
+// number of lambdas (2..5), fields of T = double or float, the app's KBC
+// D3Q27 (its default build) or BGK D3Q19 collision.  initLidDrivenCavity
+// and run replace the app's initLidDrivenCavity + initSumStore
+// (lidDrivenCavity.h:8-78, init.h:8-86) and the skeleton of
+// nonUniformTimestepRecursive (lbmMultiRes.h:141-263, 300-376).
+class mGrid
+{
+    struct State
+    {
+        Backend  bk;
+        nx_mgrid g = nullptr;
+        index_3d dim;
+        int      depth = 0, storage = NX_F64, q = 19;
+        ~State()
+        {
+            if (g) nx_mgrid_destroy(g);
+        }
+    };
+    std::shared_ptr<State> s_;
+
+   public:
+    enum class Collision { BGK, KBC };  // the app's #define BGK / #define KBC
+
+    mGrid() = default;
+    template <typename T>
+    static mGrid make(const Backend& bk, const index_3d& dim,
+                      const std::vector<std::function<bool(const index_3d&)>>& activeCellLambda,
+                      Collision collision = Collision::KBC)
+    {
+        static_assert(std::is_same_v<T, double> || std::is_same_v<T, float>, "mGrid fields are double or float");
+        mGrid m;
+        m.s_ = std::make_shared<State>();
+        State& s = *m.s_;
+        s.bk = bk;
+        s.dim = dim;
+        s.depth = int(activeCellLambda.size());
+        s.storage = std::is_same_v<T, double> ? NX_F64 : NX_F32;
+        std::vector<std::vector<uint8_t>> lam(s.depth);
+        std::vector<const uint8_t*>       ptr(s.depth);
+        index_3d                          d = dim;
+        for (int l = 0; l < s.depth; l++) {
+            lam[l].resize(d.rMulTyped());
+            size_t i = 0;
+            for (int z = 0; z < d.z; z++)
+                for (int y = 0; y < d.y; y++)
+                    for (int x = 0; x < d.x; x++) lam[l][i++] = activeCellLambda[l](index_3d(x << l, y << l, z << l));
+            ptr[l] = lam[l].data();
+            d = index_3d((d.x + 1) / 2, (d.y + 1) / 2, (d.z + 1) / 2);
+        }
+        const int32_t dd[3] = {dim.x, dim.y, dim.z};
+        check(nx_mgrid_create_levels(dd, s.depth, ptr.data(), s.storage,
+                                     collision == Collision::KBC ? NX_MGRID_KBC27 : NX_MGRID_BGK19, &s.g),
+              "mGrid");
+        check(nx_mgrid_populations(s.g, &s.q), "mGrid");
+        return m;
+    }
+
+    int      getDepth() const { return s_->depth; }
+    int      populations() const { return s_->q; }
+    index_3d getDimension() const { return s_->dim; }
+    // voxels per axis of level l: ceil(dim / 2^l)
+    index_3d levelDimension(int level) const
+    {
+        std::vector<int32_t> d(3 * size_t(s_->depth));
+        check(nx_mgrid_depth(s_->g, nullptr, d.data()), "mGrid::levelDimension");
+        return index_3d(d[3 * level], d[3 * level + 1], d[3 * level + 2]);
+    }
+    // mGrid::getDimension(level) (mGrid.cpp:684-687): the level's blocks of
+    // 2^3 voxels times 2
+    index_3d getDimension(int level) const
+    {
+        const int sp = 1 << (level + 1);
+        return index_3d((s_->dim.x + sp - 1) / sp * 2, (s_->dim.y + sp - 1) / sp * 2, (s_->dim.z + sp - 1) / sp * 2);
+    }
+    // leaf voxels per level (the app's countActiveVoxels, init.h:93-130)
+    std::vector<uint64_t> numLeafVoxels() const
+    {
+        std::vector<uint32_t> nb(size_t(s_->depth));
+        std::vector<uint64_t> na(size_t(s_->depth)), nl(size_t(s_->depth));
+        check(nx_mgrid_info(s_->g, nb.data(), na.data(), nl.data(), nullptr, nullptr), "mGrid::numLeafVoxels");
+        return nl;
+    }
+    void initLidDrivenCavity(double ulb) { check(nx_mgrid_cavity(s_->g, ulb), "initLidDrivenCavity"); }
+    // nIter steps of the coarsest level with its relaxation omega0
+    void run(double omega0, int nIter, int streamIdx = Backend::mainStreamIdx)
+    {
+        check(nx_mgrid_step(s_->g, omega0, nIter, s_->bk.stream(streamIdx)), "mGrid::run");
+    }
+    // state bytes of a level: bit 0 active, bit 1 refined, bits 2..3 the class
+    std::vector<uint8_t> state(int level) const
+    {
+        std::vector<uint8_t> st(levelDimension(level).rMulTyped());
+        check(nx_mgrid_state_level(s_->g, level, st.data()), "mGrid::state");
+        return st;
+    }
+    // fin (fout = false) or fout of a level, dense [q][z][y][x] in T
+    template <typename T>
+    std::vector<T> populations(int level, bool fout) const
+    {
+        if ((s_->storage == NX_F64) != std::is_same_v<T, double>)
+            throw NeonException("mGrid::populations: the field type differs");
+        std::vector<T> f(levelDimension(level).rMulTyped() * size_t(s_->q));
+        check(nx_mgrid_fields(s_->g, level, fout ? 1 : 0, f.data()), "mGrid::populations");
+        return f;
+    }
+};
 }  // namespace neon_amd
 
 #if defined(__HIPCC__)

@@ -1,0 +1,51 @@
+"""The C++ multi-resolution program on the Neon-shaped host API
+(include/neon_amd.hpp mGrid, neon_amd/apps/lbm_multires.cpp): run with the
+reference app's command line (apps/lbmMultiRes/lbmMultiRes.cu:53-95), its
+centre-line files must match the reference's own output of the same run,
+tests/golden/NeonMultiResLBM_5000_{X,Y}.dat (--scale 1, Re 100, KBC D3Q27,
+float, 5000 iterations), and its report carries the reference's members."""
+import glob
+import json
+import pathlib
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = pathlib.Path(__file__).resolve().parents[1]
+EXE = ROOT / "neon_amd" / "lib" / "lbm_multires"
+GOLDEN = ROOT / "tests" / "golden"
+
+
+def test_cli_rejects_what_is_not_built(tmp_path):
+    for extra in (["--deviceType", "cpu"], ["--problemType", "sphere"], ["--dataType", "half"],
+                  ["--vtk"], ["--collision", "mrt"], ["--numIter", "x"]):
+        r = subprocess.run([str(EXE), *extra], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+        assert r.returncode == 2 and "usage" in r.stderr, extra
+
+
+@pytest.mark.gpu
+def test_app_reproduces_reference_multires_output(tmp_path):
+    cmd = [str(EXE), "--numIter", "5000", "--deviceType", "gpu", "--deviceId", "0", "--problemType", "lid",
+           "--benchmark", "--dataType", "float", "--re", "100", "--scale", "1"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=200)
+    assert r.returncode == 0, r.stdout + r.stderr
+    print(r.stdout)
+    for name in ("NeonMultiResLBM_5000_Y.dat", "NeonMultiResLBM_5000_X.dat"):
+        got_text = (tmp_path / name).read_text().split("\n")
+        ref_text = (GOLDEN / name).read_text().split("\n")
+        assert len(got_text) == len(ref_text)
+        # the same leaves at the same positions, printed the same way
+        assert [l.split(" ")[0] for l in got_text] == [l.split(" ")[0] for l in ref_text]
+        got, ref = np.loadtxt(tmp_path / name), np.loadtxt(GOLDEN / name)
+        dev = np.abs(got[:, 1] - ref[:, 1])
+        print(f"{name}: max |u - u_ref| / U {dev.max():.3e}, median {np.median(dev):.3e}")
+        assert dev.max() < 0.005
+    reports = glob.glob(str(tmp_path / "MultiResLBM_P1_CH-SEO+_F_D*.json"))
+    assert len(reports) == 1
+    rep = json.load(open(reports[0]))
+    assert rep["Depth"] == 3 and rep["Collision"] == "KBC" and rep["NumIter"] == 5000
+    assert rep["Grid Size X"] == 160 and rep["clength"] == 40 and rep["Algorithm"] == "CH-SEO+"
+    assert len(rep["NumActiveVoxels"]) == 3 and rep["MLUPS"] > 0 and rep["EMLUPS"] > rep["MLUPS"]
+    assert abs(rep["omega"] - 1.0 / (3 * 0.04 * 40 / 100 + 0.5)) < 1e-6
