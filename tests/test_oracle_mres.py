@@ -469,3 +469,29 @@ def test_kbc27_three_levels_close_to_bgk(orc):
         u[lat] = ((c.T @ f) / f.sum(0))[:, leaf]
     d = np.sqrt(((u[orc.BGK19] - u[orc.KBC27]) ** 2).sum(0)) / ulb
     assert d.max() < 0.045 and np.sqrt((d ** 2).mean()) < 0.003
+
+
+def _random_levels(seed):
+    rng = np.random.default_rng(seed)
+    depth = int(rng.integers(2, 6))
+    dim = tuple(int(v) for v in rng.integers(2, 40, size=3))
+    dims = mg.level_dims(dim, depth)
+    dens = rng.uniform(0, 0.05, size=depth)
+    lams = [(rng.random(int(np.prod(d))) < dens[l]).astype(np.uint8) for l, d in enumerate(dims)]
+    lams[-1][:] = 1
+    return dim, lams
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_grids_builder_matches_oracle(orc, seed):
+    """Scattered activation at depth 2..5 on odd and even boxes (most of them
+    make the strong-balance pass add voxels, some cull the top level away):
+    the library's builder equals the oracle's voxel for voxel, and the
+    result keeps every neighbour of an active voxel covered at most one
+    level up."""
+    dim, lams = _random_levels(seed)
+    m = orc.MresOracle(dim, lams, orc.F64)
+    g = mg.MGrid(dim, lams, "f64")
+    for l, st in enumerate(g.state()):
+        assert np.array_equal(st & 3, m.st[l] & 3), l
+    assert _balance_violations(m) == 0
