@@ -426,6 +426,12 @@ int nx_batch_destroy(nx_batch b);
  *   level 0 on the domain faces are bounce-back, y = Y-1 the lid moving at
  *   (ulb, 0, 0); populations w_q / 0 / -6 w_q c_q.ulid in both fields; the
  *   coalescence weights 1 / (2 n) of every refined voxel.
+ * nx_mgrid_setup_cells is the same set-up for any problem: cls[l] gives the
+ *   class of every level-l voxel (NX_CELL_BULK, NX_CELL_BOUNCEBACK or
+ *   NX_CELL_MOVINGWALL; read for the leaves, x fastest) and u_wall the
+ *   velocity of the moving-wall cells -- the lid, or the inlet of the flow
+ *   over a shape (flowOverShape.h:17-113), whose cells the stream treats
+ *   alike.
  * nx_mgrid_step runs n steps of the coarsest level, each the reference's
  *   nonUniformTimestepRecursive (apps/lbmMultiRes/lbmMultiRes.h:141-263)
  *   with the baseline operation order and the fine store: per level l from
@@ -462,6 +468,7 @@ int nx_mgrid_info(nx_mgrid g, uint32_t* n_blocks, uint64_t* n_active, uint64_t* 
 int nx_mgrid_state_level(nx_mgrid g, int32_t level, uint8_t* st);
 int nx_mgrid_state(nx_mgrid g, uint8_t* st0, uint8_t* st1);
 int nx_mgrid_cavity(nx_mgrid g, double ulb);
+int nx_mgrid_setup_cells(nx_mgrid g, const uint8_t* const* cls, const double u_wall[3]);
 int nx_mgrid_step(nx_mgrid g, double omega0, int32_t n_steps, nx_stream s);
 int nx_mgrid_fields(nx_mgrid g, int32_t level, int32_t which, void* dense);
 int nx_mgrid_destroy(nx_mgrid g);

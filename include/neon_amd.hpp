@@ -999,6 +999,21 @@ class mGrid
         return nl;
     }
     void initLidDrivenCavity(double ulb) { check(nx_mgrid_cavity(s_->g, ulb), "initLidDrivenCavity"); }
+    // the set-up of any problem (flowOverShape.h:17-113): cls[l] holds the
+    // class of every level-l voxel (NX_CELL_BULK, NX_CELL_BOUNCEBACK,
+    // NX_CELL_MOVINGWALL for the lid or an inlet), uWall the velocity of the
+    // moving-wall cells
+    void initCells(const std::vector<std::vector<uint8_t>>& cls, const double uWall[3])
+    {
+        if (int(cls.size()) != s_->depth) throw NeonException("mGrid::initCells: one class array per level");
+        std::vector<const uint8_t*> ptr;
+        for (int l = 0; l < s_->depth; l++) {
+            if (cls[size_t(l)].size() != levelDimension(l).rMulTyped())
+                throw NeonException("mGrid::initCells: class array size differs from the level's");
+            ptr.push_back(cls[size_t(l)].data());
+        }
+        check(nx_mgrid_setup_cells(s_->g, ptr.data(), uWall), "mGrid::initCells");
+    }
     // nIter steps of the coarsest level with its relaxation omega0
     void run(double omega0, int nIter, int streamIdx = Backend::mainStreamIdx)
     {

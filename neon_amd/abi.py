@@ -153,6 +153,7 @@ SIGNATURES = {
     "nx_mgrid_info": (_I, [_P, _P, _P, _P, _P, _P]),
     "nx_mgrid_state": (_I, [_P, _P, _P]),
     "nx_mgrid_cavity": (_I, [_P, _D]),
+    "nx_mgrid_setup_cells": (_I, [_P, _P, _P]),
     "nx_mgrid_step": (_I, [_P, _D, _I32, _P]),
     "nx_mgrid_fields": (_I, [_P, _I32, _I32, _P]),
     "nx_mgrid_destroy": (_I, [_P]),

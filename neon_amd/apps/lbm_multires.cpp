@@ -4,7 +4,7 @@
 // one MI355X.
 //
 //   lbm_multires [--deviceType gpu] [--deviceId 0] [--numIter N]
-//                [--problemType lid] [--dataType float|double] [--re R]
+//                [--problemType lid|sphere] [--dataType float|double] [--re R]
 //                [--scale S] [--benchmark|--visual] [--freq F]
 //                [--storeFine|--storeCoarse|--collisionFusedStore] [--fusedFinest]
 //                [--streamFusedExpl|--streamFusedCoal|--streamFuseAll]
@@ -24,8 +24,9 @@
 // and stream options select kernel fusions that reorder the same arithmetic
 // (with the command line's choices the store is always the fine one): they
 // are recorded in the report, and this build runs its own fused schedule for
-// all of them.  Not part of this build: --deviceType cpu, the sphere and mesh
-// problems, --vtk / --binary / --gui output.
+// all of them.  --problemType sphere runs the flow over a sphere
+// (flowOverShape.h:115-190).  Not part of this build: --deviceType cpu, the
+// mesh problem, --vtk / --binary / --gui output.
 #include "neon_amd.hpp"
 
 #include <algorithm>
@@ -68,7 +69,7 @@ struct Params
 [[noreturn]] void usage(const std::string& why)
 {
     std::cerr << "lbm_multires: " << why << "\n"
-              << "usage: lbm_multires [--deviceType gpu] [--deviceId 0] [--numIter N] [--problemType lid]\n"
+              << "usage: lbm_multires [--deviceType gpu] [--deviceId 0] [--numIter N] [--problemType lid|sphere]\n"
                  "       [--dataType float|double] [--re R] [--scale 0..11] [--benchmark|--visual] [--freq F]\n"
                  "       [--storeFine|--storeCoarse|--collisionFusedStore] [--fusedFinest]\n"
                  "       [--streamFusedExpl|--streamFusedCoal|--streamFuseAll] [--collision kbc|bgk]\n";
@@ -121,7 +122,8 @@ Params parse(int argc, char** argv)
         else usage("unknown option " + a);
     }
     if (p.deviceType != "gpu") usage("only --deviceType gpu is part of this build");
-    if (p.problemType != "lid") usage("only --problemType lid is part of this build");
+    if (p.problemType != "lid" && p.problemType != "sphere")
+        usage("--problemType lid or sphere (the mesh problem is not part of this build)");
     if (p.dataType != "float" && p.dataType != "double") usage("--dataType must be float or double");
     if (p.collision != "kbc" && p.collision != "bgk") usage("--collision must be kbc or bgk");
     if (p.numIter < 0 || p.Re <= 0 || p.freq <= 0) usage("bad numIter, re or freq");
@@ -271,33 +273,28 @@ void writeCentreLines(const mGrid& grid, const Lattice& L, T ulb, int iteration)
     write(xPosVal, "NeonMultiResLBM_" + suffix.str() + "_X.dat");
 }
 
-template <typename T>
-int lidDrivenCavity(const Backend& backend, const Params& params)
+struct Problem
 {
-    const LidGrid  lg = lidGrid(params.scale);
-    const int      depth = int(lg.sdf.size()) - 1;
-    const index_3d gridDim(lg.edge, lg.edge, lg.edge);
-    const index_3d box(gridDim.x - 1, gridDim.y - 1, gridDim.z - 1);
-    std::vector<std::function<bool(const index_3d&)>> activeCellLambda(static_cast<size_t>(depth), nullptr);
-    for (int i = 0; i < depth; i++) {
-        const float hi = float(lg.sdf[size_t(i)]), lo = float(lg.sdf[size_t(i) + 1]);
-        activeCellLambda[size_t(i)] = [=](const index_3d& id) {
-            const float sdf = sdfCube(id, box);
-            return sdf <= hi && sdf > lo;
-        };
-    }
-    const bool  kbc = params.collision == "kbc";
-    const mGrid grid = mGrid::make<T>(backend, gridDim, activeCellLambda, kbc ? mGrid::Collision::KBC
-                                                                              : mGrid::Collision::BGK);
-    // LBM problem (lidDrivenCavity.h:240-244)
-    const T      ulb = 0.04;
-    const T      clength = T(grid.getDimension(depth - 1).x);
-    const T      visclb = ulb * clength / static_cast<T>(params.Re);
-    const T      omega = 1.0 / (3. * visclb + 0.5);
-    const Lattice L = lattice(kbc);
+    std::string         record;  // the report's record name
+    index_3d            gridDim;
+    int                 depth = 0;
+    double              clength = 0;
+    std::array<double, 3> velocity{};  // lid or inlet
+    bool                verify = false;  // centre-line files (the lid problem)
+};
 
-    mGrid g = grid;
-    g.initLidDrivenCavity(double(ulb));
+// runNonUniformLBM (lbmMultiRes.h:266-491): time numIter iterations, print
+// and report MLUPS / EMLUPS, and for the lid problem write the centre lines
+template <typename T>
+int runNonUniformLBM(const Backend& backend, const Params& params, const Problem& pb, mGrid& g, const Lattice& L)
+{
+    const int      depth = pb.depth;
+    const index_3d gridDim = pb.gridDim;
+    const T        ulb = T(pb.velocity[0]);
+    const T        clength = T(pb.clength);
+    const T        visclb = ulb * clength / static_cast<T>(params.Re);
+    const T        omega = 1.0 / (3. * visclb + 0.5);
+
     const std::vector<uint64_t> leaves = g.numLeafVoxels();
     backend.syncAll();
     const auto start = std::chrono::high_resolution_clock::now();
@@ -324,11 +321,11 @@ int lidDrivenCavity(const Backend& backend, const Params& params)
     const std::string typeName = std::is_same_v<T, float> ? "F" : "D";
     auto b = [](bool v) { return v ? "true" : "false"; };
     std::ostringstream vel;
-    vel << "(" << double(ulb) << ", 0, 0)";
+    vel << "(" << pb.velocity[0] << ", " << pb.velocity[1] << ", " << pb.velocity[2] << ")";
     std::ofstream r("MultiResLBM_P" + std::to_string(params.scale) + "_" + algo + "_" + typeName + reportTimeSuffix() +
                     ".json");
     r << std::setprecision(17) << "{\n"
-      << "    \"Record Name\": \"Lid Driven Cavity MultiRes LBM\",\n"
+      << "    \"Record Name\": \"" << pb.record << "\",\n"
       << "    \"date\": \"" << dateString() << "\",\n"
       << "    \"command_line\": \"" << params.commandLine << "\",\n"
       << "    \"DeviceType\": \"gpu\",\n"
@@ -339,7 +336,7 @@ int lidDrivenCavity(const Backend& backend, const Params& params)
       << "    \"problemType\": \"" << params.problemType << "\",\n    \"omega\": " << double(omega) << ",\n"
       << "    \"Re\": " << params.Re << ",\n    \"velocity\": \"" << vel.str() << "\",\n"
       << "    \"clength\": " << double(clength) << ",\n    \"visclb\": " << double(visclb) << ",\n"
-      << "    \"Collision\": \"" << (kbc ? "KBC" : "BGK") << "\",\n"
+      << "    \"Collision\": \"" << (params.collision == "kbc" ? "KBC" : "BGK") << "\",\n"
       << "    \"fineInitStore\": " << b(params.fineInitStore) << ",\n"
       << "    \"streamFusedExpl\": " << b(params.streamFusedExpl) << ",\n"
       << "    \"streamFusedCoal\": " << b(params.streamFusedCoal) << ",\n"
@@ -353,9 +350,80 @@ int lidDrivenCavity(const Backend& backend, const Params& params)
     r << "],\n    \"EMLUPS\": " << effMLUPS << ",\n    \"ENumIter\": " << effNumIter << ",\n"
       << "    \"ENumVoxels\": " << gridDim.rMulTyped() << "\n}\n";
 
-    // the app runs its lid problem with verify on (lidDrivenCavity.h:256-266)
-    writeCentreLines<T>(g, L, ulb, params.numIter);
+    if (pb.verify) writeCentreLines<T>(g, L, ulb, params.numIter);
     return 0;
+}
+
+template <typename T>
+int lidDrivenCavity(const Backend& backend, const Params& params)
+{
+    const LidGrid  lg = lidGrid(params.scale);
+    Problem        pb;
+    pb.record = "Lid Driven Cavity MultiRes LBM";
+    pb.depth = int(lg.sdf.size()) - 1;
+    pb.gridDim = index_3d(lg.edge, lg.edge, lg.edge);
+    pb.velocity = {double(T(0.04)), 0., 0.};
+    pb.verify = true;  // lidDrivenCavity.h:256-266
+    const index_3d box(pb.gridDim.x - 1, pb.gridDim.y - 1, pb.gridDim.z - 1);
+    std::vector<std::function<bool(const index_3d&)>> activeCellLambda(static_cast<size_t>(pb.depth), nullptr);
+    for (int i = 0; i < pb.depth; i++) {
+        const float hi = float(lg.sdf[size_t(i)]), lo = float(lg.sdf[size_t(i) + 1]);
+        activeCellLambda[size_t(i)] = [=](const index_3d& id) {
+            const float sdf = sdfCube(id, box);
+            return sdf <= hi && sdf > lo;
+        };
+    }
+    const bool kbc = params.collision == "kbc";
+    mGrid      g = mGrid::make<T>(backend, pb.gridDim, activeCellLambda, kbc ? mGrid::Collision::KBC
+                                                                        : mGrid::Collision::BGK);
+    pb.clength = double(T(g.getDimension(pb.depth - 1).x));  // lidDrivenCavity.h:241
+    g.initLidDrivenCavity(pb.velocity[0]);
+    return runNonUniformLBM<T>(backend, params, pb, g, lattice(kbc));
+}
+
+// flowOverSphere (flowOverShape.h:115-190): a 136s x 96s x 136s channel,
+// level boxes, a sphere of radius 8s at (52s, 52s, 68s), the inlet at x = 0
+template <typename T>
+int flowOverSphere(const Backend& backend, const Params& params)
+{
+    const int s = params.scale;
+    if (s < 1) usage("--scale must be at least 1 for the sphere");
+    Problem pb;
+    pb.record = "Sphere MultiRes LBM";
+    pb.depth = 3;
+    pb.gridDim = index_3d(136 * s, 96 * s, 136 * s);
+    const int sx = 52 * s, sy = 52 * s, sz = 68 * s, sr = 8 * s;
+    pb.velocity = {double(T(0.04)), 0., 0.};
+    pb.clength = double(T(sr / (1 << (pb.depth - 1))));
+    auto box = [](int x0, int x1, int y0, int y1, int z0, int z1) {
+        return [=](const index_3d& i) { return i.x >= x0 && i.x < x1 && i.y >= y0 && i.y < y1 && i.z >= z0 && i.z < z1; };
+    };
+    const std::vector<std::function<bool(const index_3d&)>> activeCellLambda = {
+        box(40 * s, 96 * s, 40 * s, 64 * s, 40 * s, 96 * s), box(24 * s, 112 * s, 24 * s, 72 * s, 24 * s, 112 * s),
+        [](const index_3d&) { return true; }};
+    const bool kbc = params.collision == "kbc";
+    mGrid      g = mGrid::make<T>(backend, pb.gridDim, activeCellLambda, kbc ? mGrid::Collision::KBC
+                                                                        : mGrid::Collision::BGK);
+    // cell classes (flowOverShape.h:56-83), from each voxel's level-0 origin
+    std::vector<std::vector<uint8_t>> cls(size_t(pb.depth));
+    for (int l = 0; l < pb.depth; l++) {
+        const index_3d d = g.levelDimension(l);
+        auto&          c = cls[size_t(l)];
+        c.assign(d.rMulTyped(), NX_CELL_BULK);
+        size_t i = 0;
+        for (int z = 0; z < d.z; z++)
+            for (int y = 0; y < d.y; y++)
+                for (int x = 0; x < d.x; x++, i++) {
+                    const int gx = x << l, gy = y << l, gz = z << l;
+                    if (gx == 0) c[i] = NX_CELL_MOVINGWALL;  // inlet
+                    const T dx = T(sx - gx), dy = T(sy - gy), dz = T(sz - gz);
+                    if ((dx * dx + dy * dy + dz * dz) < T(sr) * T(sr)) c[i] = NX_CELL_BOUNCEBACK;
+                    if (gy == 0 || gy == pb.gridDim.y - (1 << l) || gz == 0 || gz == pb.gridDim.z - (1 << l))
+                        c[i] = NX_CELL_BOUNCEBACK;
+                }
+    }
+    g.initCells(cls, pb.velocity.data());
+    return runNonUniformLBM<T>(backend, params, pb, g, lattice(kbc));
 }
 
 }  // namespace
@@ -365,6 +433,9 @@ int main(int argc, char** argv)
     const Params params = parse(argc, argv);
     try {
         Backend backend({params.deviceId}, Runtime::stream);
+        if (params.problemType == "sphere")
+            return params.dataType == "float" ? flowOverSphere<float>(backend, params)
+                                              : flowOverSphere<double>(backend, params);
         if (params.dataType == "float") return lidDrivenCavity<float>(backend, params);
         return lidDrivenCavity<double>(backend, params);
     } catch (const NeonException& e) {

@@ -694,13 +694,15 @@ inline int uoff(int x, int d)
     return s <= 0 ? s : s - 1;
 }
 
+// The class of every leaf (cls[l] per level-l voxel; nullptr: the lid-driven
+// cavity of lidDrivenCavity.h:8-78, leaf voxels of level 0 on the domain
+// faces bounce-back, those with y = Y-1 the lid) and its populations: w_q
+// (bulk), 0 (bounce-back), -6 w_q (c_q . uw) for moving-wall cells (the lid,
+// or the inlet of flowOverShape.h:62-104), evaluated as the reference does in
+// T with binary64 constants.
 template <typename T>
-void fill_cavity(nx_mgrid_s& g, double ulb, std::vector<T>* pop)
+void fill_cells(nx_mgrid_s& g, const uint8_t* const* cls_in, const double uw[3], std::vector<T>* pop)
 {
-    // lidDrivenCavity.h:8-78: leaf voxels of level 0 on the domain faces are
-    // bounce-back, those with y = Y-1 the lid; populations w_q / 0 /
-    // -6 w_q (c_q . ulid), ulid = (ulb, 0, 0), evaluated as the reference
-    // does in T with binary64 constants
     const HostLat& L = g.lat;
     for (int l = 0; l < g.depth; l++) {
         Level& lv = g.L[l];
@@ -713,16 +715,18 @@ void fill_cavity(nx_mgrid_s& g, double ulb, std::vector<T>* pop)
                     int cls = NX_CELL_BULK;
                     const int64_t s = lv.slot(x, y, z);
                     if (!(lv.st[i] & kRefined)) {
-                        if (l == 0 && (x == 0 || y == 0 || z == 0 || x == lv.dim[0] - 1 || y == lv.dim[1] - 1 ||
-                                       z == lv.dim[2] - 1))
+                        if (cls_in)
+                            cls = cls_in[l][i];
+                        else if (l == 0 && (x == 0 || y == 0 || z == 0 || x == lv.dim[0] - 1 ||
+                                            y == lv.dim[1] - 1 || z == lv.dim[2] - 1))
                             cls = y == lv.dim[1] - 1 ? NX_CELL_MOVINGWALL : NX_CELL_BOUNCEBACK;
                         for (int q = 0; q < L.Q; q++) {
                             T p = (T)L.w[q];
                             if (cls != NX_CELL_BULK) p = 0;
                             if (cls == NX_CELL_MOVINGWALL) {
-                                p = (T)((double)p + (double)L.c[q][0] * ulb);
-                                p = (T)((double)p + (double)L.c[q][1] * 0.);
-                                p = (T)((double)p + (double)L.c[q][2] * 0.);
+                                p = (T)((double)p + (double)L.c[q][0] * uw[0]);
+                                p = (T)((double)p + (double)L.c[q][1] * uw[1]);
+                                p = (T)((double)p + (double)L.c[q][2] * uw[2]);
                                 p = (T)((double)p * (-6. * L.w[q]));
                             }
                             pop[l][lv.elem(s, q)] = p;
@@ -892,19 +896,19 @@ extern "C" int nx_mgrid_state(nx_mgrid g, uint8_t* st0, uint8_t* st1)
     return NX_OK;
 }
 
-extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
+namespace {
+
+int setup_cells(nx_mgrid g, const uint8_t* const* cls, const double uw[3])
 {
-    if (!g) return set_error(NX_ERR_INVALID, "null mgrid");
-    if (!std::isfinite(ulb)) return set_error(NX_ERR_INVALID, "nx_mgrid_cavity: non-finite lid speed");
     const int D = g->depth;
     const size_t es = g->storage == NX_F64 ? 8 : 4;
     // populations
     std::vector<double> p64[kMaxDepth];
     std::vector<float> p32[kMaxDepth];
     if (g->storage == NX_F64)
-        fill_cavity<double>(*g, ulb, p64);
+        fill_cells<double>(*g, cls, uw, p64);
     else
-        fill_cavity<float>(*g, ulb, p32);
+        fill_cells<float>(*g, cls, uw, p32);
     for (int l = 0; l < D; l++) {
         Level& lv = g->L[l];
         const size_t elems = (size_t)g->lat.Q * kBV * lv.n_blocks;
@@ -982,6 +986,33 @@ extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
     }
     g->ready = true;
     return NX_OK;
+}
+
+}  // namespace
+
+extern "C" int nx_mgrid_cavity(nx_mgrid g, double ulb)
+{
+    if (!g) return set_error(NX_ERR_INVALID, "null mgrid");
+    if (!std::isfinite(ulb)) return set_error(NX_ERR_INVALID, "nx_mgrid_cavity: non-finite lid speed");
+    const double uw[3] = {ulb, 0., 0.};
+    return setup_cells(g, nullptr, uw);
+}
+
+extern "C" int nx_mgrid_setup_cells(nx_mgrid g, const uint8_t* const* cls, const double u_wall[3])
+{
+    if (!g || !cls || !u_wall) return set_error(NX_ERR_INVALID, "nx_mgrid_setup_cells: null argument");
+    for (int l = 0; l < g->depth; l++) {
+        if (!cls[l]) return set_error(NX_ERR_INVALID, "nx_mgrid_setup_cells: null class array");
+        const Level& lv = g->L[l];
+        for (size_t i = 0; i < lv.nvox(); i++)
+            if ((lv.st[i] & kActive) && !(lv.st[i] & kRefined) && cls[l][i] != NX_CELL_BULK &&
+                cls[l][i] != NX_CELL_BOUNCEBACK && cls[l][i] != NX_CELL_MOVINGWALL)
+                return set_error(NX_ERR_INVALID, "nx_mgrid_setup_cells: classes NX_CELL_BULK, NX_CELL_BOUNCEBACK or "
+                                                 "NX_CELL_MOVINGWALL");
+    }
+    for (int a = 0; a < 3; a++)
+        if (!std::isfinite(u_wall[a])) return set_error(NX_ERR_INVALID, "nx_mgrid_setup_cells: non-finite velocity");
+    return setup_cells(g, cls, u_wall);
 }
 
 namespace {

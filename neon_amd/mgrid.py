@@ -75,6 +75,53 @@ LID_SCALES = {
 }
 
 
+def sphere_problem(scale: int = 2):
+    """The multi-resolution app's flow over a sphere (flowOverShape.h:115-190)
+    at --scale s: a 136s x 96s x 136s channel, three levels (level 0 inside
+    [40s, 96s) x [40s, 64s) x [40s, 96s), level 1 inside [24s, 112s) x
+    [24s, 72s) x [24s, 112s), level 2 everywhere), a sphere of radius 8s at
+    (52s, 52s, 68s).  Returns (dim, lambdas per level, classes per level,
+    sphere (cx, cy, cz, r)): the class of every level-l voxel from its level-0
+    origin (flowOverShape.h:56-83): inlet (moving wall with the inlet
+    velocity) at x = 0, bounce-back inside the sphere and on the y and z
+    faces (y = 0 or the level's last voxel, likewise z), bulk elsewhere;
+    the x = X - 1 face is left open."""
+    from .abi import NX_CELL_BULK, NX_CELL_BOUNCEBACK, NX_CELL_MOVINGWALL
+    s_ = int(scale)
+    dim = (136 * s_, 96 * s_, 136 * s_)
+    sph = (52 * s_, 52 * s_, 68 * s_, 8 * s_)
+    boxes = [((40, 96), (40, 64), (40, 96)), ((24, 112), (24, 72), (24, 112))]
+    dims = level_dims(dim, 3)
+    lams, cls = [], []
+    for l, d in enumerate(dims):
+        z, y, x = np.meshgrid(*(np.arange(n) << l for n in d[::-1]), indexing="ij")
+        if l < 2:
+            (x0, x1), (y0, y1), (z0, z1) = boxes[l]
+            lam = (x >= x0 * s_) & (x < x1 * s_) & (y >= y0 * s_) & (y < y1 * s_) & (z >= z0 * s_) & (z < z1 * s_)
+        else:
+            lam = np.ones(x.shape, bool)
+        lams.append(lam.astype(np.uint8).reshape(-1))
+        c = np.full(x.shape, NX_CELL_BULK, np.uint8)
+        c[x == 0] = NX_CELL_MOVINGWALL
+        inside = (sph[0] - x) ** 2 + (sph[1] - y) ** 2 + (sph[2] - z) ** 2 < sph[3] ** 2
+        c[inside] = NX_CELL_BOUNCEBACK
+        wall = (y == 0) | (y == dim[1] - (1 << l)) | (z == 0) | (z == dim[2] - (1 << l))
+        c[wall] = NX_CELL_BOUNCEBACK
+        cls.append(c.reshape(-1))
+    return dim, lams, cls, sph
+
+
+def sphere_parameters(scale: int, re: float, storage: str = "f64", uin: float = 0.04):
+    """(uin, omega0) of the flow over a sphere in the field type T
+    (flowOverShape.h:142-147): clength = radius / 2^(depth-1) with depth 3,
+    nu = uin clength / Re, omega0 = 1 / (3 nu + 1/2)."""
+    T = np.float64 if storage == "f64" else np.float32
+    u = T(uin)
+    clength = T((8 * int(scale)) // 4)
+    nu = T(u * clength / T(re))
+    return float(u), float(T(1.0 / (3.0 * float(nu) + 0.5)))
+
+
 def cavity_parameters(dim, re: float, storage: str = "f64", ulb: float = 0.04, depth: int = 2):
     """(ulb, omega0) of the refined cavity in the field type T
     (lidDrivenCavity.h:246-251): ulb = 0.04 in T, the characteristic length is
@@ -154,6 +201,18 @@ class MGrid:
     def setup_cavity(self, ulb: float) -> None:
         self._init()
         check(lib.nx_mgrid_cavity(self.h, float(ulb)))
+
+    def setup_cells(self, cls, u_wall) -> None:
+        """Set-up with the class of every leaf per level (cls[l]: uint8 per
+        level-l voxel, NX_CELL_BULK / BOUNCEBACK / MOVINGWALL) and the
+        moving-wall velocity (lid or inlet)."""
+        self._init()
+        cls = [np.ascontiguousarray(c, np.uint8).reshape(-1) for c in cls]
+        for c, d in zip(cls, self.dims):
+            if c.size != int(np.prod(d)):
+                raise ValueError(f"class array of {c.size} voxels for a level of {d}")
+        ptrs = (C.c_void_p * self.depth)(*[c.ctypes.data for c in cls])
+        check(lib.nx_mgrid_setup_cells(self.h, ptrs, (C.c_double * 3)(*[float(v) for v in u_wall])))
 
     def run(self, n: int, omega0: float, stream=None) -> None:
         check(lib.nx_mgrid_step(self.h, float(omega0), int(n), stream))

@@ -107,6 +107,9 @@ void or_mres_dims(const int32_t dim[3], int depth, int32_t* dims);
 int or_mres_build(const int32_t dim[3], int depth, const uint8_t* const* lam, uint8_t* const* st);
 void or_mres_setup(const int32_t dim[3], int depth, int lat, int storage, double ulb, uint8_t* const* st,
                    void* const* fin, void* const* fout, float* const* ss);
+void or_mres_setup_cells(const int32_t dim[3], int depth, int lat, int storage, const uint8_t* const* cls,
+                         const double uw[3], uint8_t* const* st, void* const* fin, void* const* fout,
+                         float* const* ss);
 double or_mres_omega(double omega0, int level, int num_levels);
 void or_mres_step(const int32_t dim[3], int depth, int lat, int s, const uint8_t* const* st, void* const* fin,
                   void* const* fout, const float* const* ss, double omega0);

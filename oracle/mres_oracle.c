@@ -289,6 +289,21 @@ static inline double get(const void* p, int s, size_t i)
 void or_mres_setup(const int32_t dim[3], int depth, int lat, int storage, double ulb, uint8_t* const* st,
                    void* const* fin, void* const* fout, float* const* ss)
 {
+    const double uw[3] = {ulb, 0., 0.};
+    or_mres_setup_cells(dim, depth, lat, storage, NULL, uw, st, fin, fout, ss);
+}
+
+/* The set-up with the cell class of every level-l leaf given by cls[l]
+ * (OR_BULK / OR_BOUNCEBACK / OR_MOVINGWALL per level-l voxel, x fastest;
+ * cls == NULL: the lid-driven cavity) and the velocity uw of the moving-wall
+ * cells (the lid, or the inlet of flowOverShape.h:62-104, whose cells the
+ * stream treats alike).  Populations: w_q (bulk), 0 (bounce-back),
+ * -6 w_q (c_q . uw) (moving wall), accumulated over the three components in
+ * T as the reference does. */
+void or_mres_setup_cells(const int32_t dim[3], int depth, int lat, int storage, const uint8_t* const* cls_in,
+                         const double uw[3], uint8_t* const* st, void* const* fin, void* const* fout,
+                         float* const* ss)
+{
     const mr_lat* L = lat_of(lat);
     const int Q = L->Q;
     int32_t dims[3 * OR_MRES_MAX_DEPTH];
@@ -307,8 +322,10 @@ void or_mres_setup(const int32_t dim[3], int depth, int lat, int storage, double
                         put(fout[lvl], storage, (size_t)q * n + i, 0.);
                     }
                     if (!(st[lvl][i] & MR_REFINED)) {
-                        if (lvl == 0 && (x == 0 || x == d[0] - 1 || y == 0 || y == d[1] - 1 || z == 0 ||
-                                         z == d[2] - 1)) {
+                        if (cls_in) {
+                            cls = cls_in[lvl][i];
+                        } else if (lvl == 0 && (x == 0 || x == d[0] - 1 || y == 0 || y == d[1] - 1 || z == 0 ||
+                                                z == d[2] - 1)) {
                             cls = OR_BOUNCEBACK;
                             if (y == d[1] - 1) cls = OR_MOVINGWALL;
                         }
@@ -319,9 +336,9 @@ void or_mres_setup(const int32_t dim[3], int depth, int lat, int storage, double
                                 if (cls == OR_BOUNCEBACK) p = 0;
                                 if (cls == OR_MOVINGWALL) {
                                     p = 0;
-                                    p += (double)L->c[q][0] * ulb;
-                                    p += (double)L->c[q][1] * 0.;
-                                    p += (double)L->c[q][2] * 0.;
+                                    p += (double)L->c[q][0] * uw[0];
+                                    p += (double)L->c[q][1] * uw[1];
+                                    p += (double)L->c[q][2] * uw[2];
                                     p *= -6. * L->w(q);
                                 }
                                 v = p;
@@ -330,9 +347,9 @@ void or_mres_setup(const int32_t dim[3], int depth, int lat, int storage, double
                                 if (cls == OR_BOUNCEBACK) p = 0;
                                 if (cls == OR_MOVINGWALL) {
                                     p = 0;
-                                    p = (float)((double)p + (double)L->c[q][0] * ulb);
-                                    p = (float)((double)p + (double)L->c[q][1] * 0.);
-                                    p = (float)((double)p + (double)L->c[q][2] * 0.);
+                                    p = (float)((double)p + (double)L->c[q][0] * uw[0]);
+                                    p = (float)((double)p + (double)L->c[q][1] * uw[1]);
+                                    p = (float)((double)p + (double)L->c[q][2] * uw[2]);
                                     p = (float)((double)p * (-6. * L->w(q)));
                                 }
                                 v = (double)p;

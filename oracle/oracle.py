@@ -72,6 +72,7 @@ _SIG = {
     "or_mres_dims": (None, [_P, C.c_int, _P]),
     "or_mres_build": (C.c_int, [_P, C.c_int, _P, _P]),
     "or_mres_setup": (None, [_P, C.c_int, C.c_int, C.c_int, _D, _P, _P, _P, _P]),
+    "or_mres_setup_cells": (None, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _P, _P]),
     "or_mres_omega": (_D, [_D, C.c_int, C.c_int]),
     "or_mres_step": (None, [_P, C.c_int, C.c_int, C.c_int, _P, _P, _P, _P, _D]),
     "or_kbc27_collide": (None, [C.c_int, _P, _P, _D]),
@@ -361,6 +362,20 @@ class MresOracle:
         self.ss1 = self.ss[1]
         lib.or_mres_setup(self._d0, self.depth, self.lattice, self.storage, float(ulb), self._ptrs(self.st),
                           self._ptrs(self.fin), self._ptrs(self.fout), self._ptrs(self.ss))
+
+    def setup_cells(self, cls, uw):
+        """Set-up with the class of every leaf per level (cls[l], uint8 per
+        level-l voxel) and the moving-wall velocity uw (lid or inlet)."""
+        dt = NP_STORAGE[self.storage]
+        self.fin = [np.zeros(self.Q * n, dt) for n in self.n]
+        self.fout = [np.zeros(self.Q * n, dt) for n in self.n]
+        self.ss = [None] + [np.zeros(self.Q * n, np.float32) for n in self.n[1:]]
+        self.ss1 = self.ss[1]
+        cls = [np.ascontiguousarray(c, np.uint8).reshape(-1) for c in cls]
+        assert all(c.size == n for c, n in zip(cls, self.n))
+        lib.or_mres_setup_cells(self._d0, self.depth, self.lattice, self.storage, self._ptrs(cls),
+                                (C.c_double * 3)(*[float(v) for v in uw]), self._ptrs(self.st),
+                                self._ptrs(self.fin), self._ptrs(self.fout), self._ptrs(self.ss))
 
     def classes(self, level):
         return (self.st[level] >> MR_CLS_SHIFT) & 3

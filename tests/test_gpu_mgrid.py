@@ -432,3 +432,59 @@ def test_mgrid_random_grids_equal_oracle(nx, orc, seed, storage, lattice):
             assert np.array_equal(np.isnan(got), nan), (lvl, which)
             n_diff = int(np.count_nonzero(got[~nan].view(np.uint8) != want[~nan].view(np.uint8)))
             assert n_diff == 0, (lvl, which, n_diff)
+
+
+@pytest.mark.parametrize("storage,lattice", [("f64", "kbc27"), ("f32", "kbc27"), ("f64", "bgk19")])
+def test_mgrid_sphere_equals_oracle(nx, orc, storage, lattice):
+    """The app's flow over a sphere (flowOverShape.h:115-190, --scale 1, Re
+    100): walls on every level, the inlet as moving-wall cells, the sphere's
+    bounce-back voxels; every population of both fields of every level bit
+    for bit with the oracle after 3 coarse steps."""
+    dim, lams, cls, _ = mg.sphere_problem(1)
+    uin, om0 = mg.sphere_parameters(1, 100.0, storage)
+    print(f"mgrid sphere {dim} {storage} {lattice} omega0={om0:.6f}", flush=True)
+    g = mg.MGrid(dim, lams, storage, lattice=lattice)
+    g.setup_cells(cls, (uin, 0.0, 0.0))
+    m = orc.MresOracle(dim, lams, orc.F64 if storage == "f64" else orc.F32,
+                       lattice=orc.KBC27 if lattice == "kbc27" else orc.BGK19)
+    m.setup_cells(cls, (uin, 0.0, 0.0))
+    for l, st in enumerate(g.state()):
+        assert np.array_equal(st, m.st[l]), l
+    g.run(3, om0)
+    m.run(3, om0)
+    for lvl in range(3):
+        act = np.tile(m.active(lvl), m.Q)
+        for which, ref in ((0, m.fin[lvl]), (1, m.fout[lvl])):
+            got = g.fields(lvl, which)
+            n_diff = int(np.count_nonzero(got[act].view(np.uint8) != ref[act].view(np.uint8)))
+            assert n_diff == 0, (lvl, which, n_diff)
+
+
+def test_mgrid_sphere_kbc_runs_to_a_wake(nx):
+    """The app's default sphere run (--scale 2, Re 100, KBC D3Q27, float),
+    3000 coarse steps (12000 on level 0): finite everywhere; the channel
+    carries the inlet's flow past the sphere -- u_x on the centre line far
+    behind the sphere is positive, and the wake right behind it is slower
+    than the flow beside it."""
+    dim, lams, cls, sph = mg.sphere_problem(2)
+    uin, om0 = mg.sphere_parameters(2, 100.0, "f32")
+    g = mg.MGrid(dim, lams, "f32", lattice="kbc27")
+    g.setup_cells(cls, (uin, 0.0, 0.0))
+    g.run(3000, om0)
+    from oracle.oracle import D3Q27_C
+    c = D3Q27_C.astype(np.float64)
+    st = g.state()
+    d = g.dims[0]
+    f = g.fields(0).reshape(27, d[2], d[1], d[0]).astype(np.float64)
+    assert all(np.isfinite(g.fields(l)).all() for l in range(3))
+    s0 = np.asarray(st[0]).reshape(d[2], d[1], d[0])
+    cx, cy, cz, r = sph
+
+    def ux(x, y, z):
+        assert (s0[z, y, x] & 3) == 1 and ((s0[z, y, x] >> 2) & 3) == 2, (x, y, z)
+        p = f[:, z, y, x]
+        return float(c[:, 0] @ p / p.sum())
+    behind, beside = ux(cx + r + 4, cy, cz), ux(cx, cy, cz + r + 6)
+    far = ux(188, cy, cz)
+    print(f"sphere wake: u_x behind {behind / uin:.3f} U, beside {beside / uin:.3f} U, downstream {far / uin:.3f} U")
+    assert far > 0 and beside > behind

@@ -3,7 +3,9 @@
 reference app's command line (apps/lbmMultiRes/lbmMultiRes.cu:53-95), its
 centre-line files must match the reference's own output of the same run,
 tests/golden/NeonMultiResLBM_5000_{X,Y}.dat (--scale 1, Re 100, KBC D3Q27,
-float, 5000 iterations), and its report carries the reference's members."""
+float, 5000 iterations), and its report carries the reference's members;
+the flow over a sphere (flowOverShape.h:115-190) runs and reports the same
+leaves as the oracle's grid of that problem."""
 import glob
 import json
 import pathlib
@@ -18,7 +20,7 @@ GOLDEN = ROOT / "tests" / "golden"
 
 
 def test_cli_rejects_what_is_not_built(tmp_path):
-    for extra in (["--deviceType", "cpu"], ["--problemType", "sphere"], ["--dataType", "half"],
+    for extra in (["--deviceType", "cpu"], ["--problemType", "mesh"], ["--dataType", "half"],
                   ["--vtk"], ["--collision", "mrt"], ["--numIter", "x"]):
         r = subprocess.run([str(EXE), *extra], cwd=tmp_path, capture_output=True, text=True, timeout=60)
         assert r.returncode == 2 and "usage" in r.stderr, extra
@@ -49,3 +51,28 @@ def test_app_reproduces_reference_multires_output(tmp_path):
     assert rep["Grid Size X"] == 160 and rep["clength"] == 40 and rep["Algorithm"] == "CH-SEO+"
     assert len(rep["NumActiveVoxels"]) == 3 and rep["MLUPS"] > 0 and rep["EMLUPS"] > rep["MLUPS"]
     assert abs(rep["omega"] - 1.0 / (3 * 0.04 * 40 / 100 + 0.5)) < 1e-6
+
+
+@pytest.mark.gpu
+def test_app_runs_the_flow_over_a_sphere(tmp_path):
+    import neon_amd.mgrid as mg
+    from oracle.oracle import MresOracle
+    cmd = [str(EXE), "--numIter", "40", "--deviceType", "gpu", "--deviceId", "0", "--problemType", "sphere",
+           "--benchmark", "--dataType", "float", "--re", "100", "--scale", "1"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=200)
+    assert r.returncode == 0, r.stdout + r.stderr
+    print(r.stdout)
+    assert not glob.glob(str(tmp_path / "*.dat"))  # no centre lines for the sphere
+    reports = glob.glob(str(tmp_path / "MultiResLBM_P1_CH-SEO+_F_D*.json"))
+    assert len(reports) == 1
+    rep = json.load(open(reports[0]))
+    assert rep["Record Name"] == "Sphere MultiRes LBM" and rep["problemType"] == "sphere"
+    assert (rep["Grid Size X"], rep["Grid Size Y"], rep["Grid Size Z"]) == (136, 96, 136)
+    assert rep["Depth"] == 3 and rep["clength"] == 2 and rep["Collision"] == "KBC"
+    uin, om0 = mg.sphere_parameters(1, 100.0, "f32")
+    assert abs(rep["omega"] - om0) < 1e-6
+    dim, lams, _, _ = mg.sphere_problem(1)
+    o = MresOracle(dim, lams)
+    leaves = [int((o.active(l) & ~o.refined(l)).sum()) if l else int(o.active(0).sum()) for l in range(3)]
+    assert rep["NumActiveVoxels"] == leaves

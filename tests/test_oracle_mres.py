@@ -495,3 +495,48 @@ def test_random_grids_builder_matches_oracle(orc, seed):
     for l, st in enumerate(g.state()):
         assert np.array_equal(st & 3, m.st[l] & 3), l
     assert _balance_violations(m) == 0
+
+
+# ---- the flow over a sphere (the app's second problem) -------------------------
+
+def test_sphere_problem_geometry():
+    """flowOverShape.h:115-190 at --scale 1: the domain and level boxes, the
+    inlet at x = 0, the sphere as the integer points strictly inside radius 8
+    around (52, 52, 68) on level 0, and the bounce-back y / z faces on every
+    level at 0 and at the level's last voxel."""
+    from neon_amd.abi import NX_CELL_BULK, NX_CELL_BOUNCEBACK, NX_CELL_MOVINGWALL
+    dim, lams, cls, sph = mg.sphere_problem(1)
+    assert dim == (136, 96, 136) and sph == (52, 52, 68, 8)
+    d0 = mg.level_dims(dim, 3)
+    l0 = lams[0].reshape(d0[0][2], d0[0][1], d0[0][0])
+    assert l0.sum() == 56 * 24 * 56 and l0[40:96, 40:64, 40:96].all()
+    assert lams[2].all()
+    c0 = cls[0].reshape(d0[0][2], d0[0][1], d0[0][0])
+    z, y, x = np.mgrid[0:136, 0:96, 0:136]
+    ball = (52 - x) ** 2 + (52 - y) ** 2 + (68 - z) ** 2 < 64
+    assert ((c0 == NX_CELL_BOUNCEBACK) & ball).sum() == ball.sum() == 2103  # lattice points with r^2 < 64
+    assert (c0[1:-1, 1:-1, 0] == NX_CELL_MOVINGWALL).all()
+    for l, (c, d) in enumerate(zip(cls, d0)):
+        c = c.reshape(d[2], d[1], d[0])
+        assert (c[:, 0, :] == NX_CELL_BOUNCEBACK).all() and (c[:, -1, :] == NX_CELL_BOUNCEBACK).all()
+        assert (c[0] == NX_CELL_BOUNCEBACK).all() and (c[-1] == NX_CELL_BOUNCEBACK).all()
+        assert (c[1:-1, 1:-1, -1] != NX_CELL_BOUNCEBACK).any()  # the outlet face stays open
+
+
+def test_sphere_flow_kbc_oracle_is_driven_by_the_inlet(orc):
+    """KBC D3Q27 on the sphere grid (--scale 1, Re 100, binary64), 60 coarse
+    steps: finite everywhere, mass enters (the level-0 leaves gain mass), and
+    the flow reaches the sphere's box: the largest level-0 u_x exceeds a tenth
+    of the inlet speed."""
+    dim, lams, cls, _ = mg.sphere_problem(1)
+    uin, om0 = mg.sphere_parameters(1, 100.0)
+    m = orc.MresOracle(dim, lams, orc.F64, lattice=orc.KBC27)
+    m.setup_cells(cls, (uin, 0.0, 0.0))
+    m.run(60, om0)
+    c = _d3q27()[0]
+    for l in range(3):
+        assert np.isfinite(m.fin[l]).all(), l
+    leaf = m.active(0) & (m.classes(0) == orc.BULK)
+    f = m.fin[0].reshape(27, -1)[:, leaf]
+    ux = (c[:, 0] @ f) / f.sum(0)
+    assert ux.max() > 0.1 * uin
