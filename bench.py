@@ -53,6 +53,10 @@ POROUS = {
     # name: (N, storage, compute, default steps, default warmup)
     "porous384_f32": (384, "f32", "f32", 200, 20),
     "porous384_f16dev": (384, "f16dev", "f32", 200, 20),
+    # the same step with the block neighbourhood staged in LDS (one workgroup
+    # per 2 x 2 x 1 tile of blocks, nx_d3q19_step_block_tiled)
+    "porous384_f32_tiled": (384, "f32", "f32", 200, 20, True),
+    "porous384_f16dev_tiled": (384, "f16dev", "f32", 200, 20, True),
 }
 
 
@@ -266,7 +270,8 @@ def shifted_copy_calibration(nx, N=256, reps=20):
 
 
 def porous(nx, a):
-    N, storage, compute, dsteps, dwarm = POROUS[a.workload]
+    N, storage, compute, dsteps, dwarm = POROUS[a.workload][:5]
+    tiled = len(POROUS[a.workload]) > 5 and POROUS[a.workload][5]
     steps = dsteps if a.steps is None else a.steps
     warmup = dwarm if a.warmup is None else a.warmup
     nu, g = 0.02, (1e-6, 0.0, 0.0)
@@ -289,6 +294,8 @@ def porous(nx, a):
         return lat
 
     lat = make(storage, compute)
+    if tiled:
+        lat.set_tiled(True)
     lat.run(warmup, omega, force=g, stream=stream.handle)
     stream.sync()
     e0, e1 = nx.Event(), nx.Event()
@@ -328,14 +335,17 @@ def porous(nx, a):
                                f"{storage} storage + {compute} collision, {steps} timed steps "
                                f"after {warmup} warm-up",
                    "grid": "bGrid", "N": N, "blocks": grid.n_blocks, "candidate_blocks": nb3,
+                   "step": ("LDS-staged, one workgroup per 2x2x1 tile of blocks" if tiled
+                            else "one workgroup per block"),
                    "active_block_fraction": round(grid.n_blocks / nb3, 4),
                    "fluid_voxels": fluid, "fluid_fraction": round(fluid / N ** 3, 4),
                    "setup_s": round(t_setup, 3), "device": nx.device_name()},
         "wall_s": round(wall, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(f"d3q19_block_{storage}_{N}"),
-                     "kernel": f"k_step<block,{storage},{compute}> ({b_alg} B/fluid voxel)"},
+                     "traffic": traffic_from_profiles(f"d3q19_block{'_tiled' if tiled else ''}_{storage}_{N}"),
+                     "kernel": f"{'k_step_tiled' if tiled else 'k_step'}<block,{storage},{compute}> "
+                               f"({b_alg} B/fluid voxel)"},
         "velocity_error_vs_f64": err,
         "cpu_baseline": None,
     }

@@ -93,6 +93,7 @@ extern "C" {
 typedef struct nx_stream_s* nx_stream;
 typedef struct nx_event_s* nx_event;
 typedef struct nx_bgrid_s* nx_bgrid;
+typedef struct nx_block_tiles_s* nx_block_tiles;
 typedef struct nx_container_s* nx_container;
 typedef struct nx_batch_s* nx_batch;
 typedef struct nx_mgrid_s* nx_mgrid;
@@ -319,6 +320,21 @@ int nx_d3q19_step_block(const nx_block_desc* d, const void* fin, void* fout,
  * contiguous range of blocks, the default; 0: hardware round-robin).  No
  * counterpart in the reference. */
 int nx_d3q19_set_block_remap(int on);
+/* The same step with the block neighbourhood staged in LDS (north-star
+ * subsystem b; the reference reads neighbour blocks through
+ * bPartition::getNghData, bPartition_imp.h:218-309): one workgroup per tile
+ * of up to 2 x 2 x 1 blocks reads each member's population planes whole and
+ * the tile's halo from the neighbour blocks once, and every voxel takes its
+ * streamed values from LDS.  Bit for bit equal to nx_d3q19_step_block.
+ * nx_block_tiles_create builds the tile table of a descriptor (blocking
+ * copies of the origins); the table belongs to that descriptor's block
+ * tables.  2- and 4-byte storage only (NX_ERR_UNSUPPORTED for NX_F64). */
+int nx_block_tiles_create(const nx_block_desc* d, nx_block_tiles* t);
+int nx_block_tiles_info(nx_block_tiles t, uint32_t* n_tiles);
+int nx_block_tiles_destroy(nx_block_tiles t);
+int nx_d3q19_step_block_tiled(const nx_block_desc* d, nx_block_tiles t, const void* fin,
+                              void* fout, const uint32_t* flags, const nx_lbm_params* p,
+                              nx_stream s);
 int nx_d3q19_moments_block(const nx_block_desc* d, const void* fin,
                            const uint32_t* flags, void* rho, void* u, nx_stream s);
 int nx_d3q19_mass_block(const nx_block_desc* d, const void* fin,

@@ -131,6 +131,10 @@ SIGNATURES = {
     "nx_d3q19_fill_block": (_I, [_BP, _P, _P, _D, C.POINTER(_D), _P]),
     "nx_d3q19_wall_mask_block": (_I, [_BP, _P, _P, _P]),
     "nx_d3q19_step_block": (_I, [_BP, _P, _P, _P, _PP, _P]),
+    "nx_block_tiles_create": (_I, [_BP, C.POINTER(_P)]),
+    "nx_block_tiles_info": (_I, [_P, C.POINTER(_U32)]),
+    "nx_block_tiles_destroy": (_I, [_P]),
+    "nx_d3q19_step_block_tiled": (_I, [_BP, _P, _P, _P, _P, _PP, _P]),
     "nx_d3q19_set_block_remap": (_I, [_I]),
     "nx_geometry_spheres": (_I, [C.POINTER(_I32), C.c_uint32, _D, _D, C.c_uint64, _I32, _P]),
     "nx_geometry_cavity": (_I, [C.POINTER(_I32), C.c_uint32, _I32, _I32, C.c_uint64, _P]),

@@ -17,6 +17,8 @@
 // bPartition::helpGetNghIdx / helpNghPitch (bPartition_imp.h:109-124,
 // 218-309).
 #include <algorithm>
+#include <utility>
+#include <vector>
 #include <type_traits>
 
 #include "block_common.hpp"
@@ -115,6 +117,66 @@ __device__ __forceinline__ bool stage(const BGeom& g, uint32_t b, uint32_t* smas
     return bit_of(smask, threadIdx.x);
 }
 
+// The rest of the step of one active bulk voxel once its 19 streamed values
+// val[q] (upstream voxel's population q, raw storage) are loaded: masked
+// links fixed up with one batch of loads (the wall's value only where the
+// link can have an upstream voxel: no NX_FLAG_OPEN, or NX_FLAG_MIXED), then
+// the collision and the 19 stores.
+template <int S, int CM, bool FORCE>
+__device__ __forceinline__ void bulk_voxel(const typename Stor<S>::T* __restrict__ fin,
+                                           typename Stor<S>::T* __restrict__ fout, const BGeom& g,
+                                           const Par& p, uint32_t b, const Conn& cb, int x, int y,
+                                           int z, uint64_t own, uint32_t fl,
+                                           const typename Stor<S>::T val[Q])
+{
+    using T = typename Stor<S>::T;
+    using P = typename Stor<S>::P;
+    using R = typename Real<S, CM>::R;
+    P f[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) f[q] = Stor<S>::bulk(val[q], q);
+    const uint32_t m = fl & NX_MASK_BITS;
+    if (m) {
+        // loads only for the masked links, issued as one batch (predicated
+        // loads, no wait between them); the wall's value only where the
+        // link can have an upstream voxel (not NX_FLAG_OPEN, or MIXED)
+        const bool open = (fl & NX_FLAG_OPEN) != 0, mixed = (fl & NX_FLAG_MIXED) != 0;
+        const bool need_nb = !open || mixed;
+        T ownv[Q], nbv[Q];
+        uint32_t okm = 0;
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            if constexpr (q != 9) {
+                const bool w = (m >> q) & 1u;
+                ownv[q] = w ? fin[own + (uint64_t)OPP[q] * kBV] : T(0);
+                bool ok = false;
+                uint32_t t = kInvalidBlock, nv = 0;
+                if (w && need_nb) {
+                    upstream<q>(b, cb, x, y, z, t, nv);
+                    ok = !open && t != kInvalidBlock;
+                    if (mixed)
+                        ok = t != kInvalidBlock &&
+                             ((g.masks[(uint64_t)t * 16 + (nv >> 5)] >> (nv & 31u)) & 1u);
+                }
+                okm |= (ok ? 1u : 0u) << q;
+                nbv[q] = ok ? fin[(uint64_t)t * kBStride + (uint32_t)OPP[q] * kBV + nv] : T(0);
+            }
+        });
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            if (q == 9) continue;
+            const P up = ((okm >> q) & 1u) ? Stor<S>::raw(nbv[q]) : P(0);
+            const P fixed = Stor<S>::bulk(ownv[q], OPP[q]) + up;
+            f[q] = ((m >> q) & 1u) ? fixed : f[q];
+        }
+    }
+    R out[Q];
+    collide<S, CM, FORCE>(f, p.omega, p.g, out);
+#pragma unroll
+    for (int q = 0; q < Q; q++)
+        __builtin_nontemporal_store(Stor<S>::template enc<R>(out[q], q), &fout[own + q * kBV]);
+}
+
 // Step of one block per workgroup.  The 19 population loads of an active
 // voxel are issued together with its flag word (their addresses depend only
 // on the block's connectivity row, read with scalar loads); inactive voxels
@@ -134,8 +196,6 @@ __global__ void __launch_bounds__(kBV)
            const uint32_t* __restrict__ flags, BGeom g, Par p, int remap)
 {
     using T = typename Stor<S>::T;
-    using P = typename Stor<S>::P;
-    using R = typename Real<S, CM>::R;
     const uint32_t b = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const uint32_t v = threadIdx.x;
     const int x = (int)(v & 7u), y = (int)((v >> 3) & 7u), z = (int)(v >> 6);
@@ -172,55 +232,277 @@ __global__ void __launch_bounds__(kBV)
         nx_pin(fl);
     }
     if (active && cls_of(fl) == NX_CELL_BULK) {
-        P f[Q];
-#pragma unroll
-        for (int q = 0; q < Q; q++) f[q] = Stor<S>::bulk(val[q], q);
-        const uint32_t m = fl & NX_MASK_BITS;
-        if (m) {
-            // loads only for the masked links, issued as one batch (predicated
-            // loads, no wait between them); the wall's value only where the
-            // link can have an upstream voxel (not NX_FLAG_OPEN, or MIXED)
-            const bool open = (fl & NX_FLAG_OPEN) != 0, mixed = (fl & NX_FLAG_MIXED) != 0;
-            const bool need_nb = !open || mixed;
-            T ownv[Q], nbv[Q];
-            uint32_t okm = 0;
-            for_q(AllQ{}, [&](auto qc) {
-                constexpr int q = decltype(qc)::value;
-                if constexpr (q != 9) {
-                    const bool w = (m >> q) & 1u;
-                    ownv[q] = w ? fin[own + (uint64_t)OPP[q] * kBV] : T(0);
-                    bool ok = false;
-                    uint32_t t = kInvalidBlock, nv = 0;
-                    if (w && need_nb) {
-                        upstream<q>(b, cb, x, y, z, t, nv);
-                        ok = !open && t != kInvalidBlock;
-                        if (mixed)
-                            ok = t != kInvalidBlock &&
-                                 ((g.masks[(uint64_t)t * 16 + (nv >> 5)] >> (nv & 31u)) & 1u);
-                    }
-                    okm |= (ok ? 1u : 0u) << q;
-                    nbv[q] = ok ? fin[(uint64_t)t * kBStride + (uint32_t)OPP[q] * kBV + nv] : T(0);
-                }
-            });
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                if (q == 9) continue;
-                const P up = ((okm >> q) & 1u) ? Stor<S>::raw(nbv[q]) : P(0);
-                const P fixed = Stor<S>::bulk(ownv[q], OPP[q]) + up;
-                f[q] = ((m >> q) & 1u) ? fixed : f[q];
-            }
-        }
-        R out[Q];
-        collide<S, CM, FORCE>(f, p.omega, p.g, out);
-#pragma unroll
-        for (int q = 0; q < Q; q++)
-            __builtin_nontemporal_store(Stor<S>::template enc<R>(out[q], q), &fout[own + q * kBV]);
+        bulk_voxel<S, CM, FORCE>(fin, fout, g, p, b, cb, x, y, z, own, fl, val);
     } else if (!active && line_any) {
         // inactive voxel in a line that also holds active voxels: write the
         // inactive value so the line is written whole
 #pragma unroll
         for (int q = 0; q < Q; q++) __builtin_nontemporal_store(T(0), &fout[own + q * kBV]);
     }
+}
+
+// ---- the step with the block neighbourhood staged in LDS --------------------
+//
+// One workgroup per tile: the up to four blocks with origins (16i + 8a,
+// 16j + 8c, 8k), a, c in {0, 1}, member a + 2c (absent members are
+// kInvalidBlock in the tile table).  For each population with an x or y
+// component (16 of 19) the workgroup reads the members' planes whole -- two
+// x-adjacent voxels per lane, every 128-byte line once -- and the tile's halo
+// -- the face and edge voxels of the neighbour blocks that stream into the
+// tile, one lane per face voxel of the tile -- and writes each value into LDS
+// at the voxel it streams to (the upstream rule of bPartition_imp.h:218-309
+// applied from the source side).  After one barrier every voxel finds its 16
+// streamed values at its own LDS index.  A face shared by two members is
+// fetched once instead of once per block: per 64 voxels about 32 line
+// requests instead of 43 at 2-byte storage.  The rest population and the two
+// z populations are read in place (their sources are whole lines already),
+// with the flag words, before the staging loads.  In a tile with an absent
+// member, a voxel whose unmasked link starts in that member's extent (only a
+// periodic wrap makes this happen: the upstream block is then another one)
+// reads that value through its block's connectivity row.  The arithmetic
+// after the loads is bulk_voxel, as in k_step, so both steps agree bit for
+// bit.
+constexpr int kTileB = 4;                 // blocks per tile (2 x 2 x 1)
+constexpr int kTileV = kTileB * kBV;      // voxels per tile
+constexpr int kStaged = 16;               // populations staged in LDS
+constexpr int kSlot[Q] = {0, 1, -1, 2, 3, 4, 5, 6, 7, -1, 8, 9, -1, 10, 11, 12, 13, 14, 15};
+
+template <int S>
+constexpr size_t tiled_lds_bytes()
+{
+    return (size_t)kStaged * kTileV * sizeof(typename Stor<S>::T);
+}
+
+// the halo voxel of population q loaded by lane l of wavefront w: wavefronts
+// 0-1 the tile's x face (member row c = w), 2-3 its y face (member column
+// a = w - 2), 4-7 its z face (member w - 4); in-block destination (hx, hy, hz)
+template <int q>
+__device__ __forceinline__ bool halo_of(uint32_t w, uint32_t l, uint32_t& m, int& hx, int& hy, int& hz)
+{
+    constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
+    if (w < 2u) {
+        if constexpr (cx == 0) return false;
+        m = (cx > 0 ? 0u : 1u) + 2u * w;
+        hx = cx > 0 ? 0 : 7;
+        hy = (int)(l & 7u);
+        hz = (int)(l >> 3);
+        return true;
+    }
+    if (w < 4u) {
+        if constexpr (cy == 0) return false;
+        m = (w - 2u) + 2u * (cy > 0 ? 0u : 1u);
+        hx = (int)(l & 7u);
+        hy = cy > 0 ? 0 : 7;
+        hz = (int)(l >> 3);
+        return true;
+    }
+    if constexpr (cz == 0) return false;
+    m = w - 4u;
+    hx = (int)(l & 7u);
+    hy = (int)(l >> 3);
+    hz = cz > 0 ? 0 : 7;
+    return true;
+}
+
+// the member block ids of a tile as four values (selected by value, never
+// through an indexed private array, which would live in scratch memory)
+struct Members {
+    uint32_t m0, m1, m2, m3;
+    template <int k>
+    __device__ __forceinline__ uint32_t at() const
+    {
+        if constexpr (k == 0) return m0;
+        if constexpr (k == 1) return m1;
+        if constexpr (k == 2) return m2;
+        return m3;
+    }
+};
+
+__device__ __forceinline__ uint32_t sel4(uint32_t m, uint32_t a, uint32_t b, uint32_t c, uint32_t d)
+{
+    return m == 0 ? a : m == 1 ? b : m == 2 ? c : d;
+}
+
+using StagedA = std::integer_sequence<int, 0, 1, 3, 4, 5, 6, 7, 8>;
+using StagedB = std::integer_sequence<int, 10, 11, 13, 14, 15, 16, 17, 18>;
+
+// Loads of the staged populations qs -- member planes (lane pair index
+// t & 255 of members mh and 2 + mh) and one halo voxel per lane -- then each
+// value written to the LDS index of the voxel it streams to.
+template <int S, int... qs>
+__device__ __forceinline__ void stage_batch(std::integer_sequence<int, qs...>,
+                                            const typename Stor<S>::T* __restrict__ fin,
+                                            typename Stor<S>::T* stage, const BGeom& g,
+                                            Members mid, uint32_t w, uint32_t mh, uint32_t l)
+{
+    using T = typename Stor<S>::T;
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    const uint32_t pr = threadIdx.x & 255u;
+    T2 pv[kStaged][2];  // indexed by slot: only the batch's slots are live
+    T hv[kStaged];
+    for_q(std::integer_sequence<int, qs...>{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int sl = kSlot[q];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const uint32_t b = j ? (mh ? mid.m3 : mid.m2) : (mh ? mid.m1 : mid.m0);
+            pv[sl][j] = b != kInvalidBlock
+                            ? *(const T2*)(fin + (uint64_t)b * kBStride + (uint32_t)q * kBV + 2u * pr)
+                            : T2(T(0));
+        }
+        hv[sl] = T(0);
+        uint32_t m = 0;
+        int hx = 0, hy = 0, hz = 0;
+        if (halo_of<q>(w, l, m, hx, hy, hz)) {
+            const uint32_t b = sel4(m, mid.m0, mid.m1, mid.m2, mid.m3);
+            if (b != kInvalidBlock) {
+                const Conn cb = load_conn(g.conn, b);
+                uint32_t tb, nv;
+                upstream<q>(b, cb, hx, hy, hz, tb, nv);
+                if (tb != kInvalidBlock) hv[sl] = fin[(uint64_t)tb * kBStride + (uint32_t)q * kBV + nv];
+            }
+        }
+    });
+    for_q(std::integer_sequence<int, qs...>{}, [&](auto qc) {
+        constexpr int sl = kSlot[decltype(qc)::value];
+        nx_pin(hv[sl]);
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            T a0 = pv[sl][j].x, a1 = pv[sl][j].y;
+            nx_pin(a0);
+            nx_pin(a1);
+            pv[sl][j].x = a0;
+            pv[sl][j].y = a1;
+        }
+    });
+    for_q(std::integer_sequence<int, qs...>{}, [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int sl = kSlot[q];
+        constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const uint32_t m = 2u * (uint32_t)j + mh;
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                const uint32_t v = 2u * pr + (uint32_t)e;
+                const int dx = 8 * (int)(m & 1u) + (int)(v & 7u) + cx;
+                const int dy = 8 * (int)(m >> 1) + (int)((v >> 3) & 7u) + cy;
+                const int dz = (int)(v >> 6) + cz;
+                if (dx >= 0 && dx < 16 && dy >= 0 && dy < 16 && dz >= 0 && dz < 8)
+                    stage[sl * kTileV + ((dx >> 3) + 2 * (dy >> 3)) * kBV + (dx & 7) + 8 * (dy & 7) + 64 * dz] =
+                        e ? pv[sl][j].y : pv[sl][j].x;
+            }
+        }
+        uint32_t m = 0;
+        int hx = 0, hy = 0, hz = 0;
+        if (halo_of<q>(w, l, m, hx, hy, hz))
+            stage[sl * kTileV + m * kBV + (uint32_t)(hx + 8 * hy + 64 * hz)] = hv[sl];
+    });
+}
+
+template <int S, int CM, bool FORCE>
+__global__ void __launch_bounds__(kBV)
+    k_step_tiled(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
+                 const uint32_t* __restrict__ flags, const uint32_t* __restrict__ tiles, BGeom g,
+                 Par p, int remap)
+{
+    using T = typename Stor<S>::T;
+    __shared__ T stage[kStaged * kTileV];
+    const uint32_t tile = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const Members mid{tiles[(uint64_t)tile * kTileB], tiles[(uint64_t)tile * kTileB + 1],
+                      tiles[(uint64_t)tile * kTileB + 2], tiles[(uint64_t)tile * kTileB + 3]};
+    const uint32_t t = threadIdx.x;
+    const int x = (int)(t & 7u), y = (int)((t >> 3) & 7u), z = (int)(t >> 6);
+    // wavefront index and member half, uniform per wavefront
+    const uint32_t w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const uint32_t mh = __builtin_amdgcn_readfirstlane(t >> 8);
+    const uint32_t l = t & 63u;
+
+    // in place, every member: the z-plane's active bits, the flag word and
+    // the rest and z populations of voxel t
+    uint64_t pbits[kTileB];
+    uint32_t fl[kTileB];
+    T vd[kTileB][3];
+    for_q(std::make_integer_sequence<int, kTileB>{}, [&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        pbits[k] = 0;
+        fl[k] = 0;
+        vd[k][0] = vd[k][1] = vd[k][2] = T(0);
+        const uint32_t b = mid.template at<k>();
+        if (b == kInvalidBlock) return;
+        pbits[k] = (uint64_t)g.masks[(uint64_t)b * 16 + 2 * (uint32_t)z] |
+                   ((uint64_t)g.masks[(uint64_t)b * 16 + 2 * (uint32_t)z + 1] << 32);
+        if ((pbits[k] >> l) & 1u) {
+            const Conn cb = load_conn(g.conn, b);
+            const uint64_t own = (uint64_t)b * kBStride + t;
+            fl[k] = flags[(uint64_t)b * kBV + t];
+            vd[k][0] = fin[own + 9 * kBV];
+            uint32_t tb, nv;
+            upstream<2>(b, cb, x, y, z, tb, nv);
+            vd[k][1] = fin[tb != kInvalidBlock ? (uint64_t)tb * kBStride + 2u * kBV + nv : own];
+            upstream<12>(b, cb, x, y, z, tb, nv);
+            vd[k][2] = fin[tb != kInvalidBlock ? (uint64_t)tb * kBStride + 12u * kBV + nv : own];
+        }
+    });
+#pragma unroll
+    for (int k = 0; k < kTileB; k++) {
+        nx_pin(fl[k]);
+#pragma unroll
+        for (int e = 0; e < 3; e++) nx_pin(vd[k][e]);
+    }
+    // staging, in two batches of 8 populations (register pressure)
+    stage_batch<S>(StagedA{}, fin, stage, g, mid, w, mh, l);
+    stage_batch<S>(StagedB{}, fin, stage, g, mid, w, mh, l);
+    __syncthreads();
+
+    constexpr uint32_t kLine = 128u / sizeof(T);
+    const uint64_t line_bits = kLine >= 64 ? ~0ull : ((1ull << kLine) - 1ull) << (l & ~(kLine - 1u));
+    const bool incomplete = mid.m0 == kInvalidBlock || mid.m1 == kInvalidBlock || mid.m2 == kInvalidBlock ||
+                            mid.m3 == kInvalidBlock;
+    for_q(std::make_integer_sequence<int, kTileB>{}, [&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        const uint32_t b = mid.template at<k>();
+        if (b == kInvalidBlock) return;
+        const uint64_t own = (uint64_t)b * kBStride + t;
+        const bool active = (pbits[k] >> l) & 1u;
+        if (active && cls_of(fl[k]) == NX_CELL_BULK) {
+            T val[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                if (kSlot[q] >= 0) val[q] = stage[kSlot[q] * kTileV + k * kBV + t];
+            }
+            val[9] = vd[k][0];
+            val[2] = vd[k][1];
+            val[12] = vd[k][2];
+            const Conn cb = load_conn(g.conn, b);
+            if (incomplete) {
+                // a source inside the tile's extent but in an absent member
+                // was not staged: where the link is not masked (a periodic
+                // wrap makes the upstream block another one) read it through
+                // this block's connectivity row
+                const uint32_t mk = fl[k] & NX_MASK_BITS;
+                for_q(AllQ{}, [&](auto qc) {
+                    constexpr int q = decltype(qc)::value;
+                    if constexpr (kSlot[q] >= 0) {
+                        const int sx = 8 * (k & 1) + x - CX[q], sy = 8 * (k >> 1) + y - CY[q], sz = z - CZ[q];
+                        if (sx >= 0 && sx < 16 && sy >= 0 && sy < 16 && sz >= 0 && sz < 8 &&
+                            !((mk >> q) & 1u) &&
+                            sel4((uint32_t)((sx >> 3) + 2 * (sy >> 3)), mid.m0, mid.m1, mid.m2, mid.m3) ==
+                                kInvalidBlock) {
+                            uint32_t tb, nv;
+                            upstream<q>(b, cb, x, y, z, tb, nv);
+                            val[q] = fin[tb != kInvalidBlock ? (uint64_t)tb * kBStride + (uint32_t)q * kBV + nv
+                                                             : own];
+                        }
+                    }
+                });
+            }
+            bulk_voxel<S, CM, FORCE>(fin, fout, g, p, b, cb, x, y, z, own, fl[k], val);
+        } else if (!active && (pbits[k] & line_bits) != 0) {
+#pragma unroll
+            for (int q = 0; q < Q; q++) __builtin_nontemporal_store(T(0), &fout[own + q * kBV]);
+        }
+    });
 }
 
 __global__ void __launch_bounds__(kBV) k_cavity_flags(uint32_t* flags, BGeom g)
@@ -511,6 +793,109 @@ extern "C" int nx_d3q19_step_block(const nx_block_desc* d, const void* fin, void
         }))
         return e;
     NX_CHECK_LAUNCH("k_step(block)");
+    return NX_OK;
+}
+
+struct nx_block_tiles_s {
+    uint32_t n_blocks = 0;
+    uint32_t n_tiles = 0;
+    uint32_t* d_tiles = nullptr;  // device, 4 x n_tiles block ids
+};
+
+extern "C" int nx_block_tiles_create(const nx_block_desc* d, nx_block_tiles* out)
+{
+    if (int e = check(d)) return e;
+    if (!out) return set_error(NX_ERR_INVALID, "null output handle");
+    *out = nullptr;
+    const uint32_t n = d->n_blocks;
+    std::vector<int32_t> org((size_t)3 * n);
+    if (hipMemcpy(org.data(), d->origins, org.size() * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return set_error(NX_ERR_HIP, "reading the block origins");
+    // (tile key, member, block id), tiles ordered z, y, x like the blocks
+    const uint64_t ntx = ((uint64_t)d->dim[0] + 15) / 16, nty = ((uint64_t)d->dim[1] + 15) / 16;
+    std::vector<std::pair<uint64_t, uint32_t>> key(n);
+    for (uint32_t b = 0; b < n; b++) {
+        const int32_t* o = &org[(size_t)3 * b];
+        if (o[0] < 0 || o[1] < 0 || o[2] < 0 || (o[0] & 7) || (o[1] & 7) || (o[2] & 7))
+            return set_error(NX_ERR_INVALID, "block origin not on the 8-voxel lattice");
+        const uint64_t tk = ((uint64_t)(o[2] >> 3) * nty + (uint64_t)(o[1] >> 4)) * ntx + (uint64_t)(o[0] >> 4);
+        const uint32_t member = (uint32_t)((o[0] >> 3) & 1) + 2u * (uint32_t)((o[1] >> 3) & 1);
+        key[b] = {tk * 4 + member, b};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<uint32_t> tab;
+    for (size_t i = 0; i < key.size(); i++) {
+        if (i && key[i].first == key[i - 1].first)
+            return set_error(NX_ERR_INVALID, "two blocks with one origin");
+        if (i == 0 || (key[i].first >> 2) != (key[i - 1].first >> 2))
+            tab.insert(tab.end(), kTileB, kInvalidBlock);
+        tab[tab.size() - kTileB + (key[i].first & 3)] = key[i].second;
+    }
+    auto* t = new nx_block_tiles_s;
+    t->n_blocks = n;
+    t->n_tiles = (uint32_t)(tab.size() / kTileB);
+    if (hipMalloc((void**)&t->d_tiles, tab.size() * sizeof(uint32_t)) != hipSuccess) {
+        delete t;
+        return set_error(NX_ERR_OOM, "tile table");
+    }
+    if (hipMemcpy(t->d_tiles, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(t->d_tiles);
+        delete t;
+        return set_error(NX_ERR_HIP, "uploading the tile table");
+    }
+    *out = t;
+    return NX_OK;
+}
+
+extern "C" int nx_block_tiles_info(nx_block_tiles t, uint32_t* n_tiles)
+{
+    if (!t || !n_tiles) return set_error(NX_ERR_INVALID, "null pointer");
+    *n_tiles = t->n_tiles;
+    return NX_OK;
+}
+
+extern "C" int nx_block_tiles_destroy(nx_block_tiles t)
+{
+    if (!t) return NX_OK;
+    if (t->d_tiles) (void)hipFree(t->d_tiles);
+    delete t;
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_step_block_tiled(const nx_block_desc* d, nx_block_tiles tiles, const void* fin,
+                                         void* fout, const uint32_t* flags, const nx_lbm_params* p,
+                                         nx_stream s)
+{
+    if (int e = check(d)) return e;
+    if (!tiles || !fin || !fout || !flags || !p) return set_error(NX_ERR_INVALID, "null pointer");
+    if (tiles->n_blocks != d->n_blocks)
+        return set_error(NX_ERR_INVALID, "tile table built for another block grid");
+    if (fin == fout)
+        return set_error(NX_ERR_INVALID, "fin and fout must differ (two-population scheme)");
+    if (d->storage == NX_F64)
+        return set_error(NX_ERR_UNSUPPORTED, "the LDS-staged step holds 2- and 4-byte storage");
+    const BGeom g = make_bgeom(d);
+    const Par par{p->omega, {p->force[0], p->force[1], p->force[2]}};
+    const bool force = p->force[0] != 0. || p->force[1] != 0. || p->force[2] != 0.;
+    if (force && d->compute == NX_COMPUTE_REF)
+        return set_error(NX_ERR_UNSUPPORTED,
+                         "body force requires NX_COMPUTE_F32 or NX_COMPUTE_F64");
+    const hipStream_t st = (hipStream_t)s;
+    if (int e = by_type(d, [&](auto sc, auto cc) {
+            constexpr int S = decltype(sc)::value;
+            constexpr int CM = decltype(cc)::value;
+            if constexpr (S != NX_F64) {
+                using T = typename Stor<S>::T;
+                if (force)
+                    hipLaunchKernelGGL((k_step_tiled<S, CM, true>), dim3(tiles->n_tiles), dim3(kBV), 0, st,
+                                       (const T*)fin, (T*)fout, flags, tiles->d_tiles, g, par, g_block_remap);
+                else
+                    hipLaunchKernelGGL((k_step_tiled<S, CM, false>), dim3(tiles->n_tiles), dim3(kBV), 0, st,
+                                       (const T*)fin, (T*)fout, flags, tiles->d_tiles, g, par, g_block_remap);
+            }
+        }))
+        return e;
+    NX_CHECK_LAUNCH("k_step_tiled(block)");
     return NX_OK;
 }
 

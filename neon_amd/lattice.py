@@ -445,10 +445,40 @@ class BlockLattice(_Lattice):
         self.desc.origins, self.desc.masks, self.desc.conn = (
             grid.d_origins.ptr, grid.d_masks.ptr, grid.d_conn.ptr)
         self.n_slots = grid.n_blocks * 512
+        self._tiles = None
         self._alloc()
 
     def _pop_bytes(self) -> int:
         return int(lib.nx_pop_bytes_block(C.byref(self.desc)))
+
+    def set_tiled(self, on: bool = True) -> None:
+        """Step with the block neighbourhood staged in LDS (one workgroup per
+        2 x 2 x 1 tile of blocks, nx_d3q19_step_block_tiled) instead of one
+        workgroup per block; both give the same populations bit for bit."""
+        if on and self._tiles is None:
+            h = C.c_void_p()
+            check(lib.nx_block_tiles_create(C.byref(self.desc), C.byref(h)))
+            self._tiles = h
+        elif not on and self._tiles is not None:
+            lib.nx_block_tiles_destroy(self._tiles)
+            self._tiles = None
+
+    @property
+    def tiled(self) -> bool:
+        return self._tiles is not None
+
+    @property
+    def n_tiles(self) -> int:
+        if self._tiles is None:
+            return 0
+        n = C.c_uint32()
+        check(lib.nx_block_tiles_info(self._tiles, C.byref(n)))
+        return int(n.value)
+
+    def __del__(self):
+        if getattr(self, "_tiles", None) is not None:
+            lib.nx_block_tiles_destroy(self._tiles)
+            self._tiles = None
 
     def setup_cavity(self, ulb: float, ulid=(1.0, 0.0, 0.0), stream=None) -> None:
         check(lib.nx_cavity_flags_block(C.byref(self.desc), self.flags.ptr, stream))
@@ -473,6 +503,10 @@ class BlockLattice(_Lattice):
         self.parity = 0
 
     def _step(self, p, stream):
+        if self._tiles is not None:
+            check(lib.nx_d3q19_step_block_tiled(C.byref(self.desc), self._tiles, self.fin.ptr,
+                                                self.fout.ptr, self.flags.ptr, C.byref(p), stream))
+            return
         check(lib.nx_d3q19_step_block(C.byref(self.desc), self.fin.ptr, self.fout.ptr,
                                       self.flags.ptr, C.byref(p), stream))
 
