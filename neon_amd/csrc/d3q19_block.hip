@@ -251,12 +251,13 @@ __global__ void __launch_bounds__(kBV)
 // -- the face and edge voxels of the neighbour blocks that stream into the
 // tile, one lane per face voxel of the tile -- and writes each value into LDS
 // at the voxel it streams to (the upstream rule of bPartition_imp.h:218-309
-// applied from the source side).  After one barrier every voxel finds its 16
+// applied from the source side).  After one barrier every voxel finds its
 // streamed values at its own LDS index.  A face shared by two members is
 // fetched once instead of once per block: per 64 voxels about 32 line
 // requests instead of 43 at 2-byte storage.  The rest population and the two
 // z populations are read in place (their sources are whole lines already),
-// with the flag words, before the staging loads.  In a tile with an absent
+// with the flag words, before the staging loads; at 4-byte storage so are
+// the eight populations with a z component (LDS: 64 KiB per workgroup).  In a tile with an absent
 // member, a voxel whose unmasked link starts in that member's extent (only a
 // periodic wrap makes this happen: the upstream block is then another one)
 // reads that value through its block's connectivity row.  The arithmetic
@@ -264,14 +265,34 @@ __global__ void __launch_bounds__(kBV)
 // bit.
 constexpr int kTileB = 4;                 // blocks per tile (2 x 2 x 1)
 constexpr int kTileV = kTileB * kBV;      // voxels per tile
-constexpr int kStaged = 16;               // populations staged in LDS
-constexpr int kSlot[Q] = {0, 1, -1, 2, 3, 4, 5, 6, 7, -1, 8, 9, -1, 10, 11, 12, 13, 14, 15};
+constexpr int kStaged = 16;               // most populations staged in LDS
+
+// LDS slot of population q (-1: read in place).  2-byte storage stages every
+// population with an x or y component (64 KiB); 4-byte storage the eight in
+// the x-y plane only, so that two workgroups still fit one CU's LDS.
+template <int S>
+constexpr int tiled_slot(int q)
+{
+    if constexpr (sizeof(typename Stor<S>::T) == 2) {
+        constexpr int s2[Q] = {0, 1, -1, 2, 3, 4, 5, 6, 7, -1, 8, 9, -1, 10, 11, 12, 13, 14, 15};
+        return s2[q];
+    } else {
+        constexpr int s4[Q] = {0, 1, -1, 2, 3, -1, -1, -1, -1, -1, 4, 5, -1, 6, 7, -1, -1, -1, -1};
+        return s4[q];
+    }
+}
 
 template <int S>
-constexpr size_t tiled_lds_bytes()
+constexpr int tiled_slots()
 {
-    return (size_t)kStaged * kTileV * sizeof(typename Stor<S>::T);
+    return sizeof(typename Stor<S>::T) == 2 ? 16 : 8;
 }
+
+template <int S>
+using TiledStaged = typename std::conditional<
+    sizeof(typename Stor<S>::T) == 2,
+    std::integer_sequence<int, 0, 1, 3, 4, 5, 6, 7, 8, 10, 11, 13, 14, 15, 16, 17, 18>,
+    std::integer_sequence<int, 0, 1, 3, 4, 10, 11, 13, 14>>::type;
 
 // the halo voxel of population q loaded by lane l of wavefront w: wavefronts
 // 0-1 the tile's x face (member row c = w), 2-3 its y face (member column
@@ -323,17 +344,15 @@ __device__ __forceinline__ uint32_t sel4(uint32_t m, uint32_t a, uint32_t b, uin
     return m == 0 ? a : m == 1 ? b : m == 2 ? c : d;
 }
 
-using StagedA = std::integer_sequence<int, 0, 1, 3, 4, 5, 6, 7, 8>;
-using StagedB = std::integer_sequence<int, 10, 11, 13, 14, 15, 16, 17, 18>;
-
 // Loads of the staged populations qs -- member planes (lane pair index
 // t & 255 of members mh and 2 + mh) and one halo voxel per lane -- then each
 // value written to the LDS index of the voxel it streams to.
 template <int S, int... qs>
 __device__ __forceinline__ void stage_batch(std::integer_sequence<int, qs...>,
                                             const typename Stor<S>::T* __restrict__ fin,
-                                            typename Stor<S>::T* stage, const BGeom& g,
-                                            Members mid, uint32_t w, uint32_t mh, uint32_t l)
+                                            typename Stor<S>::T* stage, Members mid, uint32_t w,
+                                            uint32_t mh, uint32_t l, uint32_t hb_neg, uint32_t hb_pos,
+                                            const Conn& hc_neg, const Conn& hc_pos)
 {
     using T = typename Stor<S>::T;
     typedef T T2 __attribute__((ext_vector_type(2)));
@@ -342,7 +361,7 @@ __device__ __forceinline__ void stage_batch(std::integer_sequence<int, qs...>,
     T hv[kStaged];
     for_q(std::integer_sequence<int, qs...>{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int sl = kSlot[q];
+        constexpr int sl = tiled_slot<S>(q);
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             const uint32_t b = j ? (mh ? mid.m3 : mid.m2) : (mh ? mid.m1 : mid.m0);
@@ -354,17 +373,19 @@ __device__ __forceinline__ void stage_batch(std::integer_sequence<int, qs...>,
         uint32_t m = 0;
         int hx = 0, hy = 0, hz = 0;
         if (halo_of<q>(w, l, m, hx, hy, hz)) {
-            const uint32_t b = sel4(m, mid.m0, mid.m1, mid.m2, mid.m3);
+            // the member m on the tile face: on the negative side of the
+            // wavefront's axis for a positive velocity component
+            const bool neg = w < 2u ? CX[q] > 0 : (w < 4u ? CY[q] > 0 : CZ[q] > 0);
+            const uint32_t b = neg ? hb_neg : hb_pos;
             if (b != kInvalidBlock) {
-                const Conn cb = load_conn(g.conn, b);
                 uint32_t tb, nv;
-                upstream<q>(b, cb, hx, hy, hz, tb, nv);
+                upstream<q>(b, neg ? hc_neg : hc_pos, hx, hy, hz, tb, nv);
                 if (tb != kInvalidBlock) hv[sl] = fin[(uint64_t)tb * kBStride + (uint32_t)q * kBV + nv];
             }
         }
     });
     for_q(std::integer_sequence<int, qs...>{}, [&](auto qc) {
-        constexpr int sl = kSlot[decltype(qc)::value];
+        constexpr int sl = tiled_slot<S>(decltype(qc)::value);
         nx_pin(hv[sl]);
 #pragma unroll
         for (int j = 0; j < 2; j++) {
@@ -377,7 +398,7 @@ __device__ __forceinline__ void stage_batch(std::integer_sequence<int, qs...>,
     });
     for_q(std::integer_sequence<int, qs...>{}, [&](auto qc) {
         constexpr int q = decltype(qc)::value;
-        constexpr int sl = kSlot[q];
+        constexpr int sl = tiled_slot<S>(q);
         constexpr int cx = CX[q], cy = CY[q], cz = CZ[q];
 #pragma unroll
         for (int j = 0; j < 2; j++) {
@@ -407,7 +428,7 @@ __global__ void __launch_bounds__(kBV)
                  Par p, int remap)
 {
     using T = typename Stor<S>::T;
-    __shared__ T stage[kStaged * kTileV];
+    __shared__ T stage[tiled_slots<S>() * kTileV];
     const uint32_t tile = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
     const Members mid{tiles[(uint64_t)tile * kTileB], tiles[(uint64_t)tile * kTileB + 1],
                       tiles[(uint64_t)tile * kTileB + 2], tiles[(uint64_t)tile * kTileB + 3]};
@@ -417,42 +438,56 @@ __global__ void __launch_bounds__(kBV)
     const uint32_t w = __builtin_amdgcn_readfirstlane(t >> 6);
     const uint32_t mh = __builtin_amdgcn_readfirstlane(t >> 8);
     const uint32_t l = t & 63u;
+    // the members whose tile-face voxels this wavefront's halo lanes serve
+    // (x face: row w; y face: column w - 2; z face: member w - 4), for a
+    // source beyond the negative and the positive side, with their
+    // connectivity rows read up front (block 0 stands in for an absent one)
+    const uint32_t hm_neg = w < 2u ? 2u * w : (w < 4u ? w - 2u : w - 4u);
+    const uint32_t hm_pos = w < 2u ? 2u * w + 1u : (w < 4u ? w : w - 4u);
+    const uint32_t hb_neg = sel4(hm_neg, mid.m0, mid.m1, mid.m2, mid.m3);
+    const uint32_t hb_pos = sel4(hm_pos, mid.m0, mid.m1, mid.m2, mid.m3);
+    const Conn hc_neg = load_conn(g.conn, hb_neg == kInvalidBlock ? 0u : hb_neg);
+    const Conn hc_pos = load_conn(g.conn, hb_pos == kInvalidBlock ? 0u : hb_pos);
 
     // in place, every member: the z-plane's active bits, the flag word and
     // the rest and z populations of voxel t
     uint64_t pbits[kTileB];
     uint32_t fl[kTileB];
-    T vd[kTileB][3];
+    T vd[kTileB][Q];  // only the unstaged populations are live
     for_q(std::make_integer_sequence<int, kTileB>{}, [&](auto kc) {
         constexpr int k = decltype(kc)::value;
         pbits[k] = 0;
         fl[k] = 0;
-        vd[k][0] = vd[k][1] = vd[k][2] = T(0);
+#pragma unroll
+        for (int q = 0; q < Q; q++) vd[k][q] = T(0);
         const uint32_t b = mid.template at<k>();
+        const Conn cb = load_conn(g.conn, b == kInvalidBlock ? 0u : b);  // ahead of the branches
         if (b == kInvalidBlock) return;
         pbits[k] = (uint64_t)g.masks[(uint64_t)b * 16 + 2 * (uint32_t)z] |
                    ((uint64_t)g.masks[(uint64_t)b * 16 + 2 * (uint32_t)z + 1] << 32);
         if ((pbits[k] >> l) & 1u) {
-            const Conn cb = load_conn(g.conn, b);
             const uint64_t own = (uint64_t)b * kBStride + t;
             fl[k] = flags[(uint64_t)b * kBV + t];
-            vd[k][0] = fin[own + 9 * kBV];
-            uint32_t tb, nv;
-            upstream<2>(b, cb, x, y, z, tb, nv);
-            vd[k][1] = fin[tb != kInvalidBlock ? (uint64_t)tb * kBStride + 2u * kBV + nv : own];
-            upstream<12>(b, cb, x, y, z, tb, nv);
-            vd[k][2] = fin[tb != kInvalidBlock ? (uint64_t)tb * kBStride + 12u * kBV + nv : own];
+            for_q(AllQ{}, [&](auto qc) {
+                constexpr int q = decltype(qc)::value;
+                if constexpr (q == 9) {
+                    vd[k][9] = fin[own + 9 * kBV];
+                } else if constexpr (tiled_slot<S>(q) < 0) {
+                    uint32_t tb, nv;
+                    upstream<q>(b, cb, x, y, z, tb, nv);
+                    vd[k][q] = fin[tb != kInvalidBlock ? (uint64_t)tb * kBStride + (uint32_t)q * kBV + nv : own];
+                }
+            });
         }
     });
 #pragma unroll
     for (int k = 0; k < kTileB; k++) {
         nx_pin(fl[k]);
 #pragma unroll
-        for (int e = 0; e < 3; e++) nx_pin(vd[k][e]);
+        for (int q = 0; q < Q; q++)
+            if (tiled_slot<S>(q) < 0) nx_pin(vd[k][q]);
     }
-    // staging, in two batches of 8 populations (register pressure)
-    stage_batch<S>(StagedA{}, fin, stage, g, mid, w, mh, l);
-    stage_batch<S>(StagedB{}, fin, stage, g, mid, w, mh, l);
+    stage_batch<S>(TiledStaged<S>{}, fin, stage, mid, w, mh, l, hb_neg, hb_pos, hc_neg, hc_pos);
     __syncthreads();
 
     constexpr uint32_t kLine = 128u / sizeof(T);
@@ -468,12 +503,8 @@ __global__ void __launch_bounds__(kBV)
         if (active && cls_of(fl[k]) == NX_CELL_BULK) {
             T val[Q];
 #pragma unroll
-            for (int q = 0; q < Q; q++) {
-                if (kSlot[q] >= 0) val[q] = stage[kSlot[q] * kTileV + k * kBV + t];
-            }
-            val[9] = vd[k][0];
-            val[2] = vd[k][1];
-            val[12] = vd[k][2];
+            for (int q = 0; q < Q; q++)
+                val[q] = tiled_slot<S>(q) >= 0 ? stage[tiled_slot<S>(q) * kTileV + k * kBV + t] : vd[k][q];
             const Conn cb = load_conn(g.conn, b);
             if (incomplete) {
                 // a source inside the tile's extent but in an absent member
@@ -483,7 +514,7 @@ __global__ void __launch_bounds__(kBV)
                 const uint32_t mk = fl[k] & NX_MASK_BITS;
                 for_q(AllQ{}, [&](auto qc) {
                     constexpr int q = decltype(qc)::value;
-                    if constexpr (kSlot[q] >= 0) {
+                    if constexpr (tiled_slot<S>(q) >= 0) {
                         const int sx = 8 * (k & 1) + x - CX[q], sy = 8 * (k >> 1) + y - CY[q], sz = z - CZ[q];
                         if (sx >= 0 && sx < 16 && sy >= 0 && sy < 16 && sz >= 0 && sz < 8 &&
                             !((mk >> q) & 1u) &&
