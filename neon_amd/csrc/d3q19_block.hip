@@ -830,7 +830,9 @@ extern "C" int nx_d3q19_step_block(const nx_block_desc* d, const void* fin, void
 struct nx_block_tiles_s {
     uint32_t n_blocks = 0;
     uint32_t n_tiles = 0;
-    uint32_t* d_tiles = nullptr;  // device, 4 x n_tiles block ids
+    uint32_t* d_tiles = nullptr;      // device, 4 x n_tiles block ids
+    const int32_t* origins = nullptr;  // the block tables the table was built from
+    const uint32_t* conn = nullptr;
 };
 
 extern "C" int nx_block_tiles_create(const nx_block_desc* d, nx_block_tiles* out)
@@ -864,6 +866,8 @@ extern "C" int nx_block_tiles_create(const nx_block_desc* d, nx_block_tiles* out
     }
     auto* t = new nx_block_tiles_s;
     t->n_blocks = n;
+    t->origins = d->origins;
+    t->conn = d->conn;
     t->n_tiles = (uint32_t)(tab.size() / kTileB);
     if (hipMalloc((void**)&t->d_tiles, tab.size() * sizeof(uint32_t)) != hipSuccess) {
         delete t;
@@ -899,7 +903,7 @@ extern "C" int nx_d3q19_step_block_tiled(const nx_block_desc* d, nx_block_tiles 
 {
     if (int e = check(d)) return e;
     if (!tiles || !fin || !fout || !flags || !p) return set_error(NX_ERR_INVALID, "null pointer");
-    if (tiles->n_blocks != d->n_blocks)
+    if (tiles->n_blocks != d->n_blocks || tiles->origins != d->origins || tiles->conn != d->conn)
         return set_error(NX_ERR_INVALID, "tile table built for another block grid");
     if (fin == fout)
         return set_error(NX_ERR_INVALID, "fin and fout must differ (two-population scheme)");

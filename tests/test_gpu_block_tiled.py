@@ -104,7 +104,7 @@ def test_tiled_rejects_what_it_cannot_run(nx):
     g64.set_tiled(True)
     with pytest.raises(abi.NeonError, match="2- and 4-byte"):
         g64.run(1, 1.5)
-    other = nx.BlockGrid((24, 16, 16), np.ones(24 * 16 * 16, np.uint8))
+    other = nx.BlockGrid(dim, act)      # the same block count, other tables
     g32 = nx.BlockLattice(other, "f32", "f32")
     g32.setup_cavity(0.05)
     p = nx.params(1.5)
