@@ -55,8 +55,10 @@ POROUS = {
     "porous384_f16dev": (384, "f16dev", "f32", 200, 20),
     # the same step with the block neighbourhood staged in LDS (one workgroup
     # per 2 x 2 x 1 tile of blocks, nx_d3q19_step_block_tiled)
-    "porous384_f32_tiled": (384, "f32", "f32", 200, 20, True),
-    "porous384_f16dev_tiled": (384, "f16dev", "f32", 200, 20, True),
+    "porous384_f32_tiled": (384, "f32", "f32", 200, 20, "tiled"),
+    "porous384_f16dev_tiled": (384, "f16dev", "f32", 200, 20, "tiled"),
+    # 2-byte storage, two x-adjacent voxels per lane (nx_d3q19_set_block_pairs)
+    "porous384_f16dev_pairs": (384, "f16dev", "f32", 200, 20, "pairs"),
 }
 
 
@@ -271,7 +273,10 @@ def shifted_copy_calibration(nx, N=256, reps=20):
 
 def porous(nx, a):
     N, storage, compute, dsteps, dwarm = POROUS[a.workload][:5]
-    tiled = len(POROUS[a.workload]) > 5 and POROUS[a.workload][5]
+    variant = POROUS[a.workload][5] if len(POROUS[a.workload]) > 5 else None
+    tiled = variant == "tiled"
+    from neon_amd import abi as _abi
+    _abi.check(_abi.lib.nx_d3q19_set_block_pairs(1 if variant == "pairs" else 0))
     steps = dsteps if a.steps is None else a.steps
     warmup = dwarm if a.warmup is None else a.warmup
     nu, g = 0.02, (1e-6, 0.0, 0.0)
@@ -336,6 +341,7 @@ def porous(nx, a):
                                f"after {warmup} warm-up",
                    "grid": "bGrid", "N": N, "blocks": grid.n_blocks, "candidate_blocks": nb3,
                    "step": ("LDS-staged, one workgroup per 2x2x1 tile of blocks" if tiled
+                            else "one workgroup per block, two voxels per lane" if variant == "pairs"
                             else "one workgroup per block"),
                    "active_block_fraction": round(grid.n_blocks / nb3, 4),
                    "fluid_voxels": fluid, "fluid_fraction": round(fluid / N ** 3, 4),
@@ -343,8 +349,9 @@ def porous(nx, a):
         "wall_s": round(wall, 4),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(f"d3q19_block{'_tiled' if tiled else ''}_{storage}_{N}"),
-                     "kernel": f"{'k_step_tiled' if tiled else 'k_step'}<block,{storage},{compute}> "
+                     "traffic": traffic_from_profiles(f"d3q19_block{'_' + variant if variant else ''}_{storage}_{N}"),
+                     "kernel": f"{'k_step_tiled' if tiled else 'k_step_pair' if variant == 'pairs' else 'k_step'}"
+                               f"<block,{storage},{compute}> "
                                f"({b_alg} B/fluid voxel)"},
         "velocity_error_vs_f64": err,
         "cpu_baseline": None,

@@ -320,6 +320,11 @@ int nx_d3q19_step_block(const nx_block_desc* d, const void* fin, void* fout,
  * contiguous range of blocks, the default; 0: hardware round-robin).  No
  * counterpart in the reference. */
 int nx_d3q19_set_block_remap(int on);
+/* Lanes of nx_d3q19_step_block at 2-byte storage (NX_F16DEV): 1 -- two
+ * x-adjacent voxels per lane, one 4-byte load and store per population;
+ * 0 -- one voxel per lane.  Same results bit for bit.  No counterpart in
+ * the reference. */
+int nx_d3q19_set_block_pairs(int on);
 /* The same step with the block neighbourhood staged in LDS (north-star
  * subsystem b; the reference reads neighbour blocks through
  * bPartition::getNghData, bPartition_imp.h:218-309): one workgroup per tile

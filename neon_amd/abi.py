@@ -136,6 +136,7 @@ SIGNATURES = {
     "nx_block_tiles_destroy": (_I, [_P]),
     "nx_d3q19_step_block_tiled": (_I, [_BP, _P, _P, _P, _P, _PP, _P]),
     "nx_d3q19_set_block_remap": (_I, [_I]),
+    "nx_d3q19_set_block_pairs": (_I, [_I]),
     "nx_geometry_spheres": (_I, [C.POINTER(_I32), C.c_uint32, _D, _D, C.c_uint64, _I32, _P]),
     "nx_geometry_cavity": (_I, [C.POINTER(_I32), C.c_uint32, _I32, _I32, C.c_uint64, _P]),
     "nx_batch_create": (_I, [C.c_uint32, _P, _P, _P, _P, _P, _I32, C.POINTER(_P)]),

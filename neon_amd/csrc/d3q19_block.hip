@@ -123,11 +123,10 @@ __device__ __forceinline__ bool stage(const BGeom& g, uint32_t b, uint32_t* smas
 // link can have an upstream voxel: no NX_FLAG_OPEN, or NX_FLAG_MIXED), then
 // the collision and the 19 stores.
 template <int S, int CM, bool FORCE>
-__device__ __forceinline__ void bulk_voxel(const typename Stor<S>::T* __restrict__ fin,
-                                           typename Stor<S>::T* __restrict__ fout, const BGeom& g,
-                                           const Par& p, uint32_t b, const Conn& cb, int x, int y,
-                                           int z, uint64_t own, uint32_t fl,
-                                           const typename Stor<S>::T val[Q])
+__device__ __forceinline__ void bulk_voxel_out(const typename Stor<S>::T* __restrict__ fin, const BGeom& g,
+                                               const Par& p, uint32_t b, const Conn& cb, int x, int y,
+                                               int z, uint64_t own, uint32_t fl,
+                                               const typename Stor<S>::T val[Q], typename Stor<S>::T res[Q])
 {
     using T = typename Stor<S>::T;
     using P = typename Stor<S>::P;
@@ -173,8 +172,20 @@ __device__ __forceinline__ void bulk_voxel(const typename Stor<S>::T* __restrict
     R out[Q];
     collide<S, CM, FORCE>(f, p.omega, p.g, out);
 #pragma unroll
-    for (int q = 0; q < Q; q++)
-        __builtin_nontemporal_store(Stor<S>::template enc<R>(out[q], q), &fout[own + q * kBV]);
+    for (int q = 0; q < Q; q++) res[q] = Stor<S>::template enc<R>(out[q], q);
+}
+
+template <int S, int CM, bool FORCE>
+__device__ __forceinline__ void bulk_voxel(const typename Stor<S>::T* __restrict__ fin,
+                                           typename Stor<S>::T* __restrict__ fout, const BGeom& g,
+                                           const Par& p, uint32_t b, const Conn& cb, int x, int y,
+                                           int z, uint64_t own, uint32_t fl,
+                                           const typename Stor<S>::T val[Q])
+{
+    typename Stor<S>::T res[Q];
+    bulk_voxel_out<S, CM, FORCE>(fin, g, p, b, cb, x, y, z, own, fl, val, res);
+#pragma unroll
+    for (int q = 0; q < Q; q++) __builtin_nontemporal_store(res[q], &fout[own + q * kBV]);
 }
 
 // Step of one block per workgroup.  The 19 population loads of an active
@@ -238,6 +249,117 @@ __global__ void __launch_bounds__(kBV)
         // inactive value so the line is written whole
 #pragma unroll
         for (int q = 0; q < Q; q++) __builtin_nontemporal_store(T(0), &fout[own + q * kBV]);
+    }
+}
+
+// ---- 2-byte storage: two x-adjacent voxels per lane --------------------------
+//
+// One workgroup of 256 threads per block, lane t owning voxels 2t and 2t + 1
+// (x even and x + 1 of one row), so a wavefront covers two z-planes and each
+// population is one 4-byte load per lane: at the pair's upstream voxels when
+// both lie in one block (2-byte aligned when the velocity has an x
+// component; unaligned dword loads are part of the amdhsa target), else --
+// the pair at the block's x face -- the aligned pair holding the one
+// upstream voxel inside, plus one 2-byte load from the neighbour block.  The
+// arithmetic is bulk_voxel_out per voxel, as in k_step, so the two steps
+// agree bit for bit.  Stores: one 4-byte store per population where both
+// voxels are written, else 2-byte stores for the written one (an active
+// non-bulk voxel is not written).
+template <int S, int CM, bool FORCE>
+__global__ void __launch_bounds__(kBV / 2)
+    k_step_pair(const typename Stor<S>::T* __restrict__ fin, typename Stor<S>::T* __restrict__ fout,
+                const uint32_t* __restrict__ flags, BGeom g, Par p, int remap)
+{
+    using T = typename Stor<S>::T;
+    static_assert(sizeof(T) == 2, "two voxels per 4-byte word");
+    typedef uint32_t u32a2 __attribute__((aligned(2)));
+    const uint32_t b = remap ? xcd_remap(blockIdx.x, gridDim.x) : blockIdx.x;
+    const uint32_t v0 = 2u * threadIdx.x;
+    const int x0 = (int)(v0 & 7u), y = (int)((v0 >> 3) & 7u), z = (int)(v0 >> 6);
+    const uint64_t own0 = (uint64_t)b * kBStride + v0;
+    const uint64_t plane_bits =
+        (uint64_t)g.masks[(uint64_t)b * 16 + 2 * (uint32_t)z] |
+        ((uint64_t)g.masks[(uint64_t)b * 16 + 2 * (uint32_t)z + 1] << 32);
+    const Conn cb = load_conn(g.conn, b);
+    const uint32_t lb = v0 & 63u;
+    const bool act0 = (plane_bits >> lb) & 1u, act1 = (plane_bits >> (lb + 1u)) & 1u;
+    const bool line_any = plane_bits != 0;  // 2-byte storage: the z-plane is one 128-byte line
+    uint32_t fl0 = 0, fl1 = 0;
+    T a0[Q], a1[Q];
+    if (act0 || act1) {
+        const uint64_t fw = *(const uint64_t*)(flags + (uint64_t)b * kBV + v0);
+        fl0 = (uint32_t)fw;
+        fl1 = (uint32_t)(fw >> 32);
+        for_q(AllQ{}, [&](auto qc) {
+            constexpr int q = decltype(qc)::value;
+            uint32_t t0, n0, t1, n1;
+            upstream<q>(b, cb, x0, y, z, t0, n0);
+            upstream<q>(b, cb, x0 + 1, y, z, t1, n1);
+            // the dword's first voxel and which half each voxel takes
+            uint64_t wa;
+            bool split = false;
+            if constexpr (CX[q] == 0) {
+                wa = t0 != kInvalidBlock ? (uint64_t)t0 * kBStride + (uint32_t)q * kBV + n0 : own0 + 9u * kBV;
+            } else if constexpr (CX[q] > 0) {
+                // sources x0 - 1, x0: at x0 = 0 the first lies in the -x block
+                split = x0 == 0;
+                const uint32_t tw = split ? t1 : t0, nw = split ? n1 : n0;
+                wa = tw != kInvalidBlock ? (uint64_t)tw * kBStride + (uint32_t)q * kBV + nw : own0 + 9u * kBV;
+            } else {
+                // sources x0 + 1, x0 + 2: at x0 = 6 the second lies in the +x block
+                split = x0 == 6;
+                const uint32_t nw = split ? n0 - 1u : n0;
+                wa = t0 != kInvalidBlock ? (uint64_t)t0 * kBStride + (uint32_t)q * kBV + nw : own0 + 9u * kBV;
+            }
+            const uint32_t wv = *(const u32a2*)(fin + wa);
+            const T lo = __builtin_bit_cast(T, (uint16_t)(wv & 0xFFFFu));
+            const T hi = __builtin_bit_cast(T, (uint16_t)(wv >> 16));
+            if constexpr (CX[q] == 0) {
+                a0[q] = lo;
+                a1[q] = hi;
+            } else if constexpr (CX[q] > 0) {
+                T e = lo;
+                if (split) e = t0 != kInvalidBlock ? fin[(uint64_t)t0 * kBStride + (uint32_t)q * kBV + n0] : T(0);
+                a0[q] = split ? e : lo;
+                a1[q] = split ? lo : hi;
+            } else {
+                T e = hi;
+                if (split) e = t1 != kInvalidBlock ? fin[(uint64_t)t1 * kBStride + (uint32_t)q * kBV + n1] : T(0);
+                a0[q] = split ? hi : lo;
+                a1[q] = split ? e : hi;
+            }
+        });
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            nx_pin(a0[q]);
+            nx_pin(a1[q]);
+        }
+        nx_pin(fl0);
+        nx_pin(fl1);
+    }
+    const bool bulk0 = act0 && cls_of(fl0) == NX_CELL_BULK, bulk1 = act1 && cls_of(fl1) == NX_CELL_BULK;
+    // which voxels are written: bulk ones (their new values) and inactive
+    // ones in a line that holds active voxels (the inactive value 0)
+    const bool w0 = bulk0 || (!act0 && line_any), w1 = bulk1 || (!act1 && line_any);
+    if (!w0 && !w1) return;
+    T r0[Q], r1[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) r0[q] = r1[q] = T(0);
+    if (bulk0) bulk_voxel_out<S, CM, FORCE>(fin, g, p, b, cb, x0, y, z, own0, fl0, a0, r0);
+    if (bulk1) bulk_voxel_out<S, CM, FORCE>(fin, g, p, b, cb, x0 + 1, y, z, own0 + 1u, fl1, a1, r1);
+    if (w0 && w1) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            const uint32_t wv = (uint32_t)__builtin_bit_cast(uint16_t, r0[q]) |
+                                ((uint32_t)__builtin_bit_cast(uint16_t, r1[q]) << 16);
+            __builtin_nontemporal_store(wv, (uint32_t*)(fout + own0 + (uint64_t)q * kBV));
+        }
+    } else if (w0) {
+#pragma unroll
+        for (int q = 0; q < Q; q++) __builtin_nontemporal_store(r0[q], &fout[own0 + (uint64_t)q * kBV]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; q++) __builtin_nontemporal_store(r1[q], &fout[own0 + 1u + (uint64_t)q * kBV]);
     }
 }
 
@@ -724,6 +846,8 @@ using IC = std::integral_constant<int, V>;
 
 // 1: contiguous range of blocks per XCD (nx_d3q19_set_block_remap)
 int g_block_remap = 1;
+// 1: 2-byte storage steps two voxels per lane (nx_d3q19_set_block_pairs)
+int g_block_pairs = 0;
 
 template <typename F>
 int by_type(const nx_block_desc* d, F&& f)
@@ -815,6 +939,17 @@ extern "C" int nx_d3q19_step_block(const nx_block_desc* d, const void* fin, void
             constexpr int S = decltype(sc)::value;
             constexpr int CM = decltype(cc)::value;
             using T = typename Stor<S>::T;
+            if constexpr (sizeof(T) == 2) {
+                if (g_block_pairs) {
+                    if (force)
+                        hipLaunchKernelGGL((k_step_pair<S, CM, true>), dim3(g.n_blocks), dim3(kBV / 2), 0, st,
+                                           (const T*)fin, (T*)fout, flags, g, par, g_block_remap);
+                    else
+                        hipLaunchKernelGGL((k_step_pair<S, CM, false>), dim3(g.n_blocks), dim3(kBV / 2), 0, st,
+                                           (const T*)fin, (T*)fout, flags, g, par, g_block_remap);
+                    return;
+                }
+            }
             if (force)
                 hipLaunchKernelGGL((k_step<S, CM, true>), dim3(g.n_blocks), dim3(kBV), 0, st,
                                    (const T*)fin, (T*)fout, flags, g, par, g_block_remap);
@@ -997,5 +1132,11 @@ extern "C" int nx_d3q19_residual_block(const nx_block_desc* d, const void* fin,
 extern "C" int nx_d3q19_set_block_remap(int on)
 {
     g_block_remap = on ? 1 : 0;
+    return NX_OK;
+}
+
+extern "C" int nx_d3q19_set_block_pairs(int on)
+{
+    g_block_pairs = on ? 1 : 0;
     return NX_OK;
 }
