@@ -92,7 +92,8 @@ CONVERGE = {
 # level's length.  A step is one coarse step: the level recursion, level l
 # updated 2^(depth-1-l) times.
 MRES = {
-    # name: (N, levelSDF, storage, default steps, default warmup, Re[, lattice])
+    # name: (N, levelSDF, storage, default steps, default warmup, Re[, lattice]);
+    # the sphere entries: ("sphere", --scale, storage, steps, warmup, Re[, lattice])
     "mres256_f32": (256, (0.0, -0.25, -1.0), "f32", 100, 10, 1000.0),
     "mres256_f64": (256, (0.0, -0.25, -1.0), "f64", 50, 5, 1000.0),
     "mres_scale2_f32": (240, (0.0, -24.0 / 120.0, -80.0 / 120.0, -1.0), "f32", 50, 5, 100.0),
@@ -104,6 +105,12 @@ MRES = {
     "mres_scale2_kbc_f64": (240, (0.0, -24.0 / 120.0, -80.0 / 120.0, -1.0), "f64", 25, 3, 100.0, "kbc27"),
     "mres_scale9_kbc_f32": (480, (0.0, -48.0 / 240.0, -96.0 / 240.0, -160.0 / 240.0, -1.0), "f32", 10, 1, 100.0,
                             "kbc27"),
+    # the app's flow over a sphere (flowOverShape.h:115-190) at --scale 2:
+    # 272 x 192 x 272, three levels, inlet u = 0.04 at x = 0, Re 100 on the
+    # sphere's radius / 4, the app's default KBC build in float
+    "mres_sphere2_kbc_f32": ("sphere", 2, "f32", 50, 5, 100.0, "kbc27"),
+    "mres_sphere4_kbc_f32": ("sphere", 4, "f32", 20, 2, 100.0, "kbc27"),
+    "mres_sphere2_f32": ("sphere", 2, "f32", 50, 5, 100.0),
 }
 
 
@@ -560,13 +567,24 @@ def mres_cavity(nx, a):
     warmup = dwarm if a.warmup is None else a.warmup
     from neon_amd import mgrid as mg
     nx.init(0)
-    dim = (N, N, N)
-    depth = len(sdf) - 1
+    sphere = N == "sphere"
     t0 = time.perf_counter()
-    lams = mg.cavity_lambdas(dim, sdf)
-    ulb, om0 = mg.cavity_parameters(dim, re, storage, depth=depth)
-    g = mg.MGrid(dim, list(lams), storage, lattice=lattice)
-    g.setup_cavity(ulb)
+    if sphere:
+        scale = sdf
+        dim, lams, cls, _ = mg.sphere_problem(scale)
+        depth = 3
+        ulb, om0 = mg.sphere_parameters(scale, re, storage)
+        g = mg.MGrid(dim, list(lams), storage, lattice=lattice)
+        g.setup_cells(cls, (ulb, 0.0, 0.0))
+        problem = f"flow over a sphere (--scale {scale}) {dim[0]}x{dim[1]}x{dim[2]}, 3 levels, inlet u {ulb:.4g}"
+    else:
+        dim = (N, N, N)
+        depth = len(sdf) - 1
+        lams = mg.cavity_lambdas(dim, sdf)
+        ulb, om0 = mg.cavity_parameters(dim, re, storage, depth=depth)
+        g = mg.MGrid(dim, list(lams), storage, lattice=lattice)
+        g.setup_cavity(ulb)
+        problem = f"{depth}-level refined lid-driven cavity {N}^3 (levelSDF {list(sdf)}), ulb {ulb:.4g}"
     t_setup = time.perf_counter() - t0
     info = g.info()
     stream = nx.Stream()
@@ -587,7 +605,7 @@ def mres_cavity(nx, a):
     # 2^(depth-1-l) times per coarse step
     updates = sum(n << (depth - 1 - l) for l, n in enumerate(leaf))
     mlups = updates * steps / (ms * 1e-3) / 1e6
-    emlups = (1 << (depth - 1)) * N ** 3 * steps / (ms * 1e-3) / 1e6
+    emlups = (1 << (depth - 1)) * dim[0] * dim[1] * dim[2] * steps / (ms * 1e-3) / 1e6
     s_b = STORAGE_BYTES[storage]
     # bytes the kernels must move per coarse step: every pass reads and
     # writes the 19 populations of a leaf; a sub-step of level l is one pass
@@ -600,10 +618,10 @@ def mres_cavity(nx, a):
         "metric": METRIC, "value": round(mlups, 1), "unit": "MLUPS", "n_gpus": 1,
         "steps": steps, "warmup": warmup, "ms_per_step": round(ms_step, 5),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": storage,
-        "data": f"synthetic ({depth}-level refined lid-driven cavity started at rest)",
-        "config": {"workload": f"{depth}-level refined lid-driven cavity {N}^3 (levelSDF {list(sdf)}), "
-                               f"{'D3Q27 KBC' if nq == 27 else 'D3Q19 BGK'}, "
-                               f"{storage}, ulb {ulb:.4g}, Re {re:g}, {steps} timed coarse steps after {warmup}",
+        "data": f"synthetic ({'flow over a sphere' if sphere else f'{depth}-level refined lid-driven cavity'}"
+                f" started at rest)",
+        "config": {"workload": f"{problem}, {'D3Q27 KBC' if nq == 27 else 'D3Q19 BGK'}, "
+                               f"{storage}, Re {re:g}, {steps} timed coarse steps after {warmup}",
                    "leaves": leaf, "active": info["active"], "blocks": info["blocks"],
                    "store_links": info["store_links"], "setup_s": round(t_setup, 2), "omega0": om0,
                    "device": nx.device_name()},
