@@ -325,9 +325,9 @@ int nx_d3q19_set_block_remap(int on);
  * 0 -- one voxel per lane.  Same results bit for bit.  No counterpart in
  * the reference. */
 int nx_d3q19_set_block_pairs(int on);
-/* The same step with the block neighbourhood staged in LDS (north-star
- * subsystem b; the reference reads neighbour blocks through
- * bPartition::getNghData, bPartition_imp.h:218-309): one workgroup per tile
+/* The same step with the block neighbourhood staged in LDS (the reference
+ * reads neighbour blocks through bPartition::getNghData,
+ * bPartition_imp.h:218-309): one workgroup per tile
  * of up to 2 x 2 x 1 blocks reads each member's population planes whole and
  * the tile's halo from the neighbour blocks once, and every voxel takes its
  * streamed values from LDS.  Bit for bit equal to nx_d3q19_step_block.
