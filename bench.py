@@ -663,8 +663,8 @@ def main():
                          "at 256^3 for 2/4/8-byte elements, the ceiling of the pull-streaming step")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
-              "(the multi-GPU z-slab decomposition is a design, not part of this build)",
+        print(f"bench.py: this benchmark runs on one GPU; --gpus {a.gpus} is not supported "
+              "(the multi-GPU z-slab decomposition is a design, not implemented)",
               file=sys.stderr)
         sys.exit(2)
 

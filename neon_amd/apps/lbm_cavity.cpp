@@ -18,7 +18,7 @@
 // files u_<iter>.vtk and rho_<iter>.vtk that the reference writes there
 // unconditionally (:105-106; its --vti flag, Config.cpp:90, is only
 // recorded).  float-fast, float-dev, half-dev, --Re and --ulb are additions;
-// eGrid and --deviceType cpu are not part of this build and are rejected.
+// eGrid and --deviceType cpu are not supported and are rejected.
 #include "neon_amd.hpp"
 
 #include <algorithm>
@@ -144,9 +144,9 @@ Config parse(int argc, char** argv)
         }
     }
     if (!haveType || !haveIds) usage("--deviceType and --deviceIds are required");
-    if (c.deviceType != "gpu") usage("only --deviceType gpu is part of this build");
+    if (c.deviceType != "gpu") usage("only --deviceType gpu is supported by this program");
     if (c.devices.size() != 1) usage("one process drives one GPU: pass exactly one device id");
-    if (c.gridType != "dGrid" && c.gridType != "bGrid") usage("--grid must be dGrid or bGrid (eGrid is not part of this build)");
+    if (c.gridType != "dGrid" && c.gridType != "bGrid") usage("--grid must be dGrid or bGrid (eGrid is not supported by this program)");
     if (c.N < 3 || c.benchMaxIter < 0 || c.benchIniIter < 0 || c.repetitions < 1) usage("bad sizes");
     c.setLbmParameters();
     return c;

@@ -23,9 +23,9 @@
 // (lbmMultiRes.cu:3-4, KBC by default); here --collision chooses.  The store
 // and stream options select kernel fusions that reorder the same arithmetic
 // (with the command line's choices the store is always the fine one): they
-// are recorded in the report, and this build runs its own fused schedule for
+// are recorded in the report, and this program runs its own fused schedule for
 // all of them.  --problemType sphere runs the flow over a sphere
-// (flowOverShape.h:115-190).  Not part of this build: --deviceType cpu, the
+// (flowOverShape.h:115-190).  Not supported here: --deviceType cpu, the
 // mesh problem, --vtk / --binary / --gui output.
 #include "neon_amd.hpp"
 
@@ -118,12 +118,12 @@ Params parse(int argc, char** argv)
         else if (a == "--streamFuseAll") p.streamFuseAll = true;
         else if (a == "--collision") p.collision = need(i);
         else if (a == "--vtk" || a == "--binary" || a == "--gui" || a == "--meshFile")
-            usage(a + " output / input is not part of this build");
+            usage(a + " output / input is not supported by this program");
         else usage("unknown option " + a);
     }
-    if (p.deviceType != "gpu") usage("only --deviceType gpu is part of this build");
+    if (p.deviceType != "gpu") usage("only --deviceType gpu is supported by this program");
     if (p.problemType != "lid" && p.problemType != "sphere")
-        usage("--problemType lid or sphere (the mesh problem is not part of this build)");
+        usage("--problemType lid or sphere (the mesh problem is not supported by this program)");
     if (p.dataType != "float" && p.dataType != "double") usage("--dataType must be float or double");
     if (p.collision != "kbc" && p.collision != "bgk") usage("--collision must be kbc or bgk");
     if (p.numIter < 0 || p.Re <= 0 || p.freq <= 0) usage("bad numIter, re or freq");
