@@ -200,6 +200,36 @@ def velocity_error(nx, N, storage, compute, ulb, omega, steps=ERROR_HORIZON):
             "reference": "binary64 GPU run of the same case (bit-exact with the CPU oracle)"}
 
 
+def batch_velocity_error(nx, members, storage, steps=ERROR_HORIZON, every=8):
+    """Velocity of the batch in `storage` against a binary64 batch of the same
+    members, both `steps` steps from rest, on every `every`-th member: the
+    largest max |u - u64| / max |u64| of a member and the relative L2 norm
+    over the sampled members' cells."""
+    out = {}
+    for st in (storage, "f64"):
+        b = nx.CavityBatch(members, st)
+        b.init()
+        b.step(steps)
+        out[st] = [b.moments(k)[1].astype(np.float64) for k in range(0, len(members), every)]
+        del b
+    per, d2, n2, skipped = [], 0.0, 0.0, 0
+    for u, u64 in zip(out[storage], out["f64"]):
+        if not (np.isfinite(u).all() and np.isfinite(u64).all()) or max(np.abs(u).max(), np.abs(u64).max()) > 1.0:
+            skipped += 1           # diverging: BGK unstable at this member's Re (tau near 1/2)
+            continue
+        d = u - u64
+        per.append(float(np.abs(d).max() / max(np.abs(u64).max(), 1e-300)))
+        d2 += float((d * d).sum())
+        n2 += float((u64 * u64).sum())
+    return {"steps": steps, "members_sampled": len(out["f64"]), "members_diverging": skipped,
+            "max_rel": max(per), "median_member_max_rel": float(np.median(per)),
+            "l2_rel": float(np.sqrt(d2 / n2)),
+            "norms": "per-member max |u - u64| / max |u64| (largest, median); ||u - u64||_2 / ||u64||_2 over "
+                     "the sampled members' cells; members diverging in either run (not finite, or |u| > 1 "
+                     "lattice units: 20 times the lid speed) left out",
+            "reference": "binary64 batch of the same members (bit-exact with the CPU oracle)"}
+
+
 def compute_ref_rate(nx, N, storage, ulb, omega, steps, stream):
     """MLUPS of the same container step with the reference's float-path
     collision (NX_COMPUTE_REF: LbmTools.h:209-281 with its double literals)."""
@@ -459,6 +489,9 @@ def batched(nx, a):
     mlups = fluid * steps / (ms * 1e-3) / 1e6
     b_alg = 2 * 19 * STORAGE_BYTES[storage]
     achieved = fluid * b_alg / (ms_step * 1e-3) / 1e9
+    err = None
+    if not a.no_error:
+        err = batch_velocity_error(nx, members, storage)
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(64, storage, "f32", 1.0 / (3.0 * ulb * 64 / 500.0 + 0.5), max_steps=4000)
@@ -479,7 +512,7 @@ def batched(nx, a):
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(f"d3q19_batch_{storage}_{n}"),
                      "kernel": f"k_batch_step<{storage}> ({b_alg} B/fluid cell)"},
-        "velocity_error_vs_f64": None,
+        "velocity_error_vs_f64": err,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
