@@ -476,9 +476,12 @@ class BlockLattice(_Lattice):
         return int(n.value)
 
     def __del__(self):
-        if getattr(self, "_tiles", None) is not None:
-            lib.nx_block_tiles_destroy(self._tiles)
-            self._tiles = None
+        try:
+            if getattr(self, "_tiles", None) is not None:
+                lib.nx_block_tiles_destroy(self._tiles)
+                self._tiles = None
+        except Exception:
+            pass
 
     def setup_cavity(self, ulb: float, ulid=(1.0, 0.0, 0.0), stream=None) -> None:
         check(lib.nx_cavity_flags_block(C.byref(self.desc), self.flags.ptr, stream))
