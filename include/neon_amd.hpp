@@ -477,8 +477,11 @@ class bGrid
         std::vector<int32_t>  origins;
         std::vector<uint32_t> masks, conn;
         void *                dOrigins = nullptr, *dMasks = nullptr, *dConn = nullptr;
+        // set by setStagedNeighbourhood(true); containers hold their own reference
+        std::shared_ptr<nx_block_tiles_s> tiles;
         ~State()
         {
+            tiles.reset();
             nx_free(dOrigins);
             nx_free(dMasks);
             nx_free(dConn);
@@ -553,6 +556,22 @@ class bGrid
                 f(index_3d(o.x + (v & 7), o.y + ((v >> 3) & 7), o.z + (v >> 6)), size_t(b) * kBlockVoxels + v);
             }
     }
+    // LbmContainers::iteration on this grid steps with the block
+    // neighbourhood staged in LDS (nx_d3q19_step_block_tiled, 2- and 4-byte
+    // storage; the same populations bit for bit), or, off, one workgroup
+    // per block (the default).  Containers made afterwards follow it.
+    void setStagedNeighbourhood(bool on)
+    {
+        if (on && !s_->tiles) {
+            const nx_block_desc d = desc(NX_F32, NX_COMPUTE_F32);
+            nx_block_tiles      t = nullptr;
+            check(nx_block_tiles_create(&d, &t), "bGrid::setStagedNeighbourhood");
+            s_->tiles = std::shared_ptr<nx_block_tiles_s>(t, [](nx_block_tiles_s* p) { nx_block_tiles_destroy(p); });
+        } else if (!on) {
+            s_->tiles.reset();
+        }
+    }
+    std::shared_ptr<nx_block_tiles_s> stagedTiles() const { return s_->tiles; }
     nx_block_desc desc(int storage, int compute) const
     {
         nx_block_desc d{};
@@ -819,9 +838,13 @@ struct LbmContainers
                              },
                              lazy);
         } else {
-            const nx_block_desc d = g.desc(kStorage, kCompute);
-            return Container("LbmIteration", g.getBackend(), [d, fin, fout, flags, p](nx_stream s) {
-                check(nx_d3q19_step_block(&d, fin, fout, flags, &p, s), "LbmIteration");
+            const nx_block_desc                     d = g.desc(kStorage, kCompute);
+            const std::shared_ptr<nx_block_tiles_s> t = g.stagedTiles();
+            return Container("LbmIteration", g.getBackend(), [d, t, fin, fout, flags, p](nx_stream s) {
+                if (t)
+                    check(nx_d3q19_step_block_tiled(&d, t.get(), fin, fout, flags, &p, s), "LbmIteration");
+                else
+                    check(nx_d3q19_step_block(&d, fin, fout, flags, &p, s), "LbmIteration");
             });
         }
     }

@@ -10,6 +10,7 @@
 //              [--storageFP double|float|float-dev|half-dev]
 //              [--sOCC|--nOCC] [--put|--get] [--huLattice|--huGrid]
 //              [--benchmark|--visual] [--vti] [--Re R] [--ulb U]
+//              [--stagedNeighbourhood]
 //
 // As in the reference: nu = ulb (N-2) / Re, omega = 1 / (3 nu + 1/2)
 // (Config.cpp:105-111); MLUPS = N^3 iterations / microseconds over the timed
@@ -17,7 +18,9 @@
 // iterations (RunCavityTwoPop.cu:86-151), and with --vti also the legacy VTK
 // files u_<iter>.vtk and rho_<iter>.vtk that the reference writes there
 // unconditionally (:105-106; its --vti flag, Config.cpp:90, is only
-// recorded).  float-fast, float-dev, half-dev, --Re and --ulb are additions;
+// recorded).  float-fast, float-dev, half-dev, --Re, --ulb and
+// --stagedNeighbourhood (bGrid: the step with the block neighbourhood staged
+// in LDS, bGrid::setStagedNeighbourhood) are additions;
 // eGrid and --deviceType cpu are not supported and are rejected.
 #include "neon_amd.hpp"
 
@@ -48,6 +51,7 @@ struct Config
     std::vector<int> devices;
     std::string      reportFile = "lbm-lid-driven-cavity-flow";
     std::string      gridType = "dGrid";
+    bool             stagedNeighbourhood = false;  // bGrid: the LDS-staged step
     std::string      occ = "none", transferMode = "get", stencilSemantic = "streaming";
     bool             vti = false;
     std::string      computeType = "double";
@@ -70,7 +74,7 @@ struct Config
                  "       [--warmup-iter W] [--max-iter M] [--repetitions R] [--report-filename F]\n"
                  "       [--computeFP double|float|float-fast] [--storageFP double|float|float-dev|half-dev]\n"
                  "       [--sOCC|--nOCC] [--put|--get] [--huLattice|--huGrid] [--benchmark|--visual]\n"
-                 "       [--Re R] [--ulb U]\n";
+                 "       [--Re R] [--ulb U] [--stagedNeighbourhood]\n";
     std::exit(2);
 }
 
@@ -103,6 +107,8 @@ Config parse(int argc, char** argv)
             if (c.devices.empty()) usage("--deviceIds needs at least one id");
         } else if (a == "--grid") {
             c.gridType = need(i);
+        } else if (a == "--stagedNeighbourhood") {
+            c.stagedNeighbourhood = true;
         } else if (a == "--domain-size") {
             c.N = int(num(a, need(i)));
         } else if (a == "--warmup-iter") {
@@ -147,6 +153,8 @@ Config parse(int argc, char** argv)
     if (c.deviceType != "gpu") usage("only --deviceType gpu is supported by this program");
     if (c.devices.size() != 1) usage("one process drives one GPU: pass exactly one device id");
     if (c.gridType != "dGrid" && c.gridType != "bGrid") usage("--grid must be dGrid or bGrid (eGrid is not supported by this program)");
+    if (c.stagedNeighbourhood && (c.gridType != "bGrid" || c.storeType == "double"))
+        usage("--stagedNeighbourhood needs --grid bGrid and 2- or 4-byte storage");
     if (c.N < 3 || c.benchMaxIter < 0 || c.benchIniIter < 0 || c.repetitions < 1) usage("bad sizes");
     c.setLbmParameters();
     return c;
@@ -205,6 +213,9 @@ void run(const Config& config, Report& report)
     bk.syncAll();
     auto  start = Clock::now();
     Grid  grid(bk, dim, [](const index_3d&) { return true; });
+    if constexpr (std::is_same_v<Grid, bGrid>) {
+        if (config.stagedNeighbourhood) grid.setStagedNeighbourhood(true);
+    }
     PopulationField pop0 = grid.template newField<StorageFP, 19>("Population", 19, StorageFP{});
     PopulationField pop1 = grid.template newField<StorageFP, 19>("Population", 19, StorageFP{});
     CellTypeField   flag = grid.template newField<CellType, 1>("Material", 1, CellType{});
@@ -378,6 +389,7 @@ int main(int argc, char** argv)
       << "  \"repetitions\": " << config.repetitions << ",\n"
       << "  \"deviceType\": \"" << config.deviceType << "\",\n  \"devices\": [" << config.devices[0] << "],\n"
       << "  \"gridType\": \"" << config.gridType << "\",\n"
+      << "  \"stagedNeighbourhood\": " << (config.stagedNeighbourhood ? "true" : "false") << ",\n"
       << "  \"computeType\": \"" << config.computeType << "\",\n  \"storeType\": \"" << config.storeType << "\",\n"
       << "  \"occ\": \"" << config.occ << "\",\n  \"transferMode\": \"" << config.transferMode << "\",\n"
       << "  \"transferSemantic\": \"" << config.stencilSemantic << "\",\n"

@@ -20,7 +20,8 @@ def run_app(tmp, *args):
 
 
 def test_cli_rejects_what_is_not_built(tmp_path):
-    for extra in (["--grid", "eGrid"], ["--deviceType", "cpu"], ["--storageFP", "quad"]):
+    for extra in (["--grid", "eGrid"], ["--deviceType", "cpu"], ["--storageFP", "quad"],
+                  ["--stagedNeighbourhood"], ["--grid", "bGrid", "--stagedNeighbourhood", "--storageFP", "double"]):
         r = subprocess.run([str(EXE), "--deviceType", "gpu", "--deviceIds", "0", *extra],
                            cwd=tmp_path, capture_output=True, text=True, timeout=60)
         assert r.returncode == 2 and "usage" in r.stderr
@@ -96,3 +97,20 @@ def test_app_vtk_export_matches_oracle(orc, tmp_path, grid):
     # the centre line written beside it is the same field
     y = np.loadtxt(tmp_path / "NeonUniformLBM_00100_Y.dat")
     np.testing.assert_allclose(y[:, 1], u[N // 2, :, N // 2, 0], rtol=2e-5, atol=1e-10)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("storage,compute", [("float", "float"), ("half-dev", "float")])
+def test_app_staged_neighbourhood_writes_the_same_files(tmp_path, storage, compute):
+    """lbm_cavity --grid bGrid with and without --stagedNeighbourhood (the
+    LDS-staged step through bGrid::setStagedNeighbourhood): the centre-line
+    files are identical byte for byte."""
+    out = {}
+    for staged in (False, True):
+        d = tmp_path / ("staged" if staged else "plain")
+        d.mkdir()
+        extra = ["--stagedNeighbourhood"] if staged else []
+        run_app(d, "--grid", "bGrid", "--domain-size", 40, "--storageFP", storage, "--computeFP", compute,
+                "--visual", "--max-iter", 101, "--Re", 100.0, "--ulb", 0.04, *extra)
+        out[staged] = [(d / f"NeonUniformLBM_00100_{a}.dat").read_bytes() for a in ("X", "Y")]
+    assert out[False] == out[True]
