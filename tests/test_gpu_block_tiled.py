@@ -71,6 +71,34 @@ def test_tiled_porous_periodic_forced_bit_exact(nx, orc, st, sto):
     assert np.array_equal(p3.transpose(1, 0, 2)[:, k3], po.transpose(1, 0, 2)[:, k3])
 
 
+@pytest.mark.parametrize("dim", [(24, 40, 24), (8, 8, 16), (8, 24, 8)])
+@pytest.mark.parametrize("st,sto", [("f32", 1), ("f16dev", 3)])
+def test_tiled_periodic_every_axis_bit_exact(nx, orc, dim, st, sto):
+    """Periodic in x, y and z on odd block counts and one-block-wide domains
+    (a block its own neighbour): tiles whose absent member's extent holds
+    wrapped upstream voxels, read through the connectivity rows."""
+    print(f"tiled periodic xyz {dim} {st}", flush=True)
+    n = dim[0] * dim[1] * dim[2]
+    solid = spheres(dim, 3, 2, 3, seed=11)
+    act = (~solid).astype(np.uint8).reshape(-1) if n > 4096 else np.ones(n, np.uint8)
+    grid = nx.BlockGrid(dim, act, periodic=7)
+    g = nx.BlockLattice(grid, st, "f32")
+    ref = nx.BlockLattice(grid, st, "f32")
+    o = orc.BlockOracle(dim, act, sto, 1, periodic=7)
+    g.set_tiled(True)
+    keep = o.active_slots()
+    cls = np.where(keep, 2, 3).astype(np.uint32)
+    for lat in (g, ref, o):
+        lat.set_classes(cls)
+    for lat in (g, ref, o):
+        lat.run(9, 1.6, force=(2e-5, 1e-5, 0.0))
+    pg = _raw(g.populations(), g.storage)
+    assert np.array_equal(pg, _raw(ref.populations(), ref.storage))
+    k3 = keep.reshape(-1, 512)
+    p3, po = pg.reshape(-1, 19, 512), _raw(o.fin, sto).reshape(-1, 19, 512)
+    assert np.array_equal(p3.transpose(1, 0, 2)[:, k3], po.transpose(1, 0, 2)[:, k3])
+
+
 @pytest.mark.parametrize("st", ["f16dev", "f32"])
 def test_tiled_equals_per_block_step_on_the_benchmark_grid(nx, st):
     """384^3, 2000 spheres r~U[6,20) (seed 2024), periodic x, g = 1e-6: the
