@@ -69,6 +69,27 @@ def test_pairs_porous_periodic_forced_bit_exact(nx, orc, pairs):
     assert np.array_equal(p3.transpose(1, 0, 2)[:, k3], po.transpose(1, 0, 2)[:, k3])
 
 
+@pytest.mark.parametrize("dim", [(24, 40, 24), (8, 8, 16), (8, 24, 8)])
+def test_pairs_periodic_every_axis_bit_exact(nx, orc, pairs, dim):
+    """Periodic in x, y and z, odd block counts and one-block-wide domains
+    (the pair at a block's x face reading its own block across the wrap)."""
+    print(f"pairs periodic xyz {dim}", flush=True)
+    n = dim[0] * dim[1] * dim[2]
+    solid = spheres(dim, 3, 2, 3, seed=11)
+    act = (~solid).astype(np.uint8).reshape(-1) if n > 4096 else np.ones(n, np.uint8)
+    grid = nx.BlockGrid(dim, act, periodic=7)
+    keep = np.unpackbits(grid.masks.view(np.uint8), bitorder="little").astype(bool)
+    cls = np.where(keep, 2, 3).astype(np.uint32)
+    one, two = _run_both(nx, grid, lambda l: l.set_classes(cls), 9, 1.6, (2e-5, 1e-5, 0.0), pairs)
+    assert np.array_equal(one, two)
+    o = orc.BlockOracle(dim, act, 3, 1, periodic=7)
+    o.set_classes(cls)
+    o.run(9, 1.6, force=(2e-5, 1e-5, 0.0))
+    k3 = keep.reshape(-1, 512)
+    p3, po = two.reshape(-1, 19, 512), _raw(o.fin, 3).reshape(-1, 19, 512)
+    assert np.array_equal(p3.transpose(1, 0, 2)[:, k3], po.transpose(1, 0, 2)[:, k3])
+
+
 def test_pairs_equal_single_on_the_benchmark_grid(nx, pairs):
     from neon_amd import abi
     dim = (384, 384, 384)
