@@ -25,8 +25,9 @@
 // (with the command line's choices the store is always the fine one): they
 // are recorded in the report, and this program runs its own fused schedule for
 // all of them.  --problemType sphere runs the flow over a sphere
-// (flowOverShape.h:115-190).  Not supported here: --deviceType cpu, the
-// mesh problem, --vtk / --binary / --gui output.
+// (flowOverShape.h:115-190); --visual --vtk writes postProcess's VTK files
+// (postProcess.h:94-151).  Not supported here: --deviceType cpu, the mesh
+// problem, --binary / --gui output.
 #include "neon_amd.hpp"
 
 #include <algorithm>
@@ -61,6 +62,8 @@ struct Params
     bool        collisionFusedStore = true;
     bool        fusedFinest = true;
     int         scale = 2;
+    bool        vtk = false;
+    int         slice[3] = {-1, -1, 1};  // --sliceX/Y/Z: 1 draws the leaves on that mid-plane
     std::string dataType = "float";
     std::string collision = "kbc";
     std::string commandLine;
@@ -71,6 +74,7 @@ struct Params
     std::cerr << "lbm_multires: " << why << "\n"
               << "usage: lbm_multires [--deviceType gpu] [--deviceId 0] [--numIter N] [--problemType lid|sphere]\n"
                  "       [--dataType float|double] [--re R] [--scale 0..11] [--benchmark|--visual] [--freq F]\n"
+                 "       [--vtk] [--sliceX S] [--sliceY S] [--sliceZ S]\n"
                  "       [--storeFine|--storeCoarse|--collisionFusedStore] [--fusedFinest]\n"
                  "       [--streamFusedExpl|--streamFusedCoal|--streamFuseAll] [--collision kbc|bgk]\n";
     std::exit(2);
@@ -105,7 +109,10 @@ Params parse(int argc, char** argv)
         else if (a == "--re") p.Re = integer(a, need(i));
         else if (a == "--scale") p.scale = integer(a, need(i));
         else if (a == "--freq") p.freq = integer(a, need(i));
-        else if (a == "--sliceX" || a == "--sliceY" || a == "--sliceZ") integer(a, need(i));
+        else if (a == "--sliceX") p.slice[0] = integer(a, need(i));
+        else if (a == "--sliceY") p.slice[1] = integer(a, need(i));
+        else if (a == "--sliceZ") p.slice[2] = integer(a, need(i));
+        else if (a == "--vtk") p.vtk = true;
         else if (a == "--thetaX" || a == "--thetaY" || a == "--thetaZ") need(i);
         else if (a == "--benchmark") p.benchmark = true;
         else if (a == "--visual") p.benchmark = false;
@@ -117,7 +124,7 @@ Params parse(int argc, char** argv)
         else if (a == "--streamFusedCoal") p.streamFusedCoal = true;
         else if (a == "--streamFuseAll") p.streamFuseAll = true;
         else if (a == "--collision") p.collision = need(i);
-        else if (a == "--vtk" || a == "--binary" || a == "--gui" || a == "--meshFile")
+        else if (a == "--binary" || a == "--gui" || a == "--meshFile")
             usage(a + " output / input is not supported by this program");
         else usage("unknown option " + a);
     }
@@ -218,6 +225,105 @@ std::string dateString()
     return oss.str();
 }
 
+// velocity of a leaf as postProcess computes it (postProcess.h:15-90): bulk
+// -> sum c f / rho, moving wall or inlet -> f_q / (6 w_q) along the axis of
+// the population with c = -e_a, other cells 0
+template <typename T>
+void leafVelocity(const Lattice& L, int Q, int cls, const T* f, size_t n, size_t i, T u[3])
+{
+    u[0] = u[1] = u[2] = T(0);
+    if (cls == NX_CELL_BULK) {
+        T r = 0;
+        for (int q = 0; q < Q; q++) r += f[size_t(q) * n + i];
+        for (int q = 0; q < Q; q++)
+            for (int a = 0; a < 3; a++) u[a] += f[size_t(q) * n + i] * T(L.c[q][a]);
+        for (int a = 0; a < 3; a++) u[a] /= r;
+    } else if (cls == NX_CELL_MOVINGWALL) {
+        for (int q = 0; q < Q; q++)
+            for (int a = 0; a < 3; a++)
+                if (L.c[q][a] == -1 && L.c[q][(a + 1) % 3] == 0 && L.c[q][(a + 2) % 3] == 0)
+                    u[a] = T(double(f[size_t(q) * n + i]) / (6. * L.w[q]));
+    }
+}
+
+// The --vtk file of postProcess (postProcess.h:94-151) over the leaves that
+// initVisualization draws (:232-331): every leaf of every level, or with a
+// slice set only those whose voxel reaches the domain's mid-plane on that
+// axis; each a VTK voxel of side 2^level at its level-0 origin, with its
+// velocity and level.
+template <typename T>
+void writeVtk(const mGrid& grid, const Lattice& L, const int slice[3], const std::string& fileName)
+{
+    struct Leaf
+    {
+        int      o[3];
+        int      spacing;
+        T        u[3];
+        int      level;
+    };
+    constexpr double tiny = 1e-7;
+    const index_3d   dim0 = grid.levelDimension(0);
+    const int        Q = grid.populations();
+    const bool       sliced = slice[0] == 1 || slice[1] == 1 || slice[2] == 1;
+    std::vector<Leaf> leaves;
+    for (int level = 0; level < grid.getDepth(); level++) {
+        const index_3d d = grid.levelDimension(level);
+        const double   vs[3] = {1.0 / d.x, 1.0 / d.y, 1.0 / d.z};
+        const auto     st = grid.state(level);
+        const auto     f = grid.template populations<T>(level, true);
+        const size_t   n = d.rMulTyped();
+        for (int z = 0; z < d.z; z++)
+            for (int y = 0; y < d.y; y++)
+                for (int x = 0; x < d.x; x++) {
+                    const size_t  i = size_t(x) + size_t(d.x) * (size_t(y) + size_t(d.y) * size_t(z));
+                    const uint8_t s = st[i];
+                    if (!(s & 1u) || (s & 2u)) continue;  // leaves only
+                    const int o[3] = {x << level, y << level, z << level};
+                    bool      draw = !sliced;
+                    const int d0[3] = {dim0.x, dim0.y, dim0.z};
+                    for (int a = 0; a < 3 && !draw; a++) {
+                        const double loc = double(o[a]) / double(d0[a]);
+                        if (slice[a] == 1 && loc - tiny <= 0.5 && loc + vs[a] >= 0.5 - tiny) draw = true;
+                    }
+                    if (!draw) continue;
+                    Leaf lf{{o[0], o[1], o[2]}, 1 << level, {}, level};
+                    leafVelocity<T>(L, Q, (s >> 2) & 3, f.data(), n, i, lf.u);
+                    leaves.push_back(lf);
+                }
+    }
+    std::ofstream file(fileName + ".vtk");
+    file << "# vtk DataFile Version 2.0\n"
+         << "mGrid\n"
+         << "ASCII\n"
+         << "DATASET UNSTRUCTURED_GRID\n"
+         << "POINTS " << leaves.size() * 8 << " float \n";
+    // corner order per leaf: (0,0,0) (1,0,0) (1,0,1) (0,0,1) (0,1,0) (1,1,0) (1,1,1) (0,1,1)
+    static const int corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 0, 1}, {0, 0, 1},
+                                     {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {0, 1, 1}};
+    for (const Leaf& lf : leaves)
+        for (const auto& c : corner)
+            file << float(lf.o[0] + c[0] * lf.spacing) << " " << float(lf.o[1] + c[1] * lf.spacing) << " "
+                 << float(lf.o[2] + c[2] * lf.spacing) << "\n";
+    file << "CELLS " << leaves.size() << " " << leaves.size() * 9 << " \n";
+    // the reference's point order for its VTK voxels (cell type 11): x
+    // fastest, then z, then y (VTK's voxel order with y and z exchanged;
+    // the same cube)
+    static const int voxelOrder[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+    for (size_t k = 0; k < leaves.size(); k++) {
+        file << "8 ";
+        for (int j : voxelOrder) file << k * 8 + size_t(j) << " ";
+        file << "\n";
+    }
+    file << "CELL_TYPES " << leaves.size() << " \n";
+    for (size_t k = 0; k < leaves.size(); k++) file << 11 << "\n";
+    file << "CELL_DATA " << leaves.size() << " \n";
+    file << "VECTORS Velocity float \n";
+    for (const Leaf& lf : leaves) file << lf.u[0] << " " << lf.u[1] << " " << lf.u[2] << " \n";
+    file << "SCALARS Level float 1 \n";
+    file << "LOOKUP_TABLE default \n";
+    for (const Leaf& lf : leaves) file << lf.level << "\n";
+}
+
 template <typename T>
 void writeCentreLines(const mGrid& grid, const Lattice& L, T ulb, int iteration)
 {
@@ -242,20 +348,8 @@ void writeCentreLines(const mGrid& grid, const Lattice& L, T ulb, int iteration)
                     const size_t  i = size_t(x) + size_t(d.x) * (size_t(y) + size_t(d.y) * size_t(z));
                     const uint8_t s = st[i];
                     if (!(s & 1u) || (s & 2u)) continue;  // leaves only
-                    const int cls = (s >> 2) & 3;
-                    T         u[3] = {0, 0, 0};
-                    if (cls == NX_CELL_BULK) {
-                        T r = 0;
-                        for (int q = 0; q < Q; q++) r += f[size_t(q) * n + i];
-                        for (int q = 0; q < Q; q++)
-                            for (int a = 0; a < 3; a++) u[a] += f[size_t(q) * n + i] * T(L.c[q][a]);
-                        for (int a = 0; a < 3; a++) u[a] /= r;
-                    } else if (cls == NX_CELL_MOVINGWALL) {
-                        for (int q = 0; q < Q; q++)
-                            for (int a = 0; a < 3; a++)
-                                if (L.c[q][a] == -1 && L.c[q][(a + 1) % 3] == 0 && L.c[q][(a + 2) % 3] == 0)
-                                    u[a] = T(double(f[size_t(q) * n + i]) / (6. * L.w[q]));
-                    }
+                    T u[3];
+                    leafVelocity<T>(L, Q, (s >> 2) & 3, f.data(), n, i, u);
                     if (onY) yPosVal.push_back({T(double(id.y) / double(dim.y)), u[0] * scale});
                     if (onX) xPosVal.push_back({T(double(id.x) / double(dim.x)), u[1] * scale});
                 }
@@ -296,9 +390,28 @@ int runNonUniformLBM(const Backend& backend, const Params& params, const Problem
     const T        omega = 1.0 / (3. * visclb + 0.5);
 
     const std::vector<uint64_t> leaves = g.numLeafVoxels();
+    const bool                  vtk = !params.benchmark && params.vtk;
+    auto fileName = [](int t) {
+        std::ostringstream suffix;
+        suffix << "Velocity_" << std::setw(4) << std::setfill('0') << t;
+        return suffix.str();
+    };
     backend.syncAll();
     const auto start = std::chrono::high_resolution_clock::now();
-    g.run(double(omega), params.numIter);
+    if (vtk) {
+        // the loop of lbmMultiRes.h:361-376: after iteration t (t + 1 steps)
+        // with t % freq == 0 and t != 0, the file of iteration t
+        int done = 0;
+        for (int t = params.freq; t < params.numIter; t += params.freq) {
+            g.run(double(omega), t + 1 - done);
+            done = t + 1;
+            backend.syncAll();
+            writeVtk<T>(g, L, params.slice, fileName(t));
+        }
+        g.run(double(omega), params.numIter - done);
+    } else {
+        g.run(double(omega), params.numIter);
+    }
     backend.syncAll();
     const auto   stop = std::chrono::high_resolution_clock::now();
     const double us = double(std::chrono::duration_cast<std::chrono::microseconds>(stop - start).count());
@@ -350,6 +463,7 @@ int runNonUniformLBM(const Backend& backend, const Params& params, const Problem
     r << "],\n    \"EMLUPS\": " << effMLUPS << ",\n    \"ENumIter\": " << effNumIter << ",\n"
       << "    \"ENumVoxels\": " << gridDim.rMulTyped() << "\n}\n";
 
+    if (vtk) writeVtk<T>(g, L, params.slice, fileName(params.numIter));  // lbmMultiRes.h:490-499
     if (pb.verify) writeCentreLines<T>(g, L, ulb, params.numIter);
     return 0;
 }

@@ -21,7 +21,7 @@ GOLDEN = ROOT / "tests" / "golden"
 
 def test_cli_rejects_what_is_not_built(tmp_path):
     for extra in (["--deviceType", "cpu"], ["--problemType", "mesh"], ["--dataType", "half"],
-                  ["--vtk"], ["--collision", "mrt"], ["--numIter", "x"]):
+                  ["--binary"], ["--collision", "mrt"], ["--numIter", "x"]):
         r = subprocess.run([str(EXE), *extra], cwd=tmp_path, capture_output=True, text=True, timeout=60)
         assert r.returncode == 2 and "usage" in r.stderr, extra
 
@@ -76,3 +76,57 @@ def test_app_runs_the_flow_over_a_sphere(tmp_path):
     o = MresOracle(dim, lams)
     leaves = [int((o.active(l) & ~o.refined(l)).sum()) if l else int(o.active(0).sum()) for l in range(3)]
     assert rep["NumActiveVoxels"] == leaves
+
+
+def _read_vtk(path):
+    """(points, cells, cell types, velocity, level) of the unstructured-grid
+    file postProcess writes."""
+    lines = path.read_text().split("\n")
+    assert lines[:4] == ["# vtk DataFile Version 2.0", "mGrid", "ASCII", "DATASET UNSTRUCTURED_GRID"]
+    k = 4
+    npts = int(lines[k].split()[1]); k += 1
+    pts = np.array([[float(v) for v in lines[k + i].split()] for i in range(npts)]); k += npts
+    ncell = int(lines[k].split()[1]); assert int(lines[k].split()[2]) == 9 * ncell; k += 1
+    cells = np.array([[int(v) for v in lines[k + i].split()] for i in range(ncell)]); k += ncell
+    assert lines[k].split()[:2] == ["CELL_TYPES", str(ncell)]; k += 1
+    types = np.array([int(lines[k + i]) for i in range(ncell)]); k += ncell
+    assert lines[k].split() == ["CELL_DATA", str(ncell)]; k += 1
+    assert lines[k].split() == ["VECTORS", "Velocity", "float"]; k += 1
+    vel = np.array([[float(v) for v in lines[k + i].split()] for i in range(ncell)]); k += ncell
+    assert lines[k].split() == ["SCALARS", "Level", "float", "1"] and lines[k + 1].split() == ["LOOKUP_TABLE", "default"]
+    k += 2
+    level = np.array([int(lines[k + i]) for i in range(ncell)])
+    return pts, cells, types, vel, level
+
+
+@pytest.mark.gpu
+def test_app_vtk_output(tmp_path):
+    """--visual --vtk --freq 50 --numIter 100 on the --scale 1 lid grid: the
+    files of iterations 50 and 100 (postProcess.h:94-151) hold every leaf on
+    the z mid-plane (the default --sliceZ 1) as a VTK voxel of side 2^level,
+    and the velocities of the leaves on the centre line x = z = 80 equal the
+    centre-line file of the same run."""
+    cmd = [str(EXE), "--numIter", "100", "--deviceType", "gpu", "--problemType", "lid", "--visual", "--vtk",
+           "--freq", "50", "--dataType", "float", "--re", "100", "--scale", "1"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=200)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert sorted(p.name for p in tmp_path.glob("Velocity_*.vtk")) == ["Velocity_0050.vtk", "Velocity_0100.vtk"]
+    pts, cells, types, vel, level = _read_vtk(tmp_path / "Velocity_0100.vtk")
+    assert (types == 11).all() and cells[:, 0].tolist() == [8] * len(cells)
+    assert set(level.tolist()) == {0, 1, 2}
+    corners = pts[cells[:, 1:]]                      # n x 8 x 3
+    lo, side = corners[:, 0, :], (2.0 ** level)[:, None]
+    # the reference's corner order (postProcess.h:111-126 over :283-321): x
+    # fastest, then z, then y
+    for j, (dx, dy, dz) in enumerate([(0, 0, 0), (1, 0, 0), (0, 0, 1), (1, 0, 1),
+                                      (0, 1, 0), (1, 1, 0), (0, 1, 1), (1, 1, 1)]):
+        assert np.array_equal(corners[:, j, :], lo + side * np.array([dx, dy, dz]))
+    # every drawn leaf reaches the mid-plane z = 80
+    assert ((lo[:, 2] <= 80) & (lo[:, 2] + side[:, 0] >= 80)).all()
+    # the centre line x = z = 80: the same velocities as the .dat file
+    dat = np.loadtxt(tmp_path / "NeonMultiResLBM_0100_Y.dat")
+    on = (lo[:, 0] == 80) & (lo[:, 2] == 80)
+    got = sorted(zip(lo[on, 1] / 160.0, vel[on, 0] / np.float32(0.04)))
+    assert len(got) == len(dat)
+    assert np.allclose(np.array(got), dat, rtol=1e-5, atol=1e-7)
