@@ -25,9 +25,9 @@
 // (with the command line's choices the store is always the fine one): they
 // are recorded in the report, and this program runs its own fused schedule for
 // all of them.  --problemType sphere runs the flow over a sphere
-// (flowOverShape.h:115-190); --visual --vtk writes postProcess's VTK files
-// (postProcess.h:94-151).  Not supported here: --deviceType cpu, the mesh
-// problem, --binary / --gui output.
+// (flowOverShape.h:115-190); --visual --vtk / --binary write postProcess's
+// files (postProcess.h:94-227).  Not supported here: --deviceType cpu, the
+// mesh problem, the --gui viewer.
 #include "neon_amd.hpp"
 
 #include <algorithm>
@@ -63,6 +63,7 @@ struct Params
     bool        fusedFinest = true;
     int         scale = 2;
     bool        vtk = false;
+    bool        binary = false;
     int         slice[3] = {-1, -1, 1};  // --sliceX/Y/Z: 1 draws the leaves on that mid-plane
     std::string dataType = "float";
     std::string collision = "kbc";
@@ -74,7 +75,7 @@ struct Params
     std::cerr << "lbm_multires: " << why << "\n"
               << "usage: lbm_multires [--deviceType gpu] [--deviceId 0] [--numIter N] [--problemType lid|sphere]\n"
                  "       [--dataType float|double] [--re R] [--scale 0..11] [--benchmark|--visual] [--freq F]\n"
-                 "       [--vtk] [--sliceX S] [--sliceY S] [--sliceZ S]\n"
+                 "       [--vtk] [--binary] [--sliceX S] [--sliceY S] [--sliceZ S]\n"
                  "       [--storeFine|--storeCoarse|--collisionFusedStore] [--fusedFinest]\n"
                  "       [--streamFusedExpl|--streamFusedCoal|--streamFuseAll] [--collision kbc|bgk]\n";
     std::exit(2);
@@ -113,6 +114,7 @@ Params parse(int argc, char** argv)
         else if (a == "--sliceY") p.slice[1] = integer(a, need(i));
         else if (a == "--sliceZ") p.slice[2] = integer(a, need(i));
         else if (a == "--vtk") p.vtk = true;
+        else if (a == "--binary") p.binary = true;
         else if (a == "--thetaX" || a == "--thetaY" || a == "--thetaZ") need(i);
         else if (a == "--benchmark") p.benchmark = true;
         else if (a == "--visual") p.benchmark = false;
@@ -124,7 +126,7 @@ Params parse(int argc, char** argv)
         else if (a == "--streamFusedCoal") p.streamFusedCoal = true;
         else if (a == "--streamFuseAll") p.streamFuseAll = true;
         else if (a == "--collision") p.collision = need(i);
-        else if (a == "--binary" || a == "--gui" || a == "--meshFile")
+        else if (a == "--gui" || a == "--meshFile")
             usage(a + " output / input is not supported by this program");
         else usage("unknown option " + a);
     }
@@ -324,6 +326,41 @@ void writeVtk(const mGrid& grid, const Lattice& L, const int slice[3], const std
     for (const Leaf& lf : leaves) file << lf.level << "\n";
 }
 
+// The --binary file of postProcess (postProcess.h:153-227): the velocity
+// sampled on the top level's lattice -- every leaf whose level-0 origin is a
+// multiple of 2^(depth-1), at origin / 2^(depth-1) -- as a 3-component voxel
+// field "V" of a binary legacy VTK file named "Bin" + fileName.
+template <typename T>
+void writeBinary(const mGrid& grid, const Lattice& L, const std::string& fileName)
+{
+    const int      top = grid.getDepth() - 1;
+    const index_3d dim0 = grid.levelDimension(0);
+    const index_3d box(dim0.x >> top, dim0.y >> top, dim0.z >> top);
+    const int      Q = grid.populations();
+    const int      mask = (1 << top) - 1;
+    std::vector<float> buf(box.rMulTyped() * 3, 0.f);
+    for (int level = 0; level < grid.getDepth(); level++) {
+        const index_3d d = grid.levelDimension(level);
+        const auto     st = grid.state(level);
+        const auto     f = grid.template populations<T>(level, true);
+        const size_t   n = d.rMulTyped();
+        for (int z = 0; z < d.z; z++)
+            for (int y = 0; y < d.y; y++)
+                for (int x = 0; x < d.x; x++) {
+                    const int ox = x << level, oy = y << level, oz = z << level;
+                    if ((ox & mask) || (oy & mask) || (oz & mask)) continue;
+                    const size_t  i = size_t(x) + size_t(d.x) * (size_t(y) + size_t(d.y) * size_t(z));
+                    const uint8_t s = st[i];
+                    if (!(s & 1u) || (s & 2u)) continue;  // leaves only
+                    T u[3];
+                    leafVelocity<T>(L, Q, (s >> 2) & 3, f.data(), n, i, u);
+                    const size_t at = size_t(ox >> top) + size_t(box.x) * (size_t(oy >> top) + size_t(box.y) * size_t(oz >> top));
+                    for (int c = 0; c < 3; c++) buf[at * 3 + size_t(c)] = float(u[c]);
+                }
+    }
+    vtk::writeFile("Bin" + fileName, "V", buf, box, 3, false, IoFileType::BINARY, false);
+}
+
 template <typename T>
 void writeCentreLines(const mGrid& grid, const Lattice& L, T ulb, int iteration)
 {
@@ -391,6 +428,7 @@ int runNonUniformLBM(const Backend& backend, const Params& params, const Problem
 
     const std::vector<uint64_t> leaves = g.numLeafVoxels();
     const bool                  vtk = !params.benchmark && params.vtk;
+    const bool                  bin = !params.benchmark && params.binary;
     auto fileName = [](int t) {
         std::ostringstream suffix;
         suffix << "Velocity_" << std::setw(4) << std::setfill('0') << t;
@@ -398,15 +436,16 @@ int runNonUniformLBM(const Backend& backend, const Params& params, const Problem
     };
     backend.syncAll();
     const auto start = std::chrono::high_resolution_clock::now();
-    if (vtk) {
+    if (vtk || bin) {
         // the loop of lbmMultiRes.h:361-376: after iteration t (t + 1 steps)
-        // with t % freq == 0 and t != 0, the file of iteration t
+        // with t % freq == 0 and t != 0, the files of iteration t
         int done = 0;
         for (int t = params.freq; t < params.numIter; t += params.freq) {
             g.run(double(omega), t + 1 - done);
             done = t + 1;
             backend.syncAll();
-            writeVtk<T>(g, L, params.slice, fileName(t));
+            if (vtk) writeVtk<T>(g, L, params.slice, fileName(t));
+            if (bin) writeBinary<T>(g, L, fileName(t));
         }
         g.run(double(omega), params.numIter - done);
     } else {
@@ -464,6 +503,7 @@ int runNonUniformLBM(const Backend& backend, const Params& params, const Problem
       << "    \"ENumVoxels\": " << gridDim.rMulTyped() << "\n}\n";
 
     if (vtk) writeVtk<T>(g, L, params.slice, fileName(params.numIter));  // lbmMultiRes.h:490-499
+    if (bin) writeBinary<T>(g, L, fileName(params.numIter));
     if (pb.verify) writeCentreLines<T>(g, L, ulb, params.numIter);
     return 0;
 }

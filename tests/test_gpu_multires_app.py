@@ -21,7 +21,7 @@ GOLDEN = ROOT / "tests" / "golden"
 
 def test_cli_rejects_what_is_not_built(tmp_path):
     for extra in (["--deviceType", "cpu"], ["--problemType", "mesh"], ["--dataType", "half"],
-                  ["--binary"], ["--collision", "mrt"], ["--numIter", "x"]):
+                  ["--gui"], ["--collision", "mrt"], ["--numIter", "x"]):
         r = subprocess.run([str(EXE), *extra], cwd=tmp_path, capture_output=True, text=True, timeout=60)
         assert r.returncode == 2 and "usage" in r.stderr, extra
 
@@ -130,3 +130,34 @@ def test_app_vtk_output(tmp_path):
     got = sorted(zip(lo[on, 1] / 160.0, vel[on, 0] / np.float32(0.04)))
     assert len(got) == len(dat)
     assert np.allclose(np.array(got), dat, rtol=1e-5, atol=1e-7)
+
+
+@pytest.mark.gpu
+def test_app_binary_output(tmp_path):
+    """--visual --binary (postProcess.h:153-227): the velocity sampled on the
+    top level's lattice (40^3 for the --scale 1 grid, every leaf whose origin
+    is a multiple of 4) in a binary legacy VTK file; along the centre line
+    x = z = 80 it equals the centre-line file of the same run."""
+    cmd = [str(EXE), "--numIter", "100", "--deviceType", "gpu", "--problemType", "lid", "--visual", "--binary",
+           "--freq", "50", "--dataType", "float", "--re", "100", "--scale", "1"]
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=200)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert sorted(p.name for p in tmp_path.glob("BinVelocity_*.vtk")) == ["BinVelocity_0050.vtk",
+                                                                          "BinVelocity_0100.vtk"]
+    raw = (tmp_path / "BinVelocity_0100.vtk").read_bytes()
+    head = raw[:raw.index(b"LOOKUP_TABLE default\n")].decode().split("\n")
+    assert head[:4] == ["# vtk DataFile Version 3.0", "Title Neon", "BINARY", "DATASET STRUCTURED_POINTS"]
+    assert head[4] == "DIMENSIONS 41 41 41" and head[7] == "CELL_DATA 64000"
+    assert head[8].split() == ["SCALARS", "V", "float", "3"]
+    start = raw.index(b"LOOKUP_TABLE default\n") + len(b"LOOKUP_TABLE default\n")
+    v = np.frombuffer(raw[start:start + 64000 * 12], dtype=">f4").reshape(40, 40, 40, 3)   # z, y, x, c
+    assert np.isfinite(v).all()
+    dat = np.loadtxt(tmp_path / "NeonMultiResLBM_0100_Y.dat")
+    checked = 0
+    for pos, val in dat:
+        y = int(round(pos * 160))
+        if y % 4 == 0:
+            assert abs(v[20, y // 4, 20, 0] / np.float32(0.04) - val) <= 1e-5 * abs(val) + 1e-7, (y, val)
+            checked += 1
+    assert checked >= 30
