@@ -130,6 +130,17 @@ def test_app_vtk_output(tmp_path):
     got = sorted(zip(lo[on, 1] / 160.0, vel[on, 0] / np.float32(0.04)))
     assert len(got) == len(dat)
     assert np.allclose(np.array(got), dat, rtol=1e-5, atol=1e-7)
+    # no slice: every leaf of every level, as many as the report counts
+    whole = tmp_path / "whole"
+    whole.mkdir()
+    r = subprocess.run([str(EXE), "--numIter", "1", "--deviceType", "gpu", "--problemType", "lid", "--visual",
+                        "--vtk", "--sliceZ", "-1", "--dataType", "float", "--re", "100", "--scale", "1"],
+                       cwd=whole, capture_output=True, text=True, timeout=200)
+    assert r.returncode == 0, r.stdout + r.stderr
+    _, cells1, _, vel1, level1 = _read_vtk(whole / "Velocity_0001.vtk")
+    rep = json.load(open(glob.glob(str(whole / "MultiResLBM_P1_*.json"))[0]))
+    assert len(cells1) == sum(rep["NumActiveVoxels"])
+    assert np.bincount(level1).tolist() == rep["NumActiveVoxels"]
 
 
 @pytest.mark.gpu
